@@ -1,0 +1,163 @@
+/*
+ * dugks.h -- C-ABI of libdugks_hip.so, the MI355X implementation of the DUGKS time step
+ * of dugksFoam (Foam::fvDVM).  Plain C types only: no OpenFOAM, HIP or torch types cross
+ * this boundary.  Every call returns 0 on success and a non-zero status otherwise; the
+ * message is available from dugks_last_error().  Nothing inside the library aborts.
+ *
+ * One handle = one solver instance on one GPU with its own HIP stream.  Calls on one
+ * handle are not thread-safe; separate handles are independent.  Host arrays passed in
+ * are copied; the library never keeps a host pointer.
+ *
+ * Face numbering follows an OpenFOAM surface field: internal faces [0, nIF) in polyMesh
+ * order, then the faces of the non-empty boundary patches, patch by patch.  Vectors are
+ * stored xyz-interleaved ([n][3]).
+ */
+#ifndef DUGKS_H
+#define DUGKS_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DUGKS_ABI_VERSION 1
+
+/* storage precision of the distributions */
+enum dugks_precision {
+    DUGKS_FP64 = 0,   /* g/h stored and reconstructed in FP64 */
+    DUGKS_MIXED = 1   /* g/h stored and reconstructed in FP32; moments, equilibrium,
+                         cell update and all macroscopic fields in FP64 */
+};
+
+/* rho boundary types (0/rho "type" entries) understood by the path.  The DF face type
+ * follows the map of src/fvDVM/discreteVelocity/discreteVelocity.C:188-195. */
+enum dugks_patch_type {
+    DUGKS_PATCH_MAXWELL_WALL = 1,  /* rho calculatedMaxwell -> DF maxwellWall (diffuse wall) */
+    DUGKS_PATCH_MIXED = 2          /* rho fixedValue        -> DF mixed (frozen incoming) */
+};
+
+/* Mesh geometry as OpenFOAM's fvMesh provides it (mesh.owner(), neighbour(), Sf(), Cf(),
+ * C(), V(), weights(), deltaCoeffs()); replaces the fvMesh reference held by
+ * Foam::fvDVM (src/fvDVM/fvDVM/fvDVM.H:80-409). */
+typedef struct dugks_mesh {
+    int n_cells;
+    int n_internal_faces;
+    int n_boundary_faces;      /* faces of the non-empty patches */
+    int n_solution_d;          /* mesh.nSolutionD(): 2 for one-cell-thick meshes, else 3 */
+    const int *owner;          /* [nIF + nB] */
+    const int *neighbour;      /* [nIF] */
+    const double *Sf;          /* [nIF + nB][3] face area vectors (out of the owner) */
+    const double *Cf;          /* [nIF + nB][3] face centres */
+    const double *C;           /* [n_cells][3] cell centres */
+    const double *V;           /* [n_cells] cell volumes */
+    const double *weights;     /* [nIF] linear interpolation weights */
+    const double *delta_coeffs;/* [nIF] 1/|C_N - C_P| */
+    /* optional least-squares vectors (leastSquaresVectors::pVectors()/nVectors());
+     * NULL -> computed by the library with the formula of
+     * src/fvDVM/zeroBoundaryGrad/zeroBoundaryVectors.C:115-122 plus non-coupled boundary rows */
+    const double *ls_own;      /* [nIF][3] or NULL */
+    const double *ls_nei;      /* [nIF][3] or NULL */
+} dugks_mesh;
+
+/* One non-empty boundary patch (replaces the per-patch types of 0/rho, 0/U, 0/T). */
+typedef struct dugks_patch {
+    int type;                  /* enum dugks_patch_type */
+    int start;                 /* first face, counted within the boundary faces */
+    int size;
+    const double *rho;         /* [size] patch rho (mixed: fixed value; wall: initial) */
+    const double *U;           /* [size][3] fixed U */
+    const double *T;           /* [size] fixed T */
+} dugks_patch;
+
+/* constant/Xis, constant/weights and the fvDVMparas entries (fvDVM.C:1928-1933). */
+typedef struct dugks_vset {
+    int n_per_dim;             /* nDV */
+    const double *xis;         /* [nDV] */
+    const double *weights;     /* [nDV] full quadrature measures */
+    double xi_max;             /* xiMax (used by the Courant number only) */
+    double xi_min;             /* xiMin (unused by the step, kept for the record) */
+} dugks_vset;
+
+/* gasProperties (fvDVM.C:1943-1948). */
+typedef struct dugks_gas {
+    double R, omega, Tref, muRef, Pr;
+    int KInner;
+} dugks_gas;
+
+typedef struct dugks_options {
+    int precision;             /* enum dugks_precision */
+    int q1_compat;             /* 1: wall-face relaxation of h takes T from the
+                                  internal-face Tsurf at the patch-local index, as
+                                  fvDVM.C:1367,1413 does; 0: the patch temperature */
+    int device;                /* HIP device ordinal */
+    int fused;                 /* 1: fused two-pass path (error if unavailable);
+                                  0: staged path with materialised face distributions;
+                                  -1: fused when available, else staged */
+} dugks_options;
+
+typedef struct dugks_handle dugks_handle;
+
+/* Replaces fvDVM::fvDVM (fvDVM.C:1901-2117) up to, but excluding, the initial state.
+ * Unknown or unsupported patch types fail here. */
+int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patches,
+                 const dugks_vset *vset, const dugks_gas *gas, const dugks_options *opt,
+                 dugks_handle **out);
+
+/* Initial state from cell macros: BGK Maxwellians (fvDVM::initialiseDV, fvDVM.C:414-426),
+ * q = 0, tau (fvDVM.C:2115), inComingByRho of the diffuse walls
+ * (setCalculatedMaxwellRhoBC, fvDVM.C:578-624), frozen mixed-patch DFs
+ * (discreteVelocity::initBoundaryField, discreteVelocity.C:226-258) and
+ * Usurf = linear interpolate(U) (fvDVM.C:2116). */
+int dugks_init_from_macros(dugks_handle *h, const double *rho, const double *U, const double *T);
+
+/* fvDVM::getCoNum (fvDVM.C:2177-2185) for time step dt; synchronises the stream. */
+int dugks_co_num(dugks_handle *h, double dt, double *co_max, double *co_mean);
+
+/* fvDVM::evolution (fvDVM.C:2128-2174): one step with time step dt.  Asynchronous. */
+int dugks_evolution(dugks_handle *h, double dt);
+
+/* n iterations of the dugksFoam loop body (dugksFoam.C:63-76): Courant number,
+ * setDeltaTvar.H:34-50 time-step update (when adjust != 0), evolution.  The time step
+ * lives on the device, so the loop needs no host round trip.  *dt is read as the current
+ * time step and receives the last one used.  Asynchronous unless dt_history != NULL. */
+int dugks_advance(dugks_handle *h, int n_steps, int adjust, double max_co, double max_delta_t,
+                  double *dt, double *dt_history);
+
+int dugks_synchronize(dugks_handle *h);
+
+/* field output (host copies; blocking).  Cell fields [n_cells], face fields [nIF + nB]. */
+int dugks_get_cell_macros(dugks_handle *h, double *rho, double *U, double *T, double *q, double *tau);
+int dugks_get_face_macros(dugks_handle *h, double *rho, double *U, double *T, double *q, double *tau);
+/* boundary rho of the patches (diffuse walls: the wall density of the last step) [nB] */
+int dugks_get_boundary_rho(dugks_handle *h, double *rho_b);
+
+/* distributions g~, h~ as FP64 [Nv][n_cells] (writeDFonCells, fvDVM.C:1835-1897,
+ * generalised to the full field); set_df also serves checkpoint loads. */
+int dugks_get_df(dugks_handle *h, double *g, double *hh);
+int dugks_set_df(dugks_handle *h, const double *g, const double *hh);
+int dugks_set_cell_macros(dugks_handle *h, const double *rho, const double *U, const double *T,
+                          const double *q);
+
+/* binary checkpoint of DFs + macros + wall state (exact resume) */
+int dugks_checkpoint_save(dugks_handle *h, const char *path);
+int dugks_checkpoint_load(dugks_handle *h, const char *path);
+
+/* sizes */
+int dugks_num_velocities(const dugks_handle *h);
+int dugks_velocity_set(const dugks_handle *h, double *xi, double *w);  /* [Nv][3], [Nv] */
+
+/* per-kernel timing of the last evolution (HIP events), seconds; names via index */
+int dugks_kernel_count(const dugks_handle *h);
+const char *dugks_kernel_name(const dugks_handle *h, int i);
+int dugks_profile(dugks_handle *h, int enable);
+int dugks_kernel_times(dugks_handle *h, double *seconds, int *calls);
+
+const char *dugks_last_error(void);
+int dugks_abi_version(void);
+void dugks_destroy(dugks_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,78 @@
+/*
+ * FP64 CPU oracle for the DUGKS time step -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This is a plain-C restatement of dugksFoam's fvDVM::evolution()
+ * (src/fvDVM/fvDVM/fvDVM.C:2128-2174) and the members it calls, written from the
+ * reference sources.  It is the checker for the HIP path and the CPU baseline.  Only the
+ * test suite, the smoke check and the benchmark's CPU-baseline leg may load it; the
+ * product library never links or calls it.
+ *
+ * Layout mirrors the reference: distributions are velocity-major [v][cell] and
+ * [v][face] (src/fvDVM/fvDVM/fvDVM.C:403-408).  Faces are numbered internal faces first,
+ * then the non-empty boundary faces patch by patch, as in an OpenFOAM surface field.
+ */
+#ifndef DUGKS_ORACLE_H
+#define DUGKS_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* rho boundary types (0/rho) that the path supports; the DF face type follows the map
+ * of src/fvDVM/discreteVelocity/discreteVelocity.C:188-195. */
+enum { ORC_PATCH_MAXWELL_WALL = 1, /* rho calculatedMaxwell -> DF maxwellWall */
+       ORC_PATCH_MIXED = 2 };      /* rho fixedValue       -> DF mixed       */
+
+typedef struct orc_case {
+    int nC, nIF, nB, D;
+    const int *owner;        /* [nIF+nB] */
+    const int *neighbour;    /* [nIF] */
+    const double *Sf, *Cf;   /* [nIF+nB][3] */
+    const double *C, *V;     /* [nC][3], [nC] */
+    const double *weights;   /* [nIF] surface interpolation weights */
+    const double *delta_coeffs; /* [nIF] */
+    int n_patches;
+    const int *patch_type, *patch_start, *patch_size; /* start/size within boundary faces */
+    const double *bf_rho, *bf_U, *bf_T;  /* boundary-face values [nB], [nB][3], [nB] */
+    int n1d;
+    const double *xis, *wts; /* 1-D set */
+    double xi_max;           /* DVMProperties xiMax (CFL only) */
+    double R, omega, Tref, muRef, Pr;
+    int K;                   /* KInner */
+    int q1_compat;           /* replicate the Tsurf internal-field read of fvDVM.C:1413 */
+} orc_case;
+
+typedef struct orc_state orc_state;
+
+orc_state *orc_create(const orc_case *cs);
+void orc_destroy(orc_state *s);
+const char *orc_last_error(void);
+int orc_set_threads(int n);
+
+/* initial state: BGK Maxwellians from cell macros (fvDVM.C:414-426), q = 0, tau,
+ * inComingByRho (fvDVM.C:578-624), frozen mixed-patch DFs (discreteVelocity.C:244-257),
+ * Usurf = linear interpolate(U) (fvDVM.C:2116). */
+int orc_init(orc_state *s, const double *rho, const double *U, const double *T);
+
+int orc_co_num(const orc_state *s, double dt, double *co_max, double *co_mean);
+int orc_evolution(orc_state *s, double dt);
+
+/* getters (host copies) */
+int orc_nv(const orc_state *s);
+int orc_get_cell_macros(const orc_state *s, double *rho, double *U, double *T, double *q, double *tau);
+int orc_get_face_macros(const orc_state *s, double *rho, double *U, double *T, double *q, double *tau);
+int orc_get_df(const orc_state *s, double *g, double *h);        /* [Nv][nC] */
+int orc_set_df(orc_state *s, const double *g, const double *h);
+int orc_set_cell_macros(orc_state *s, const double *rho, const double *U, const double *T, const double *q);
+int orc_get_bar(const orc_state *s, double *g, double *h);       /* last f-bar-plus [Nv][nC] */
+int orc_get_surf(const orc_state *s, double *g, double *h);      /* last face DFs [Nv][nIF+nB] */
+int orc_get_wall_rho(const orc_state *s, double *rho_b);         /* [nB] (walls: rho_w) */
+int orc_get_incoming_by_rho(const orc_state *s, double *in_b);   /* [nB] */
+int orc_get_ls_vectors(const orc_state *s, double *own_ls, double *nei_ls, double *patch_ls);
+/* velocity set as built (tensor product, ix fastest) */
+int orc_get_vset(const orc_state *s, double *xi, double *w);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

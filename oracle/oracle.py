@@ -1,0 +1,179 @@
+"""ctypes binding of the FP64 CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Loaded by the test suite, by the smoke check and by the benchmark's CPU-baseline leg,
+always as the checker.  The product library never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+_D = ct.POINTER(ct.c_double)
+_I = ct.POINTER(ct.c_int)
+
+
+class OrcCase(ct.Structure):
+    _fields_ = [
+        ("nC", ct.c_int), ("nIF", ct.c_int), ("nB", ct.c_int), ("D", ct.c_int),
+        ("owner", _I), ("neighbour", _I), ("Sf", _D), ("Cf", _D), ("C", _D), ("V", _D),
+        ("weights", _D), ("delta_coeffs", _D),
+        ("n_patches", ct.c_int), ("patch_type", _I), ("patch_start", _I), ("patch_size", _I),
+        ("bf_rho", _D), ("bf_U", _D), ("bf_T", _D),
+        ("n1d", ct.c_int), ("xis", _D), ("wts", _D), ("xi_max", ct.c_double),
+        ("R", ct.c_double), ("omega", ct.c_double), ("Tref", ct.c_double), ("muRef", ct.c_double),
+        ("Pr", ct.c_double), ("K", ct.c_int), ("q1_compat", ct.c_int),
+    ]
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ct.CDLL(_LIB_PATH)
+        L.orc_create.restype = ct.c_void_p
+        L.orc_create.argtypes = [ct.POINTER(OrcCase)]
+        L.orc_destroy.argtypes = [ct.c_void_p]
+        L.orc_last_error.restype = ct.c_char_p
+        for name in ("orc_init", "orc_set_df", "orc_set_cell_macros"):
+            getattr(L, name).argtypes = [ct.c_void_p] + [_D] * {"orc_init": 3, "orc_set_df": 2,
+                                                                "orc_set_cell_macros": 4}[name]
+        L.orc_evolution.argtypes = [ct.c_void_p, ct.c_double]
+        L.orc_co_num.argtypes = [ct.c_void_p, ct.c_double, _D, _D]
+        L.orc_nv.argtypes = [ct.c_void_p]
+        L.orc_set_threads.argtypes = [ct.c_int]
+        for name, n in (("orc_get_cell_macros", 5), ("orc_get_face_macros", 5), ("orc_get_df", 2),
+                        ("orc_get_bar", 2), ("orc_get_surf", 2), ("orc_get_wall_rho", 1),
+                        ("orc_get_incoming_by_rho", 1), ("orc_get_ls_vectors", 3), ("orc_get_vset", 2)):
+            getattr(L, name).argtypes = [ct.c_void_p] + [_D] * n
+        _lib = L
+    return _lib
+
+
+def _p(a, typ=_D):
+    return a.ctypes.data_as(typ)
+
+
+class Oracle:
+    """FP64 reference-restatement solver over one case (see oracle/dugks_oracle.c)."""
+
+    def __init__(self, case, threads: int | None = None):
+        L = lib()
+        if threads:
+            L.orc_set_threads(int(threads))
+        a = case.arrays()
+        self._a = a  # keep arrays alive while the C side copies them
+        cs = OrcCase(
+            nC=a["nC"], nIF=a["nIF"], nB=a["nB"], D=a["D"],
+            owner=_p(a["owner"], _I), neighbour=_p(a["neighbour"], _I),
+            Sf=_p(a["Sf"]), Cf=_p(a["Cf"]), C=_p(a["C"]), V=_p(a["V"]),
+            weights=_p(a["weights"]), delta_coeffs=_p(a["delta_coeffs"]),
+            n_patches=len(a["patch_type"]), patch_type=_p(a["patch_type"], _I),
+            patch_start=_p(a["patch_start"], _I), patch_size=_p(a["patch_size"], _I),
+            bf_rho=_p(a["bf_rho"]), bf_U=_p(a["bf_U"]), bf_T=_p(a["bf_T"]),
+            n1d=len(a["xis"]), xis=_p(a["xis"]), wts=_p(a["wts"]), xi_max=a["xi_max"],
+            R=a["R"], omega=a["omega"], Tref=a["Tref"], muRef=a["muRef"], Pr=a["Pr"],
+            K=a["K"], q1_compat=a["q1_compat"])
+        self._h = L.orc_create(ct.byref(cs))
+        if not self._h:
+            raise RuntimeError(L.orc_last_error().decode())
+        self.L = L
+        self.nC, self.nIF, self.nB = a["nC"], a["nIF"], a["nB"]
+        self.nF = self.nIF + self.nB
+        self.Nv = L.orc_nv(self._h)
+        self.D = a["D"]
+        self.controls = case.controls
+        self.dt = case.controls.deltaT
+        L.orc_init(self._h, _p(a["rho0"]), _p(a["U0"]), _p(a["T0"]))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self.L.orc_destroy(self._h)
+            self._h = None
+
+    def co_num(self, dt=None):
+        mx, mean = ct.c_double(), ct.c_double()
+        self.L.orc_co_num(self._h, self.dt if dt is None else dt, ct.byref(mx), ct.byref(mean))
+        return mx.value, mean.value
+
+    def evolution(self, dt):
+        if self.L.orc_evolution(self._h, float(dt)) != 0:
+            raise RuntimeError(self.L.orc_last_error().decode())
+
+    def step(self, n=1):
+        """n steps of the dugksFoam loop: CFL, setDeltaT, evolution (dugksFoam.C:63-76)."""
+        from dugks_amd.timecontrol import next_delta_t
+        for _ in range(n):
+            co, _ = self.co_num()
+            self.dt = next_delta_t(self.dt, co, self.controls)
+            self.evolution(self.dt)
+
+    def cell_macros(self):
+        nC = self.nC
+        rho, U, T, q, tau = np.empty(nC), np.empty((nC, 3)), np.empty(nC), np.empty((nC, 3)), np.empty(nC)
+        self.L.orc_get_cell_macros(self._h, _p(rho), _p(U), _p(T), _p(q), _p(tau))
+        return dict(rho=rho, U=U, T=T, q=q, tau=tau)
+
+    def set_cell_macros(self, rho, U, T, q):
+        f = lambda x: np.ascontiguousarray(x, dtype=np.float64)
+        rho, U, T, q = f(rho), f(U), f(T), f(q)
+        self.L.orc_set_cell_macros(self._h, _p(rho), _p(U), _p(T), _p(q))
+
+    def face_macros(self):
+        nF = self.nF
+        rho, U, T, q, tau = np.empty(nF), np.empty((nF, 3)), np.empty(nF), np.empty((nF, 3)), np.empty(nF)
+        self.L.orc_get_face_macros(self._h, _p(rho), _p(U), _p(T), _p(q), _p(tau))
+        return dict(rho=rho, U=U, T=T, q=q, tau=tau)
+
+    def df(self):
+        g, h = np.empty((self.Nv, self.nC)), np.empty((self.Nv, self.nC))
+        self.L.orc_get_df(self._h, _p(g), _p(h))
+        return g, h
+
+    def set_df(self, g, h):
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        self.L.orc_set_df(self._h, _p(g), _p(h))
+
+    def bar(self):
+        g, h = np.empty((self.Nv, self.nC)), np.empty((self.Nv, self.nC))
+        self.L.orc_get_bar(self._h, _p(g), _p(h))
+        return g, h
+
+    def surf(self):
+        g, h = np.empty((self.Nv, self.nF)), np.empty((self.Nv, self.nF))
+        self.L.orc_get_surf(self._h, _p(g), _p(h))
+        return g, h
+
+    def wall_rho(self):
+        r = np.empty(self.nB)
+        self.L.orc_get_wall_rho(self._h, _p(r))
+        return r
+
+    def incoming_by_rho(self):
+        r = np.empty(self.nB)
+        self.L.orc_get_incoming_by_rho(self._h, _p(r))
+        return r
+
+    def ls_vectors(self):
+        o, n, p = np.empty((self.nIF, 3)), np.empty((self.nIF, 3)), np.empty((self.nB, 3))
+        self.L.orc_get_ls_vectors(self._h, _p(o), _p(n), _p(p))
+        return o, n, p
+
+    def vset(self):
+        xi, w = np.empty((self.Nv, 3)), np.empty(self.Nv)
+        self.L.orc_get_vset(self._h, _p(xi), _p(w))
+        return xi, w

@@ -1,0 +1,692 @@
+// C-ABI of libdugks_hip.so (declared in include/dugks.h).  Owns the device buffers of
+// one solver instance, sequences the kernels of one DUGKS step on the handle's stream
+// and keeps the time step on the device so that n steps need no host round trip.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dugks.h"
+#include "dugks_fused.h"
+#include "dugks_internal.h"
+
+using namespace dugks;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::string &m)
+{
+    g_err = m;
+    return 1;
+}
+
+#define HIPCHK(call)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess) return fail(std::string(#call) + ": " + hipGetErrorString(e_));          \
+    } while (0)
+
+__global__ void k_set_scalar(double *p, double v) { *p = v; }
+
+enum Phase {
+    kPhBar, kPhFaceRecon, kPhWallRho, kPhWallIn, kPhFaceMom, kPhFaceRelax, kPhUpdate, kPhCellMom, kPhDt,
+    kPhFusedFace, kPhFaceMacro, kPhFusedUpdate, kPhCellMacro, kNumPhases
+};
+const char *kPhaseNames[kNumPhases] = {
+    "bar", "face_recon", "wall_rho", "wall_in", "face_moments", "face_relax", "update", "cell_moments", "dt_update",
+    "fused_face", "face_macros", "fused_update", "cell_macros"};
+
+}  // namespace
+
+struct dugks_handle {
+    dugks_options opt{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevState d;
+    Geo g{};
+    size_t es = 8;                 // bytes per stored distribution value
+    bool initialised = false;
+    bool fused = false;
+    FusedPlan plan;                // fused-path data (tensor velocity sets)
+    int hist_cap = 0;
+    std::vector<void *> allocs;
+    // profiling
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<int> ev_phase;
+    int ev_used = 0;
+    double ptime[kNumPhases] = {0};
+    int pcalls[kNumPhases] = {0};
+};
+
+namespace {
+
+template <class T>
+int dalloc(dugks_handle *h, T **p, size_t n)
+{
+    void *q = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&q, n * sizeof(T));
+    if (e != hipSuccess)
+        return fail("hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed: " + hipGetErrorString(e));
+    h->allocs.push_back(q);
+    *p = static_cast<T *>(q);
+    return 0;
+}
+
+int dalloc_bytes(dugks_handle *h, void **p, size_t bytes)
+{
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes ? bytes : 1);
+    if (e != hipSuccess)
+        return fail("hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+    h->allocs.push_back(q);
+    *p = q;
+    return 0;
+}
+
+template <class T>
+int upload(T *dst, const T *src, size_t n, hipStream_t st)
+{
+    if (n == 0) return 0;
+    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+// ---------------------------------------------------------------- profiling helpers
+int prof_begin(dugks_handle *h, int phase, hipEvent_t *stop)
+{
+    if (!h->prof) return 0;
+    if (h->ev_used + 2 > (int)h->ev_pool.size()) {
+        // fold what has been recorded so far
+        HIPCHK(hipStreamSynchronize(h->stream));
+        for (int i = 0; i + 1 < h->ev_used; i += 2) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, h->ev_pool[i], h->ev_pool[i + 1]));
+            h->ptime[h->ev_phase[i / 2]] += ms * 1e-3;
+            h->pcalls[h->ev_phase[i / 2]] += 1;
+        }
+        h->ev_used = 0;
+    }
+    hipEvent_t a = h->ev_pool[h->ev_used], b = h->ev_pool[h->ev_used + 1];
+    h->ev_phase[h->ev_used / 2] = phase;
+    h->ev_used += 2;
+    HIPCHK(hipEventRecord(a, h->stream));
+    *stop = b;
+    return 0;
+}
+
+int prof_end(dugks_handle *h, hipEvent_t stop)
+{
+    if (!h->prof) return 0;
+    HIPCHK(hipEventRecord(stop, h->stream));
+    return 0;
+}
+
+#define PHASE(ph, launch)                                                                              \
+    do {                                                                                               \
+        hipEvent_t stop_ = nullptr;                                                                    \
+        if (prof_begin(h, ph, &stop_)) return 1;                                                       \
+        launch;                                                                                        \
+        if (prof_end(h, stop_)) return 1;                                                              \
+    } while (0)
+
+template <class S>
+int staged_step(dugks_handle *h)
+{
+    const Geo &g = h->g;
+    DevState &d = h->d;
+    hipStream_t st = h->stream;
+    S *gT = (S *)d.gT, *hT = (S *)d.hT, *gB = (S *)d.gB, *hB = (S *)d.hB, *gS = (S *)d.gS, *hS = (S *)d.hS;
+    PHASE(kPhBar, Staged<S>::bar(g, d.cell, gT, hT, gB, hB, d.dt, st));
+    PHASE(kPhFaceRecon, Staged<S>::face_recon(g, gB, hB, gS, hS, d.dt, st));
+    PHASE(kPhWallRho, Staged<S>::wall_rho(g, gS, d.rho_b, st));
+    PHASE(kPhWallIn, Staged<S>::wall_in(g, d.rho_b, gS, hS, st));
+    PHASE(kPhFaceMom, Staged<S>::face_moments(g, gS, hS, d.face, d.dt, d.cfl_part, st));
+    PHASE(kPhFaceRelax, Staged<S>::face_relax(g, d.face, gS, hS, d.dt, st));
+    PHASE(kPhUpdate, Staged<S>::update(g, gB, hB, gS, hS, gT, hT, d.dt, st));
+    PHASE(kPhCellMom, Staged<S>::cell_moments(g, gT, hT, d.cell, d.dt, st));
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+template <class S>
+int fused_step(dugks_handle *h)
+{
+    const Geo &g = h->g;
+    DevState &d = h->d;
+    hipStream_t st = h->stream;
+    FusedPlan &p = h->plan;
+    PHASE(kPhFusedFace, Fused<S>::face_pass(g, p, (const S *)d.gT, (const S *)d.hT, d.cell, d.dt, st));
+    PHASE(kPhFaceMacro, Fused<S>::face_macros(g, p, d.face, d.rho_b, d.dt, d.cfl_part, st));
+    PHASE(kPhFusedUpdate, Fused<S>::update_pass(g, p, (S *)d.gT, (S *)d.hT, d.cell, d.face, d.rho_b, d.dt, st));
+    PHASE(kPhCellMacro, Fused<S>::cell_macros(g, p, d.cell, d.dt, st));
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int do_step(dugks_handle *h)
+{
+    if (h->fused) return h->es == 4 ? fused_step<float>(h) : fused_step<double>(h);
+    return h->es == 4 ? staged_step<float>(h) : staged_step<double>(h);
+}
+
+int check_handle(dugks_handle *h, bool need_init = true)
+{
+    if (!h) return fail("null handle");
+    if (need_init && !h->initialised) return fail("handle not initialised: call dugks_init_from_macros first");
+    HIPCHK(hipSetDevice(h->device));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *dugks_last_error(void) { return g_err.c_str(); }
+int dugks_abi_version(void) { return DUGKS_ABI_VERSION; }
+
+int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patches, const dugks_vset *vset,
+                 const dugks_gas *gas, const dugks_options *opt, dugks_handle **out)
+{
+    if (!out) return fail("dugks_create: out is NULL");
+    *out = nullptr;
+    if (!mesh || !vset || !gas) return fail("dugks_create: mesh, vset and gas are required");
+    const int nC = mesh->n_cells, nIF = mesh->n_internal_faces, nB = mesh->n_boundary_faces;
+    const int D = mesh->n_solution_d, n1 = vset->n_per_dim;
+    if (nC <= 0 || nIF < 0 || nB < 0) return fail("dugks_create: invalid mesh sizes");
+    if (D < 1 || D > 3) return fail("dugks_create: n_solution_d must be 1, 2 or 3");
+    if (n1 <= 0 || !vset->xis || !vset->weights) return fail("dugks_create: empty velocity set");
+    if (!mesh->owner || (nIF > 0 && !mesh->neighbour) || !mesh->Sf || !mesh->Cf || !mesh->C || !mesh->V ||
+        (nIF > 0 && (!mesh->weights || !mesh->delta_coeffs)))
+        return fail("dugks_create: missing mesh arrays");
+    if (n_patches < 0 || (n_patches > 0 && !patches)) return fail("dugks_create: invalid patch list");
+    if (!(gas->R > 0) || !(gas->Tref > 0) || !(gas->muRef > 0)) return fail("dugks_create: invalid gas properties");
+    dugks_options o{};
+    o.precision = DUGKS_FP64;
+    o.q1_compat = 1;
+    o.device = 0;
+    o.fused = -1;
+    if (opt) o = *opt;
+    if (o.precision != DUGKS_FP64 && o.precision != DUGKS_MIXED) return fail("dugks_create: unknown precision");
+
+    const int nF = nIF + nB;
+    long long nv_ll = 1;
+    for (int k = 0; k < D; ++k) nv_ll *= n1;
+    if (nv_ll > 65535) return fail("dugks_create: more than 65535 discrete velocities");
+    const int Nv = (int)nv_ll;
+
+    // boundary-face patch map
+    std::vector<int> bf_type(nB, -1), bf_local(nB, 0);
+    std::vector<double> bf_rho(nB), bf_T(nB);
+    std::vector<d3> bf_U(nB);
+    for (int p = 0; p < n_patches; ++p) {
+        const dugks_patch &pp = patches[p];
+        if (pp.type != DUGKS_PATCH_MAXWELL_WALL && pp.type != DUGKS_PATCH_MIXED)
+            return fail("dugks_create: patch " + std::to_string(p) + " has unsupported type " + std::to_string(pp.type) +
+                        " (supported: calculatedMaxwell -> maxwellWall, fixedValue -> mixed)");
+        if (pp.size < 0 || pp.start < 0 || pp.start + pp.size > nB)
+            return fail("dugks_create: patch " + std::to_string(p) + " range outside the boundary faces");
+        if (pp.size > 0 && (!pp.rho || !pp.U || !pp.T))
+            return fail("dugks_create: patch " + std::to_string(p) + " lacks rho/U/T values");
+        if (o.q1_compat && pp.size > nIF)
+            return fail("dugks_create: q1_compat reads the internal-face temperature at patch-local indices; patch " +
+                        std::to_string(p) + " is larger than the internal-face count");
+        for (int i = 0; i < pp.size; ++i) {
+            const int b = pp.start + i;
+            if (bf_type[b] != -1) return fail("dugks_create: boundary face " + std::to_string(b) + " in two patches");
+            bf_type[b] = pp.type;
+            bf_local[b] = i;
+            bf_rho[b] = pp.rho[i];
+            bf_U[b] = d3{pp.U[3 * i], pp.U[3 * i + 1], pp.U[3 * i + 2]};
+            bf_T[b] = pp.T[i];
+            if (!(bf_T[b] > 0)) return fail("dugks_create: non-positive patch temperature");
+        }
+    }
+    for (int b = 0; b < nB; ++b)
+        if (bf_type[b] == -1) return fail("dugks_create: boundary face " + std::to_string(b) + " is in no patch");
+
+    // least-squares vectors and stencils
+    std::string err;
+    std::vector<double> own_ls, nei_ls;
+    if (mesh->ls_own && mesh->ls_nei) {
+        own_ls.assign(mesh->ls_own, mesh->ls_own + 3 * (size_t)nIF);
+        nei_ls.assign(mesh->ls_nei, mesh->ls_nei + 3 * (size_t)nIF);
+    } else if (!compute_ls_vectors(nC, nIF, nB, mesh->owner, mesh->neighbour, mesh->Sf, mesh->Cf, mesh->C,
+                                   mesh->weights, own_ls, nei_ls, err)) {
+        return fail("dugks_create: " + err);
+    }
+    HostPrep prep;
+    if (!build_stencils(nC, nIF, nB, mesh->owner, mesh->neighbour, own_ls.data(), nei_ls.data(), prep, err))
+        return fail("dugks_create: " + err);
+
+    // tensor-product velocity set, ix fastest (fvDVM.C:293-373)
+    std::vector<d3> xi(Nv);
+    std::vector<double> w(Nv);
+    {
+        const double *X = vset->xis, *W = vset->weights;
+        int i = 0;
+        if (D == 3) {
+            for (int iz = 0; iz < n1; ++iz)
+                for (int iy = 0; iy < n1; ++iy)
+                    for (int ix = 0; ix < n1; ++ix, ++i) {
+                        w[i] = W[iz] * W[iy] * W[ix];
+                        xi[i] = d3{X[ix], X[iy], X[iz]};
+                    }
+        } else if (D == 2) {
+            for (int iy = 0; iy < n1; ++iy)
+                for (int ix = 0; ix < n1; ++ix, ++i) {
+                    w[i] = W[iy] * W[ix] * 1;
+                    xi[i] = d3{X[ix], X[iy], 0.0};
+                }
+        } else {
+            for (int ix = 0; ix < n1; ++ix, ++i) {
+                w[i] = W[ix] * 1 * 1;
+                xi[i] = d3{X[ix], 0.0, 0.0};
+            }
+        }
+    }
+
+    dugks_handle *h = new dugks_handle();
+    h->opt = o;
+    h->device = o.device;
+    h->es = o.precision == DUGKS_MIXED ? 4 : 8;
+    auto bail = [&](int rc) {
+        std::string m = g_err;
+        dugks_destroy(h);
+        g_err = m;
+        return rc;
+    };
+    if (hipSetDevice(h->device) != hipSuccess) return bail(fail("dugks_create: hipSetDevice failed"));
+    if (hipStreamCreate(&h->stream) != hipSuccess) return bail(fail("dugks_create: hipStreamCreate failed"));
+    DevState &d = h->d;
+    hipStream_t st = h->stream;
+    // geometry
+    if (dalloc(h, &d.owner, nF) || dalloc(h, &d.neighbour, nIF) || dalloc(h, &d.Sf, nF) || dalloc(h, &d.Cf, nF) ||
+        dalloc(h, &d.C, nC) || dalloc(h, &d.V, nC) || dalloc(h, &d.cf_off, nC + 1) ||
+        dalloc(h, &d.cf_face, prep.cf_face.size()) || dalloc(h, &d.cg_off, nC + 1) ||
+        dalloc(h, &d.cg_nb, prep.cg_nb.size()) || dalloc(h, &d.cg_ls, prep.cg_ls.size()) ||
+        dalloc(h, &d.bf_type, nB) || dalloc(h, &d.bf_local, nB) || dalloc(h, &d.bf_rho, nB) ||
+        dalloc(h, &d.bf_T, nB) || dalloc(h, &d.bf_U, nB) || dalloc(h, &d.in_by_rho, nB) || dalloc(h, &d.dc, nIF) ||
+        dalloc(h, &d.weights, nIF) || dalloc(h, &d.xi, Nv) || dalloc(h, &d.w, Nv) || dalloc(h, &d.rho_b, nB) ||
+        dalloc(h, &d.cfl_part, 2 * (size_t)cfl_blocks(nIF)) || dalloc(h, &d.dt, 4))
+        return bail(1);
+    if (dalloc(h, &d.cell.rho, nC) || dalloc(h, &d.cell.U, nC) || dalloc(h, &d.cell.T, nC) ||
+        dalloc(h, &d.cell.q, nC) || dalloc(h, &d.cell.tau, nC) || dalloc(h, &d.face.rho, nF) ||
+        dalloc(h, &d.face.U, nF) || dalloc(h, &d.face.T, nF) || dalloc(h, &d.face.q, nF) ||
+        dalloc(h, &d.face.tau, nF))
+        return bail(1);
+    std::vector<d3> Sf(nF), Cf(nF), C(nC);
+    std::memcpy(Sf.data(), mesh->Sf, sizeof(d3) * nF);
+    std::memcpy(Cf.data(), mesh->Cf, sizeof(d3) * nF);
+    std::memcpy(C.data(), mesh->C, sizeof(d3) * nC);
+    if (upload(d.owner, mesh->owner, nF, st) || upload(d.neighbour, mesh->neighbour, nIF, st) ||
+        upload(d.Sf, Sf.data(), nF, st) || upload(d.Cf, Cf.data(), nF, st) || upload(d.C, C.data(), nC, st) ||
+        upload(d.V, mesh->V, nC, st) || upload(d.cf_off, prep.cf_off.data(), nC + 1, st) ||
+        upload(d.cf_face, prep.cf_face.data(), prep.cf_face.size(), st) ||
+        upload(d.cg_off, prep.cg_off.data(), nC + 1, st) || upload(d.cg_nb, prep.cg_nb.data(), prep.cg_nb.size(), st) ||
+        upload(d.cg_ls, prep.cg_ls.data(), prep.cg_ls.size(), st) || upload(d.bf_type, bf_type.data(), nB, st) ||
+        upload(d.bf_local, bf_local.data(), nB, st) || upload(d.bf_rho, bf_rho.data(), nB, st) ||
+        upload(d.bf_T, bf_T.data(), nB, st) || upload(d.bf_U, bf_U.data(), nB, st) ||
+        upload(d.dc, mesh->delta_coeffs, nIF, st) || upload(d.weights, mesh->weights, nIF, st) ||
+        upload(d.xi, xi.data(), Nv, st) || upload(d.w, w.data(), Nv, st) || upload(d.rho_b, bf_rho.data(), nB, st))
+        return bail(1);
+
+    Geo &g = h->g;
+    g.nC = nC; g.nIF = nIF; g.nB = nB; g.nF = nF; g.D = D; g.K = gas->KInner; g.Nv = Nv; g.n1d = n1;
+    g.R = gas->R; g.omega = gas->omega; g.Tref = gas->Tref; g.muRef = gas->muRef; g.Pr = gas->Pr;
+    g.xi_max = vset->xi_max; g.sqrtD = std::sqrt((double)D); g.q1 = o.q1_compat ? 1 : 0;
+    g.owner = d.owner; g.neighbour = d.neighbour; g.Sf = d.Sf; g.Cf = d.Cf; g.C = d.C; g.V = d.V;
+    g.cf_off = d.cf_off; g.cf_face = d.cf_face; g.cg_off = d.cg_off; g.cg_nb = d.cg_nb; g.cg_ls = d.cg_ls;
+    g.bf_type = d.bf_type; g.bf_local = d.bf_local; g.bf_rho = d.bf_rho; g.bf_T = d.bf_T; g.bf_U = d.bf_U;
+    g.in_by_rho = d.in_by_rho; g.delta_coeffs = d.dc; g.weights = d.weights; g.xi = d.xi; g.w = d.w;
+
+    // distributions
+    const size_t ncell = (size_t)Nv * nC, nface = (size_t)Nv * nF;
+    if (dalloc_bytes(h, &d.gT, ncell * h->es) || dalloc_bytes(h, &d.hT, ncell * h->es)) return bail(1);
+    h->fused = o.fused != 0;
+    if (h->fused) {
+        if (!fused_plan_build(h->plan, g, mesh, prep, vset, h->es, err)) {
+            if (o.fused == 1) return bail(fail("dugks_create: fused path unavailable: " + err));
+            h->fused = false;
+        }
+    }
+    if (h->fused) {
+        if (!fused_plan_upload(h->plan, g, st, err)) return bail(fail("dugks_create: " + err));
+    } else {
+        if (dalloc_bytes(h, &d.gB, ncell * h->es) || dalloc_bytes(h, &d.hB, ncell * h->es) ||
+            dalloc_bytes(h, &d.gS, nface * h->es) || dalloc_bytes(h, &d.hS, nface * h->es))
+            return bail(1);
+    }
+    // profiling event pool
+    h->ev_pool.resize(2 * 1024);
+    h->ev_phase.resize(1024);
+    for (auto &e : h->ev_pool)
+        if (hipEventCreate(&e) != hipSuccess) return bail(fail("dugks_create: hipEventCreate failed"));
+    *out = h;
+    return 0;
+}
+
+int dugks_init_from_macros(dugks_handle *h, const double *rho, const double *U, const double *T)
+{
+    if (check_handle(h, false)) return 1;
+    if (!rho || !U || !T) return fail("dugks_init_from_macros: NULL field");
+    const Geo &g = h->g;
+    DevState &d = h->d;
+    hipStream_t st = h->stream;
+    for (int c = 0; c < g.nC; ++c)
+        if (!(rho[c] > 0) || !(T[c] > 0)) return fail("dugks_init_from_macros: non-positive rho or T at cell " +
+                                                        std::to_string(c));
+    if (upload(d.cell.rho, rho, g.nC, st) || upload(d.cell.U, (const d3 *)U, g.nC, st) ||
+        upload(d.cell.T, T, g.nC, st))
+        return 1;
+    HIPCHK(hipMemsetAsync(d.cell.q, 0, sizeof(d3) * g.nC, st));
+    HIPCHK(hipMemsetAsync(d.face.rho, 0, sizeof(double) * g.nF, st));
+    HIPCHK(hipMemsetAsync(d.face.T, 0, sizeof(double) * g.nF, st));
+    HIPCHK(hipMemsetAsync(d.face.q, 0, sizeof(d3) * g.nF, st));
+    HIPCHK(hipMemsetAsync(d.face.tau, 0, sizeof(double) * g.nF, st));
+    launch_tau(g, d.cell, g.nC, st);
+    if (h->es == 4)
+        launch_init_df<float>(g, d.cell, (float *)d.gT, (float *)d.hT, st);
+    else
+        launch_init_df<double>(g, d.cell, (double *)d.gT, (double *)d.hT, st);
+    launch_init_incoming(g, d.in_by_rho, st);
+    if (!h->fused) {
+        if (h->es == 4)
+            launch_init_mixed_faces<float>(g, (float *)d.gS, (float *)d.hS, st);
+        else
+            launch_init_mixed_faces<double>(g, (double *)d.gS, (double *)d.hS, st);
+    } else {
+        std::string err;
+        if (!fused_init_boundary(h->plan, g, st, err)) return fail("dugks_init_from_macros: " + err);
+    }
+    HIPCHK(hipMemcpyAsync(d.rho_b, g.bf_rho, sizeof(double) * g.nB, hipMemcpyDeviceToDevice, st));
+    launch_init_usurf(g, d.cell, d.face, st);
+    launch_cfl_partials(g, d.face, d.cfl_part, st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    h->initialised = true;
+    return 0;
+}
+
+int dugks_co_num(dugks_handle *h, double dt, double *co_max, double *co_mean)
+{
+    if (check_handle(h)) return 1;
+    double res[4];
+    hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->stream, h->d.dt, dt);
+    launch_dt_update(h->g, h->d.cfl_part, h->d.dt, 0, 0.0, 0.0, nullptr, 0, h->stream);
+    HIPCHK(hipMemcpyAsync(res, h->d.dt, sizeof res, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (co_max) *co_max = res[1];
+    if (co_mean) *co_mean = res[2];
+    return 0;
+}
+
+int dugks_evolution(dugks_handle *h, double dt)
+{
+    if (check_handle(h)) return 1;
+    if (!(dt > 0) || !std::isfinite(dt)) return fail("dugks_evolution: dt must be positive and finite");
+    hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->stream, h->d.dt, dt);
+    return do_step(h);
+}
+
+int dugks_advance(dugks_handle *h, int n_steps, int adjust, double max_co, double max_delta_t, double *dt,
+                  double *dt_history)
+{
+    if (check_handle(h)) return 1;
+    if (n_steps < 0) return fail("dugks_advance: negative step count");
+    if (!dt || !(*dt > 0)) return fail("dugks_advance: *dt must be a positive time step");
+    if (adjust && (!(max_co > 0) || !(max_delta_t > 0))) return fail("dugks_advance: maxCo and maxDeltaT must be > 0");
+    DevState &d = h->d;
+    if (dt_history && n_steps > h->hist_cap) {
+        if (d.dt_hist) {
+            (void)hipFree(d.dt_hist);
+            for (auto &p : h->allocs)
+                if (p == d.dt_hist) p = nullptr;
+        }
+        d.dt_hist = nullptr;
+        if (dalloc(h, &d.dt_hist, n_steps)) return 1;
+        h->hist_cap = n_steps;
+    }
+    hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->stream, d.dt, *dt);
+    for (int i = 0; i < n_steps; ++i) {
+        PHASE(kPhDt, launch_dt_update(h->g, d.cfl_part, d.dt, adjust, max_co, max_delta_t,
+                                      dt_history ? d.dt_hist : nullptr, i, h->stream));
+        if (do_step(h)) return 1;
+    }
+    HIPCHK(hipMemcpyAsync(dt, d.dt, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (dt_history) {
+        if (n_steps > 0)
+            HIPCHK(hipMemcpyAsync(dt_history, d.dt_hist, sizeof(double) * n_steps, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int dugks_synchronize(dugks_handle *h)
+{
+    if (!h) return fail("null handle");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+static int get_macros(dugks_handle *h, const Macros &m, int n, double *rho, double *U, double *T, double *q,
+                      double *tau)
+{
+    hipStream_t st = h->stream;
+    if (rho) HIPCHK(hipMemcpyAsync(rho, m.rho, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    if (U) HIPCHK(hipMemcpyAsync(U, m.U, sizeof(d3) * n, hipMemcpyDeviceToHost, st));
+    if (T) HIPCHK(hipMemcpyAsync(T, m.T, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    if (q) HIPCHK(hipMemcpyAsync(q, m.q, sizeof(d3) * n, hipMemcpyDeviceToHost, st));
+    if (tau) HIPCHK(hipMemcpyAsync(tau, m.tau, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+int dugks_get_cell_macros(dugks_handle *h, double *rho, double *U, double *T, double *q, double *tau)
+{
+    if (check_handle(h)) return 1;
+    return get_macros(h, h->d.cell, h->g.nC, rho, U, T, q, tau);
+}
+
+int dugks_get_face_macros(dugks_handle *h, double *rho, double *U, double *T, double *q, double *tau)
+{
+    if (check_handle(h)) return 1;
+    return get_macros(h, h->d.face, h->g.nF, rho, U, T, q, tau);
+}
+
+int dugks_get_boundary_rho(dugks_handle *h, double *rho_b)
+{
+    if (check_handle(h)) return 1;
+    if (h->g.nB == 0) return 0;
+    HIPCHK(hipMemcpyAsync(rho_b, h->d.rho_b, sizeof(double) * h->g.nB, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int dugks_get_df(dugks_handle *h, double *g, double *hh)
+{
+    if (check_handle(h)) return 1;
+    const size_t n = (size_t)h->g.Nv * h->g.nC;
+    double *outs[2] = {g, hh};
+    void *src[2] = {h->d.gT, h->d.hT};
+    for (int k = 0; k < 2; ++k) {
+        if (!outs[k]) continue;
+        if (h->es == 8) {
+            HIPCHK(hipMemcpyAsync(outs[k], src[k], n * 8, hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+        } else {
+            std::vector<float> tmp(n);
+            HIPCHK(hipMemcpyAsync(tmp.data(), src[k], n * 4, hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+            for (size_t i = 0; i < n; ++i) outs[k][i] = tmp[i];
+        }
+    }
+    return 0;
+}
+
+int dugks_set_df(dugks_handle *h, const double *g, const double *hh)
+{
+    if (check_handle(h)) return 1;
+    if (!g || !hh) return fail("dugks_set_df: NULL array");
+    const size_t n = (size_t)h->g.Nv * h->g.nC;
+    const double *ins[2] = {g, hh};
+    void *dst[2] = {h->d.gT, h->d.hT};
+    for (int k = 0; k < 2; ++k) {
+        if (h->es == 8) {
+            HIPCHK(hipMemcpyAsync(dst[k], ins[k], n * 8, hipMemcpyHostToDevice, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+        } else {
+            std::vector<float> tmp(n);
+            for (size_t i = 0; i < n; ++i) tmp[i] = (float)ins[k][i];
+            HIPCHK(hipMemcpyAsync(dst[k], tmp.data(), n * 4, hipMemcpyHostToDevice, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+        }
+    }
+    return 0;
+}
+
+int dugks_set_cell_macros(dugks_handle *h, const double *rho, const double *U, const double *T, const double *q)
+{
+    if (check_handle(h)) return 1;
+    const Geo &g = h->g;
+    DevState &d = h->d;
+    if (upload(d.cell.rho, rho, g.nC, h->stream) || upload(d.cell.U, (const d3 *)U, g.nC, h->stream) ||
+        upload(d.cell.T, T, g.nC, h->stream) || upload(d.cell.q, (const d3 *)q, g.nC, h->stream))
+        return 1;
+    launch_tau(g, d.cell, g.nC, h->stream);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+// ------------------------------------------------------------------- checkpoints
+namespace {
+struct CkptHeader {
+    char magic[8];
+    int version, nC, nF, nB, Nv, es;
+    double reserved[4];
+};
+
+int ckpt_io(dugks_handle *h, const char *path, bool save)
+{
+    if (check_handle(h)) return 1;
+    if (!path) return fail("checkpoint: NULL path");
+    const Geo &g = h->g;
+    DevState &d = h->d;
+    std::FILE *fp = std::fopen(path, save ? "wb" : "rb");
+    if (!fp) return fail(std::string("checkpoint: cannot open ") + path);
+    CkptHeader hd{};
+    std::memcpy(hd.magic, "DUGKSCK", 8);
+    hd.version = 1; hd.nC = g.nC; hd.nF = g.nF; hd.nB = g.nB; hd.Nv = g.Nv; hd.es = (int)h->es;
+    const size_t ndf = (size_t)g.Nv * g.nC * h->es;
+    struct Blk { void *p; size_t bytes; };
+    std::vector<Blk> blocks = {
+        {d.gT, ndf}, {d.hT, ndf},
+        {d.cell.rho, 8ull * g.nC}, {d.cell.U, 24ull * g.nC}, {d.cell.T, 8ull * g.nC}, {d.cell.q, 24ull * g.nC},
+        {d.cell.tau, 8ull * g.nC},
+        {d.face.rho, 8ull * g.nF}, {d.face.U, 24ull * g.nF}, {d.face.T, 8ull * g.nF}, {d.face.q, 24ull * g.nF},
+        {d.face.tau, 8ull * g.nF}, {d.rho_b, 8ull * g.nB}, {d.cfl_part, 16ull * cfl_blocks(g.nIF)}, {d.dt, 32}};
+    int rc = 0;
+    if (save) {
+        HIPCHK(hipStreamSynchronize(h->stream));
+        if (std::fwrite(&hd, sizeof hd, 1, fp) != 1) rc = fail("checkpoint: write failed");
+    } else {
+        CkptHeader in{};
+        if (std::fread(&in, sizeof in, 1, fp) != 1 || std::memcmp(in.magic, "DUGKSCK", 8) != 0 || in.version != 1)
+            rc = fail("checkpoint: not a checkpoint file");
+        else if (in.nC != hd.nC || in.nF != hd.nF || in.nB != hd.nB || in.Nv != hd.Nv || in.es != hd.es)
+            rc = fail("checkpoint: sizes or precision differ from this handle");
+    }
+    const size_t chunk = 64ull << 20;
+    std::vector<char> buf;
+    for (size_t bi = 0; rc == 0 && bi < blocks.size(); ++bi) {
+        if (!blocks[bi].p || blocks[bi].bytes == 0) continue;
+        for (size_t off = 0; off < blocks[bi].bytes && rc == 0; off += chunk) {
+            const size_t n = std::min(chunk, blocks[bi].bytes - off);
+            buf.resize(n);
+            char *dev = (char *)blocks[bi].p + off;
+            if (save) {
+                if (hipMemcpy(buf.data(), dev, n, hipMemcpyDeviceToHost) != hipSuccess) rc = fail("checkpoint: copy");
+                else if (std::fwrite(buf.data(), 1, n, fp) != n) rc = fail("checkpoint: write failed");
+            } else {
+                if (std::fread(buf.data(), 1, n, fp) != n) rc = fail("checkpoint: truncated file");
+                else if (hipMemcpy(dev, buf.data(), n, hipMemcpyHostToDevice) != hipSuccess) rc = fail("checkpoint: copy");
+            }
+        }
+    }
+    std::fclose(fp);
+    return rc;
+}
+}  // namespace
+
+int dugks_checkpoint_save(dugks_handle *h, const char *path) { return ckpt_io(h, path, true); }
+int dugks_checkpoint_load(dugks_handle *h, const char *path) { return ckpt_io(h, path, false); }
+
+int dugks_num_velocities(const dugks_handle *h) { return h ? h->g.Nv : 0; }
+
+int dugks_velocity_set(const dugks_handle *h, double *xi, double *w)
+{
+    if (!h) return fail("null handle");
+    if (xi && hipMemcpy(xi, h->d.xi, sizeof(d3) * h->g.Nv, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail("dugks_velocity_set: copy failed");
+    if (w && hipMemcpy(w, h->d.w, sizeof(double) * h->g.Nv, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail("dugks_velocity_set: copy failed");
+    return 0;
+}
+
+int dugks_kernel_count(const dugks_handle *) { return kNumPhases; }
+const char *dugks_kernel_name(const dugks_handle *, int i) { return (i >= 0 && i < kNumPhases) ? kPhaseNames[i] : ""; }
+
+int dugks_profile(dugks_handle *h, int enable)
+{
+    if (!h) return fail("null handle");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->prof = enable != 0;
+    h->ev_used = 0;
+    for (int i = 0; i < kNumPhases; ++i) {
+        h->ptime[i] = 0;
+        h->pcalls[i] = 0;
+    }
+    return 0;
+}
+
+int dugks_kernel_times(dugks_handle *h, double *seconds, int *calls)
+{
+    if (!h) return fail("null handle");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int i = 0; i + 1 < h->ev_used; i += 2) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, h->ev_pool[i], h->ev_pool[i + 1]));
+        h->ptime[h->ev_phase[i / 2]] += ms * 1e-3;
+        h->pcalls[h->ev_phase[i / 2]] += 1;
+    }
+    h->ev_used = 0;
+    for (int i = 0; i < kNumPhases; ++i) {
+        if (seconds) seconds[i] = h->ptime[i];
+        if (calls) calls[i] = h->pcalls[i];
+    }
+    return 0;
+}
+
+void dugks_destroy(dugks_handle *h)
+{
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (auto &e : h->ev_pool)
+        if (e) (void)hipEventDestroy(e);
+    fused_plan_free(h->plan);
+    for (void *p : h->allocs)
+        if (p) (void)hipFree(p);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+}  // extern "C"

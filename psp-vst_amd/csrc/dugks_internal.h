@@ -1,0 +1,122 @@
+// Internal declarations shared by the HIP kernels and the C-ABI layer of libdugks_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace dugks {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kVSmall = 1.0e-300;   // OpenFOAM VSMALL (double)
+constexpr double kSmall = 1.0e-15;     // OpenFOAM SMALL (double)
+
+struct d3 {
+    double x, y, z;
+};
+
+__host__ __device__ inline double dot(const d3 &a, const d3 &b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__host__ __device__ inline double magsqr(const d3 &a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+
+enum PatchType : int { kMaxwellWall = 1, kMixed = 2 };
+
+// Read-only geometry and constants, passed by value to every kernel.
+struct Geo {
+    int nC, nIF, nB, nF, D, K, Nv, n1d;
+    double R, omega, Tref, muRef, Pr, xi_max, sqrtD;
+    int q1;
+    const int *owner;        // [nF]
+    const int *neighbour;    // [nIF]
+    const d3 *Sf, *Cf, *C;   // [nF], [nF], [nC]
+    const double *V;         // [nC]
+    // cell -> faces (all faces of the cell, ascending face label); sign +1 owner, -1 neighbour
+    const int *cf_off, *cf_face;
+    // cell -> gradient stencil (internal faces, ascending face label)
+    const int *cg_off, *cg_nb;
+    const d3 *cg_ls;
+    // boundary faces
+    const int *bf_type, *bf_local;
+    const double *bf_rho, *bf_T;
+    const d3 *bf_U;
+    const double *in_by_rho; // [nB] inComingByRho (walls)
+    // internal-face CFL data
+    const double *delta_coeffs; // [nIF]
+    const double *weights;      // [nIF] linear interpolation weights
+    // velocity set
+    const d3 *xi;            // [Nv]
+    const double *w;         // [Nv]
+};
+
+struct Macros {
+    double *rho;
+    d3 *U;
+    double *T;
+    d3 *q;
+    double *tau;
+};
+
+// Dense device buffers of one solver instance.
+struct DevState {
+    // geometry storage (owned)
+    int *owner = nullptr, *neighbour = nullptr, *cf_off = nullptr, *cf_face = nullptr;
+    int *cg_off = nullptr, *cg_nb = nullptr, *bf_type = nullptr, *bf_local = nullptr;
+    d3 *Sf = nullptr, *Cf = nullptr, *C = nullptr, *cg_ls = nullptr, *bf_U = nullptr, *xi = nullptr;
+    double *V = nullptr, *bf_rho = nullptr, *bf_T = nullptr, *in_by_rho = nullptr, *dc = nullptr, *w = nullptr,
+           *weights = nullptr;
+    // distributions (storage type chosen at run time; void* + element size)
+    void *gT = nullptr, *hT = nullptr, *gB = nullptr, *hB = nullptr, *gS = nullptr, *hS = nullptr;
+    // macros
+    Macros cell{}, face{};
+    double *rho_b = nullptr;      // boundary rho (walls: rho_w)
+    double *cfl_part = nullptr;   // [2 * n_cfl_blocks] partial max / sum of deltaCoeffs*(|U|+sqrt(D)xiMax)
+    double *dt = nullptr;         // [4]: dt, co_max, co_mean, spare
+    double *dt_hist = nullptr;    // [hist_cap]
+};
+
+// Host-side geometry preparation (dugks_setup.cpp)
+struct HostPrep {
+    std::vector<int> cf_off, cf_face, cg_off, cg_nb, bf_type, bf_local;
+    std::vector<d3> cg_ls;
+    std::vector<double> own_ls, nei_ls; // [nIF*3]
+};
+
+bool compute_ls_vectors(int nC, int nIF, int nB, const int *owner, const int *neighbour, const double *Sf,
+                        const double *Cf, const double *C, const double *weights, std::vector<double> &own_ls,
+                        std::vector<double> &nei_ls, std::string &err);
+bool build_stencils(int nC, int nIF, int nB, const int *owner, const int *neighbour, const double *own_ls,
+                    const double *nei_ls, HostPrep &p, std::string &err);
+
+// kernel launchers (dugks_kernels.hip); S = float or double storage
+constexpr int kCflBlock = 256;
+int cfl_blocks(int nIF);
+
+template <class S>
+void launch_init_df(const Geo &g, const Macros &cm, S *gT, S *hT, hipStream_t st);
+template <class S>
+void launch_init_mixed_faces(const Geo &g, S *gS, S *hS, hipStream_t st);
+void launch_init_incoming(const Geo &g, double *in_by_rho, hipStream_t st);
+void launch_init_usurf(const Geo &g, const Macros &cm, Macros fm, hipStream_t st);
+void launch_tau(const Geo &g, Macros m, int n, hipStream_t st);
+void launch_cfl_partials(const Geo &g, const Macros &fm, double *part, hipStream_t st);
+void launch_dt_update(const Geo &g, const double *part, double *dt, int adjust, double max_co,
+                      double max_delta_t, double *hist, int step, hipStream_t st);
+
+template <class S>
+struct Staged {
+    static void bar(const Geo &g, const Macros &cm, const S *gT, const S *hT, S *gB, S *hB, const double *dt,
+                    hipStream_t st);
+    static void face_recon(const Geo &g, const S *gB, const S *hB, S *gS, S *hS, const double *dt, hipStream_t st);
+    static void wall_rho(const Geo &g, const S *gS, double *rho_b, hipStream_t st);
+    static void wall_in(const Geo &g, const double *rho_b, S *gS, S *hS, hipStream_t st);
+    static void face_moments(const Geo &g, const S *gS, const S *hS, Macros fm, const double *dt, double *cfl_part,
+                             hipStream_t st);
+    static void face_relax(const Geo &g, const Macros &fm, S *gS, S *hS, const double *dt, hipStream_t st);
+    static void update(const Geo &g, const S *gB, const S *hB, const S *gS, const S *hS, S *gT, S *hT,
+                       const double *dt, hipStream_t st);
+    static void cell_moments(const Geo &g, const S *gT, const S *hT, Macros cm, const double *dt, hipStream_t st);
+};
+
+}  // namespace dugks

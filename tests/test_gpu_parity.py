@@ -1,0 +1,110 @@
+"""GPU parity of the DUGKS step against the FP64 CPU oracle (oracle/dugks_oracle.c).
+
+Tolerances (relative to the field's max magnitude, see conftest.field_rel_err):
+  * FP64 GPU path vs oracle: 1e-12 on the tiny cavity (30 steps), 1e-10 on the 40x40
+    cavity with the 28x28 Newton-Cotes set (1000 steps, the configuration the FP64 parity
+    requirement names).
+  * mixed path (FP32 g/h storage) vs oracle: 1e-5 in rho, U, T (the stated mixed budget);
+    q is reported and bounded at 1e-3 of its scale.
+"""
+import numpy as np
+import pytest
+
+from conftest import field_rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _cases():
+    from dugks_amd import cases
+    return cases
+
+
+def _run_pair(case, steps, precision, fused=-1):
+    from dugks_amd import hip
+    from oracle import Oracle
+    o = Oracle(case)
+    g = hip.FvDVM(case, precision=precision, fused=fused)
+    hist_o = []
+    for _ in range(steps):
+        o.step()
+        hist_o.append(o.dt)
+    hist_g = g.run(steps, record_dt=True)
+    return o, g, np.array(hist_o), hist_g
+
+
+def _errs(o, g):
+    mo, mg = o.cell_macros(), g.cell_macros()
+    return {k: field_rel_err(mg[k], mo[k]) for k in ("rho", "U", "T", "q")}
+
+
+@pytest.mark.parametrize("fused", [0, -1])
+def test_tiny_cavity_fp64_matches_oracle(fused):
+    c = _cases().cavity_case(6, "NC38", 10, 4.0, 0.1)
+    o, g, ho, hg = _run_pair(c, 30, 0, fused)
+    e = _errs(o, g)
+    print("tiny fp64", fused, e)
+    assert np.max(np.abs(hg - ho) / ho) < 1e-12
+    for k, v in e.items():
+        assert v < 1e-12, (k, v)
+    fo, fg = o.face_macros(), g.face_macros()
+    for k in ("rho", "U", "T", "q"):
+        assert field_rel_err(fg[k], fo[k]) < 1e-12, k
+    assert field_rel_err(g.boundary_rho(), o.wall_rho()) < 1e-12
+    go, ho_ = o.df()
+    gg, hg_ = g.df()
+    assert field_rel_err(gg, go) < 1e-12
+    assert field_rel_err(hg_, ho_) < 1e-12
+
+
+def test_tiny_cavity_odd_grid_zero_normal_velocity():
+    """Odd Newton-Cotes grid: xi_d = 0 exists, so xi.Sf = 0 on axis-aligned faces and the
+    averaging branch of the internal reconstruction (fvDVM.C:847-863) is exercised."""
+    c = _cases().cavity_case(5, "NC", 9, 4.0, 0.3)
+    o, g, ho, hg = _run_pair(c, 20, 0)
+    e = _errs(o, g)
+    print("odd grid", e)
+    for k, v in e.items():
+        assert v < 1e-12, (k, v)
+
+
+def test_config0_fp64_1000_steps():
+    c = _cases().config0()
+    o, g, ho, hg = _run_pair(c, 1000, 0)
+    e = _errs(o, g)
+    print("config0 fp64 1000 steps", e, "dt rel", np.max(np.abs(hg - ho) / ho))
+    assert np.max(np.abs(hg - ho) / ho) < 1e-10
+    for k, v in e.items():
+        assert v < 1e-10, (k, v)
+
+
+def test_config0_mixed_200_steps():
+    c = _cases().config0()
+    o, g, ho, hg = _run_pair(c, 200, 1)
+    e = _errs(o, g)
+    print("config0 mixed 200 steps", e)
+    for k in ("rho", "U", "T"):
+        assert e[k] < 1e-5, (k, e[k])
+    assert e["q"] < 1e-3
+
+
+def test_determinism_and_checkpoint_resume(tmp_path):
+    from dugks_amd import hip
+    c = _cases().cavity_case(8, "NC38", 10, 4.0, 0.1)
+    a = hip.FvDVM(c)
+    a.run(12)
+    b = hip.FvDVM(c)
+    b.run(5)
+    path = tmp_path / "ck.bin"
+    b.checkpoint_save(path)
+    dt_b = b.dt
+    d = hip.FvDVM(c)
+    d.checkpoint_load(path)
+    d.dt = dt_b
+    d.run(7)
+    ga, ha = a.df()
+    gd, hd = d.df()
+    assert np.array_equal(ga, gd) and np.array_equal(ha, hd)
+    ma, md = a.cell_macros(), d.cell_macros()
+    for k in ma:
+        assert np.array_equal(ma[k], md[k]), k
