@@ -61,7 +61,7 @@ def run_gpu(case, precision, steps, warmup, fused):
     ktimes = solver.kernel_times()
     solver.profile(False)
     macros = solver.cell_macros()
-    info = dict(wall=wall, ktimes=ktimes, macros=macros, fused=bool(fused != 0))
+    info = dict(wall=wall, ktimes=ktimes, macros=macros, fused="fused_update" in ktimes)
     del solver
     return info
 

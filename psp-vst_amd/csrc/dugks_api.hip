@@ -52,7 +52,8 @@ struct dugks_handle {
     size_t es = 8;                 // bytes per stored distribution value
     bool initialised = false;
     bool fused = false;
-    FusedPlan plan;                // fused-path data (tensor velocity sets)
+    FusedPlan plan;                // fused-path data
+    std::vector<int> perm;         // internal cell -> caller's cell label
     int hist_cap = 0;
     std::vector<void *> allocs;
     // profiling
@@ -165,7 +166,10 @@ int fused_step(dugks_handle *h)
     FusedPlan &p = h->plan;
     PHASE(kPhFusedFace, Fused<S>::face_pass(g, p, (const S *)d.gT, (const S *)d.hT, d.cell, d.dt, st));
     PHASE(kPhFaceMacro, Fused<S>::face_macros(g, p, d.face, d.rho_b, d.dt, d.cfl_part, st));
-    PHASE(kPhFusedUpdate, Fused<S>::update_pass(g, p, (S *)d.gT, (S *)d.hT, d.cell, d.face, d.rho_b, d.dt, st));
+    PHASE(kPhFusedUpdate, Fused<S>::update_pass(g, p, (const S *)d.gT, (const S *)d.hT, (S *)p.gT2, (S *)p.hT2,
+                                                d.cell, d.face, d.rho_b, d.dt, st));
+    std::swap(d.gT, p.gT2);
+    std::swap(d.hT, p.hT2);
     PHASE(kPhCellMacro, Fused<S>::cell_macros(g, p, d.cell, d.dt, st));
     HIPCHK(hipGetLastError());
     return 0;
@@ -252,19 +256,43 @@ int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patch
     for (int b = 0; b < nB; ++b)
         if (bf_type[b] == -1) return fail("dugks_create: boundary face " + std::to_string(b) + " is in no patch");
 
+    // Internal cell numbering: recursive coordinate bisection into blocks of <= 256 cells
+    // (contiguous in the internal numbering).  Faces keep the caller's numbering; every
+    // cell-indexed array crossing the C-ABI is permuted on the way in and out.
+    std::vector<int> perm, block_start, inv(nC);
+    rcb_partition(nC, mesh->C, 256, perm, block_start);
+    for (int i = 0; i < nC; ++i) inv[perm[i]] = i;
+    std::vector<int> own_p(nF), nei_p(nIF);
+    for (int f = 0; f < nF; ++f) {
+        if (mesh->owner[f] < 0 || mesh->owner[f] >= nC) return fail("dugks_create: owner label out of range");
+        own_p[f] = inv[mesh->owner[f]];
+    }
+    for (int f = 0; f < nIF; ++f) {
+        if (mesh->neighbour[f] < 0 || mesh->neighbour[f] >= nC) return fail("dugks_create: neighbour label out of range");
+        nei_p[f] = inv[mesh->neighbour[f]];
+    }
+    std::vector<double> C_p(3 * (size_t)nC), V_p(nC);
+    for (int i = 0; i < nC; ++i) {
+        for (int k = 0; k < 3; ++k) C_p[3 * i + k] = mesh->C[3 * (size_t)perm[i] + k];
+        V_p[i] = mesh->V[perm[i]];
+        if (!(V_p[i] > 0)) return fail("dugks_create: non-positive cell volume");
+    }
+
     // least-squares vectors and stencils
     std::string err;
     std::vector<double> own_ls, nei_ls;
     if (mesh->ls_own && mesh->ls_nei) {
         own_ls.assign(mesh->ls_own, mesh->ls_own + 3 * (size_t)nIF);
         nei_ls.assign(mesh->ls_nei, mesh->ls_nei + 3 * (size_t)nIF);
-    } else if (!compute_ls_vectors(nC, nIF, nB, mesh->owner, mesh->neighbour, mesh->Sf, mesh->Cf, mesh->C,
-                                   mesh->weights, own_ls, nei_ls, err)) {
+    } else if (!compute_ls_vectors(nC, nIF, nB, own_p.data(), nei_p.data(), mesh->Sf, mesh->Cf, C_p.data(),
+                                   mesh->weights, inv[0], own_ls, nei_ls, err)) {
         return fail("dugks_create: " + err);
     }
     HostPrep prep;
-    if (!build_stencils(nC, nIF, nB, mesh->owner, mesh->neighbour, own_ls.data(), nei_ls.data(), prep, err))
+    if (!build_stencils(nC, nIF, nB, own_p.data(), nei_p.data(), own_ls.data(), nei_ls.data(), prep, err))
         return fail("dugks_create: " + err);
+    prep.owner_h = own_p;
+    prep.neighbour_h = nei_p;
 
     // tensor-product velocity set, ix fastest (fvDVM.C:293-373)
     std::vector<d3> xi(Nv);
@@ -297,6 +325,7 @@ int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patch
     h->opt = o;
     h->device = o.device;
     h->es = o.precision == DUGKS_MIXED ? 4 : 8;
+    h->perm = perm;
     auto bail = [&](int rc) {
         std::string m = g_err;
         dugks_destroy(h);
@@ -325,10 +354,10 @@ int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patch
     std::vector<d3> Sf(nF), Cf(nF), C(nC);
     std::memcpy(Sf.data(), mesh->Sf, sizeof(d3) * nF);
     std::memcpy(Cf.data(), mesh->Cf, sizeof(d3) * nF);
-    std::memcpy(C.data(), mesh->C, sizeof(d3) * nC);
-    if (upload(d.owner, mesh->owner, nF, st) || upload(d.neighbour, mesh->neighbour, nIF, st) ||
+    std::memcpy(C.data(), C_p.data(), sizeof(d3) * nC);
+    if (upload(d.owner, own_p.data(), nF, st) || upload(d.neighbour, nei_p.data(), nIF, st) ||
         upload(d.Sf, Sf.data(), nF, st) || upload(d.Cf, Cf.data(), nF, st) || upload(d.C, C.data(), nC, st) ||
-        upload(d.V, mesh->V, nC, st) || upload(d.cf_off, prep.cf_off.data(), nC + 1, st) ||
+        upload(d.V, V_p.data(), nC, st) || upload(d.cf_off, prep.cf_off.data(), nC + 1, st) ||
         upload(d.cf_face, prep.cf_face.data(), prep.cf_face.size(), st) ||
         upload(d.cg_off, prep.cg_off.data(), nC + 1, st) || upload(d.cg_nb, prep.cg_nb.data(), prep.cg_nb.size(), st) ||
         upload(d.cg_ls, prep.cg_ls.data(), prep.cg_ls.size(), st) || upload(d.bf_type, bf_type.data(), nB, st) ||
@@ -352,14 +381,14 @@ int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patch
     if (dalloc_bytes(h, &d.gT, ncell * h->es) || dalloc_bytes(h, &d.hT, ncell * h->es)) return bail(1);
     h->fused = o.fused != 0;
     if (h->fused) {
-        if (!fused_plan_build(h->plan, g, mesh, prep, vset, h->es, err)) {
+        if (!fused_plan_build(h->plan, g, prep, block_start, h->es, 0, err) ||
+            !fused_plan_upload(h->plan, g, h->es, err)) {
             if (o.fused == 1) return bail(fail("dugks_create: fused path unavailable: " + err));
+            fused_plan_free(h->plan);
             h->fused = false;
         }
     }
-    if (h->fused) {
-        if (!fused_plan_upload(h->plan, g, st, err)) return bail(fail("dugks_create: " + err));
-    } else {
+    if (!h->fused) {
         if (dalloc_bytes(h, &d.gB, ncell * h->es) || dalloc_bytes(h, &d.hB, ncell * h->es) ||
             dalloc_bytes(h, &d.gS, nface * h->es) || dalloc_bytes(h, &d.hS, nface * h->es))
             return bail(1);
@@ -383,8 +412,16 @@ int dugks_init_from_macros(dugks_handle *h, const double *rho, const double *U, 
     for (int c = 0; c < g.nC; ++c)
         if (!(rho[c] > 0) || !(T[c] > 0)) return fail("dugks_init_from_macros: non-positive rho or T at cell " +
                                                         std::to_string(c));
-    if (upload(d.cell.rho, rho, g.nC, st) || upload(d.cell.U, (const d3 *)U, g.nC, st) ||
-        upload(d.cell.T, T, g.nC, st))
+    std::vector<double> rp(g.nC), Tp(g.nC);
+    std::vector<d3> Up(g.nC);
+    for (int i = 0; i < g.nC; ++i) {
+        const int c = h->perm[i];
+        rp[i] = rho[c];
+        Tp[i] = T[c];
+        Up[i] = d3{U[3 * (size_t)c], U[3 * (size_t)c + 1], U[3 * (size_t)c + 2]};
+    }
+    if (upload(d.cell.rho, rp.data(), g.nC, st) || upload(d.cell.U, Up.data(), g.nC, st) ||
+        upload(d.cell.T, Tp.data(), g.nC, st))
         return 1;
     HIPCHK(hipMemsetAsync(d.cell.q, 0, sizeof(d3) * g.nC, st));
     HIPCHK(hipMemsetAsync(d.face.rho, 0, sizeof(double) * g.nF, st));
@@ -404,7 +441,7 @@ int dugks_init_from_macros(dugks_handle *h, const double *rho, const double *U, 
             launch_init_mixed_faces<double>(g, (double *)d.gS, (double *)d.hS, st);
     } else {
         std::string err;
-        if (!fused_init_boundary(h->plan, g, st, err)) return fail("dugks_init_from_macros: " + err);
+        if (!fused_init_boundary(h->plan, g, d.cell, st, err)) return fail("dugks_init_from_macros: " + err);
     }
     HIPCHK(hipMemcpyAsync(d.rho_b, g.bf_rho, sizeof(double) * g.nB, hipMemcpyDeviceToDevice, st));
     launch_init_usurf(g, d.cell, d.face, st);
@@ -478,28 +515,40 @@ int dugks_synchronize(dugks_handle *h)
 }
 
 static int get_macros(dugks_handle *h, const Macros &m, int n, double *rho, double *U, double *T, double *q,
-                      double *tau)
+                      double *tau, const std::vector<int> *perm)
 {
     hipStream_t st = h->stream;
-    if (rho) HIPCHK(hipMemcpyAsync(rho, m.rho, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-    if (U) HIPCHK(hipMemcpyAsync(U, m.U, sizeof(d3) * n, hipMemcpyDeviceToHost, st));
-    if (T) HIPCHK(hipMemcpyAsync(T, m.T, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-    if (q) HIPCHK(hipMemcpyAsync(q, m.q, sizeof(d3) * n, hipMemcpyDeviceToHost, st));
-    if (tau) HIPCHK(hipMemcpyAsync(tau, m.tau, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    double *outs[5] = {rho, U, T, q, tau};
+    const void *src[5] = {m.rho, m.U, m.T, m.q, m.tau};
+    const int width[5] = {1, 3, 1, 3, 1};
+    std::vector<double> tmp;
+    for (int k = 0; k < 5; ++k) {
+        if (!outs[k]) continue;
+        const size_t bytes = sizeof(double) * width[k] * (size_t)n;
+        if (!perm) {
+            HIPCHK(hipMemcpyAsync(outs[k], src[k], bytes, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            continue;
+        }
+        tmp.resize(width[k] * (size_t)n);
+        HIPCHK(hipMemcpyAsync(tmp.data(), src[k], bytes, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < width[k]; ++j) outs[k][(size_t)width[k] * (*perm)[i] + j] = tmp[(size_t)width[k] * i + j];
+    }
     return 0;
 }
 
 int dugks_get_cell_macros(dugks_handle *h, double *rho, double *U, double *T, double *q, double *tau)
 {
     if (check_handle(h)) return 1;
-    return get_macros(h, h->d.cell, h->g.nC, rho, U, T, q, tau);
+    return get_macros(h, h->d.cell, h->g.nC, rho, U, T, q, tau, &h->perm);
 }
 
 int dugks_get_face_macros(dugks_handle *h, double *rho, double *U, double *T, double *q, double *tau)
 {
     if (check_handle(h)) return 1;
-    return get_macros(h, h->d.face, h->g.nF, rho, U, T, q, tau);
+    return get_macros(h, h->d.face, h->g.nF, rho, U, T, q, tau, nullptr);
 }
 
 int dugks_get_boundary_rho(dugks_handle *h, double *rho_b)
@@ -514,20 +563,20 @@ int dugks_get_boundary_rho(dugks_handle *h, double *rho_b)
 int dugks_get_df(dugks_handle *h, double *g, double *hh)
 {
     if (check_handle(h)) return 1;
-    const size_t n = (size_t)h->g.Nv * h->g.nC;
+    const int nC = h->g.nC, Nv = h->g.Nv;
+    const size_t n = (size_t)Nv * nC;
     double *outs[2] = {g, hh};
     void *src[2] = {h->d.gT, h->d.hT};
     for (int k = 0; k < 2; ++k) {
         if (!outs[k]) continue;
-        if (h->es == 8) {
-            HIPCHK(hipMemcpyAsync(outs[k], src[k], n * 8, hipMemcpyDeviceToHost, h->stream));
-            HIPCHK(hipStreamSynchronize(h->stream));
-        } else {
-            std::vector<float> tmp(n);
-            HIPCHK(hipMemcpyAsync(tmp.data(), src[k], n * 4, hipMemcpyDeviceToHost, h->stream));
-            HIPCHK(hipStreamSynchronize(h->stream));
-            for (size_t i = 0; i < n; ++i) outs[k][i] = tmp[i];
-        }
+        std::vector<char> tmp(n * h->es);
+        HIPCHK(hipMemcpyAsync(tmp.data(), src[k], n * h->es, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        for (int v = 0; v < Nv; ++v)
+            for (int i = 0; i < nC; ++i) {
+                const size_t s = (size_t)v * nC + i, t = (size_t)v * nC + h->perm[i];
+                outs[k][t] = h->es == 8 ? ((const double *)tmp.data())[s] : (double)((const float *)tmp.data())[s];
+            }
     }
     return 0;
 }
@@ -536,19 +585,22 @@ int dugks_set_df(dugks_handle *h, const double *g, const double *hh)
 {
     if (check_handle(h)) return 1;
     if (!g || !hh) return fail("dugks_set_df: NULL array");
-    const size_t n = (size_t)h->g.Nv * h->g.nC;
+    const int nC = h->g.nC, Nv = h->g.Nv;
+    const size_t n = (size_t)Nv * nC;
     const double *ins[2] = {g, hh};
     void *dst[2] = {h->d.gT, h->d.hT};
     for (int k = 0; k < 2; ++k) {
-        if (h->es == 8) {
-            HIPCHK(hipMemcpyAsync(dst[k], ins[k], n * 8, hipMemcpyHostToDevice, h->stream));
-            HIPCHK(hipStreamSynchronize(h->stream));
-        } else {
-            std::vector<float> tmp(n);
-            for (size_t i = 0; i < n; ++i) tmp[i] = (float)ins[k][i];
-            HIPCHK(hipMemcpyAsync(dst[k], tmp.data(), n * 4, hipMemcpyHostToDevice, h->stream));
-            HIPCHK(hipStreamSynchronize(h->stream));
-        }
+        std::vector<char> tmp(n * h->es);
+        for (int v = 0; v < Nv; ++v)
+            for (int i = 0; i < nC; ++i) {
+                const size_t s = (size_t)v * nC + h->perm[i], t = (size_t)v * nC + i;
+                if (h->es == 8)
+                    ((double *)tmp.data())[t] = ins[k][s];
+                else
+                    ((float *)tmp.data())[t] = (float)ins[k][s];
+            }
+        HIPCHK(hipMemcpyAsync(dst[k], tmp.data(), n * h->es, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
     }
     return 0;
 }
@@ -556,12 +608,23 @@ int dugks_set_df(dugks_handle *h, const double *g, const double *hh)
 int dugks_set_cell_macros(dugks_handle *h, const double *rho, const double *U, const double *T, const double *q)
 {
     if (check_handle(h)) return 1;
+    if (!rho || !U || !T || !q) return fail("dugks_set_cell_macros: NULL field");
     const Geo &g = h->g;
     DevState &d = h->d;
-    if (upload(d.cell.rho, rho, g.nC, h->stream) || upload(d.cell.U, (const d3 *)U, g.nC, h->stream) ||
-        upload(d.cell.T, T, g.nC, h->stream) || upload(d.cell.q, (const d3 *)q, g.nC, h->stream))
+    std::vector<double> rp(g.nC), Tp(g.nC);
+    std::vector<d3> Up(g.nC), qp(g.nC);
+    for (int i = 0; i < g.nC; ++i) {
+        const size_t c = h->perm[i];
+        rp[i] = rho[c];
+        Tp[i] = T[c];
+        Up[i] = d3{U[3 * c], U[3 * c + 1], U[3 * c + 2]};
+        qp[i] = d3{q[3 * c], q[3 * c + 1], q[3 * c + 2]};
+    }
+    if (upload(d.cell.rho, rp.data(), g.nC, h->stream) || upload(d.cell.U, Up.data(), g.nC, h->stream) ||
+        upload(d.cell.T, Tp.data(), g.nC, h->stream) || upload(d.cell.q, qp.data(), g.nC, h->stream))
         return 1;
     launch_tau(g, d.cell, g.nC, h->stream);
+    if (h->fused) Fused<double>::cell_coef(g, h->plan, d.cell, h->stream);
     HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
 }
@@ -627,7 +690,15 @@ int ckpt_io(dugks_handle *h, const char *path, bool save)
 }  // namespace
 
 int dugks_checkpoint_save(dugks_handle *h, const char *path) { return ckpt_io(h, path, true); }
-int dugks_checkpoint_load(dugks_handle *h, const char *path) { return ckpt_io(h, path, false); }
+int dugks_checkpoint_load(dugks_handle *h, const char *path)
+{
+    if (ckpt_io(h, path, false)) return 1;
+    if (h->fused) {
+        Fused<double>::cell_coef(h->g, h->plan, h->d.cell, h->stream);
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
+    return 0;
+}
 
 int dugks_num_velocities(const dugks_handle *h) { return h ? h->g.Nv : 0; }
 

@@ -1,0 +1,193 @@
+// Host-side planning of the fused path: cell blocks by recursive coordinate bisection,
+// the two halo rings of each block, and the block-local tables (gradient stencils, owned
+// faces, faces of the block cells with their gather order) that the kernels follow.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "dugks_fused.h"
+
+namespace dugks {
+
+namespace {
+
+void rcb(std::vector<int> &ids, int lo, int hi, const double *C, int max_block, std::vector<int> &starts)
+{
+    const int n = hi - lo;
+    if (n <= max_block) {
+        starts.push_back(lo);
+        return;
+    }
+    double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+    for (int i = lo; i < hi; ++i)
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = std::min(mn[k], C[3 * ids[i] + k]);
+            mx[k] = std::max(mx[k], C[3 * ids[i] + k]);
+        }
+    int ax = 0;
+    for (int k = 1; k < 3; ++k)
+        if (mx[k] - mn[k] > mx[ax] - mn[ax]) ax = k;
+    // split into two parts whose sizes are multiples of max_block where possible, so that
+    // regular meshes give square / cubic blocks
+    const int nblocks = (n + max_block - 1) / max_block;
+    const int left_blocks = nblocks / 2;
+    const int mid = lo + std::min(n - 1, std::max(1, left_blocks * max_block));
+    std::nth_element(ids.begin() + lo, ids.begin() + mid, ids.begin() + hi, [&](int a, int b) {
+        const double ca = C[3 * a + ax], cb = C[3 * b + ax];
+        return ca < cb || (ca == cb && a < b);
+    });
+    // keep each half in ascending original order for locality inside a block
+    std::sort(ids.begin() + lo, ids.begin() + mid);
+    std::sort(ids.begin() + mid, ids.begin() + hi);
+    rcb(ids, lo, mid, C, max_block, starts);
+    rcb(ids, mid, hi, C, max_block, starts);
+}
+
+}  // namespace
+
+void rcb_partition(int nC, const double *C, int max_block, std::vector<int> &perm, std::vector<int> &block_start)
+{
+    perm.resize(nC);
+    std::iota(perm.begin(), perm.end(), 0);
+    block_start.clear();
+    rcb(perm, 0, nC, C, max_block, block_start);
+    block_start.push_back(nC);
+}
+
+bool fused_plan_build(FusedPlan &p, const Geo &g, const HostPrep &prep, const std::vector<int> &block_start,
+                      size_t es, int nvg_hint, std::string &err)
+{
+    p = FusedPlan();
+    if (g.D < 2) {
+        err = "fused path needs a 2-D or 3-D mesh";
+        return false;
+    }
+    const int nC = g.nC, nIF = g.nIF;
+    const int nblk = (int)block_start.size() - 1;
+    // cell adjacency comes from the gradient stencil (internal faces)
+    const std::vector<int> &cg_off = prep.cg_off, &cg_nb = prep.cg_nb;
+    const std::vector<int> &cf_off = prep.cf_off, &cf_face = prep.cf_face;
+    const std::vector<int> &owner = prep.owner_h, &neighbour = prep.neighbour_h;
+
+    p.nblk = nblk;
+    p.blk.resize(nblk);
+    std::vector<int> mark(nC, -1);
+    std::vector<int> local(nC, -1);
+    for (int b = 0; b < nblk; ++b) {
+        BlockInfo bi{};
+        const int c0 = block_start[b], c1 = block_start[b + 1];
+        bi.c0 = c0;
+        bi.nc = c1 - c0;
+        bi.l_off = (int)p.lcell.size();
+        std::vector<int> L;
+        for (int c = c0; c < c1; ++c) {
+            L.push_back(c);
+            mark[c] = b;
+        }
+        std::vector<int> h1, h2;
+        for (int c = c0; c < c1; ++c)
+            for (int k = cg_off[c]; k < cg_off[c + 1]; ++k) {
+                const int n = cg_nb[k];
+                if (mark[n] != b) {
+                    mark[n] = b;
+                    h1.push_back(n);
+                }
+            }
+        std::sort(h1.begin(), h1.end());
+        for (int c : h1)
+            for (int k = cg_off[c]; k < cg_off[c + 1]; ++k) {
+                const int n = cg_nb[k];
+                if (mark[n] != b) {
+                    mark[n] = b;
+                    h2.push_back(n);
+                }
+            }
+        std::sort(h2.begin(), h2.end());
+        L.insert(L.end(), h1.begin(), h1.end());
+        L.insert(L.end(), h2.begin(), h2.end());
+        bi.nl = (int)L.size();
+        bi.ng = bi.nc + (int)h1.size();
+        for (int l = 0; l < bi.nl; ++l) local[L[l]] = l;
+        p.lcell.insert(p.lcell.end(), L.begin(), L.end());
+        // gradient stencils of block + ring 1, ascending face label (the order of cg)
+        bi.gsb_off = (int)p.gs_beg.size();
+        for (int l = 0; l < bi.ng; ++l) {
+            const int c = L[l];
+            p.gs_beg.push_back((int)p.gs_nb.size());
+            for (int k = cg_off[c]; k < cg_off[c + 1]; ++k) {
+                const int n = cg_nb[k];
+                if (mark[n] != b || local[n] < 0 || L[local[n]] != n) {
+                    err = "internal error: stencil cell outside the halo";
+                    return false;
+                }
+                p.gs_nb.push_back(local[n]);
+                p.gs_ls.push_back(prep.cg_ls[k]);
+            }
+        }
+        p.gs_beg.push_back((int)p.gs_nb.size());
+        // owned faces (owner in block), ascending face label
+        bi.fa_off = (int)p.fa_face.size();
+        for (int c = c0; c < c1; ++c)
+            for (int k = cf_off[c]; k < cf_off[c + 1]; ++k) {
+                const int f = cf_face[k];
+                if (owner[f] != c) continue;
+                p.fa_face.push_back(f);
+                p.fa_own.push_back(local[c]);
+                p.fa_nei.push_back(f < nIF ? local[neighbour[f]] : -1);
+            }
+        bi.nfa = (int)p.fa_face.size() - bi.fa_off;
+        // faces of the block cells: unique list + per-cell gather order
+        bi.fb_off = (int)p.fb_face.size();
+        std::unordered_map<int, int> fb_index;
+        bi.fbc_off = (int)p.fbc_beg.size();
+        for (int c = c0; c < c1; ++c) {
+            p.fbc_beg.push_back((int)p.fbc_idx.size());
+            for (int k = cf_off[c]; k < cf_off[c + 1]; ++k) {
+                const int f = cf_face[k];
+                auto it = fb_index.find(f);
+                int idx;
+                if (it == fb_index.end()) {
+                    idx = (int)p.fb_face.size() - bi.fb_off;
+                    fb_index.emplace(f, idx);
+                    p.fb_face.push_back(f);
+                    p.fb_own.push_back(local[owner[f]]);
+                    p.fb_nei.push_back(f < nIF ? local[neighbour[f]] : -1);
+                } else {
+                    idx = it->second;
+                }
+                p.fbc_idx.push_back(idx);
+                // the owner subtracts the flux, the neighbour adds it (fvDVM.C:1493-1496)
+                p.fbc_sgn.push_back((signed char)((f < nIF && owner[f] != c) ? 1 : -1));
+            }
+        }
+        p.fbc_beg.push_back((int)p.fbc_idx.size());
+        bi.nfb = (int)p.fb_face.size() - bi.fb_off;
+        for (int l = 0; l < bi.nl; ++l) local[L[l]] = -1;
+        p.blk[b] = bi;
+        p.max_nl = std::max(p.max_nl, bi.nl);
+        p.max_ng = std::max(p.max_ng, bi.ng);
+        p.max_nfa = std::max(p.max_nfa, bi.nfa);
+        p.max_nfb = std::max(p.max_nfb, bi.nfb);
+        p.max_nc = std::max(p.max_nc, bi.nc);
+    }
+    if (p.max_nc > kFusedThreadsB || p.max_nfa > kFusedThreadsA * kFusedSlotsA) {
+        err = "block tables exceed the work-group shape (cells " + std::to_string(p.max_nc) + ", owned faces " +
+              std::to_string(p.max_nfa) + ")";
+        return false;
+    }
+    p.nm = g.D == 3 ? 17 : 11;
+    p.nma = p.nm + 1;
+    // velocity groups: enough work-groups to fill 256 CUs several times, few partials
+    int nvg = nvg_hint > 0 ? nvg_hint : std::max(1, std::min(64, (2048 + nblk - 1) / nblk));
+    nvg = std::min(nvg, g.Nv);
+    p.vper = (g.Nv + nvg - 1) / nvg;
+    p.nvg = (g.Nv + p.vper - 1) / p.vper;
+    (void)es;
+    p.ready = true;
+    return true;
+}
+
+}  // namespace dugks

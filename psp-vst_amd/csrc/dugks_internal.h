@@ -81,11 +81,12 @@ struct HostPrep {
     std::vector<int> cf_off, cf_face, cg_off, cg_nb, bf_type, bf_local;
     std::vector<d3> cg_ls;
     std::vector<double> own_ls, nei_ls; // [nIF*3]
+    std::vector<int> owner_h, neighbour_h; // host copies of owner / neighbour (internal numbering)
 };
 
 bool compute_ls_vectors(int nC, int nIF, int nB, const int *owner, const int *neighbour, const double *Sf,
-                        const double *Cf, const double *C, const double *weights, std::vector<double> &own_ls,
-                        std::vector<double> &nei_ls, std::string &err);
+                        const double *Cf, const double *C, const double *weights, int first_cell,
+                        std::vector<double> &own_ls, std::vector<double> &nei_ls, std::string &err);
 bool build_stencils(int nC, int nIF, int nB, const int *owner, const int *neighbour, const double *own_ls,
                     const double *nei_ls, HostPrep &p, std::string &err);
 

@@ -51,10 +51,11 @@ double msq(const double *a) { return a[0] * a[0] + a[1] * a[1] + a[2] * a[2]; }
 //   dd_P += |Sf|/|d_b|^2 d_b d_b,  d_b = n (n.(Cf - C_P))        (non-coupled boundary faces)
 //   ownLs = (1-w) |Sf|/|d|^2 dd_P^-1 d,  neiLs = -w |Sf|/|d|^2 dd_N^-1 d
 // dd^-1 follows OpenFOAM inv(symmTensorField): directions negligible in the first cell's
-// tensor (one-cell-thick meshes) are regularised with +1 before and -1 after inversion.
+// tensor (one-cell-thick meshes) are regularised with +1 before and -1 after inversion;
+// first_cell is the internal label of the caller's cell 0.
 bool compute_ls_vectors(int nC, int nIF, int nB, const int *owner, const int *neighbour, const double *Sf,
-                        const double *Cf, const double *C, const double *weights, std::vector<double> &own_ls,
-                        std::vector<double> &nei_ls, std::string &err)
+                        const double *Cf, const double *C, const double *weights, int first_cell,
+                        std::vector<double> &own_ls, std::vector<double> &nei_ls, std::string &err)
 {
     std::vector<Symm> dd(nC);
     for (int f = 0; f < nIF; ++f) {
@@ -91,7 +92,7 @@ bool compute_ls_vectors(int nC, int nIF, int nB, const int *owner, const int *ne
     }
     bool rm[3] = {false, false, false};
     if (nC > 0) {
-        const Symm &t = dd[0];
+        const Symm &t = dd[first_cell];
         const double scale = t.xx * t.xx + 2 * t.xy * t.xy + 2 * t.xz * t.xz + t.yy * t.yy + 2 * t.yz * t.yz + t.zz * t.zz;
         rm[0] = (t.xx * t.xx) / scale < kSmall;
         rm[1] = (t.yy * t.yy) / scale < kSmall;

@@ -38,7 +38,7 @@ def _errs(o, g):
     return {k: field_rel_err(mg[k], mo[k]) for k in ("rho", "U", "T", "q")}
 
 
-@pytest.mark.parametrize("fused", [0, -1])
+@pytest.mark.parametrize("fused", [0, 1])
 def test_tiny_cavity_fp64_matches_oracle(fused):
     c = _cases().cavity_case(6, "NC38", 10, 4.0, 0.1)
     o, g, ho, hg = _run_pair(c, 30, 0, fused)
@@ -57,20 +57,22 @@ def test_tiny_cavity_fp64_matches_oracle(fused):
     assert field_rel_err(hg_, ho_) < 1e-12
 
 
-def test_tiny_cavity_odd_grid_zero_normal_velocity():
+@pytest.mark.parametrize("fused", [0, 1])
+def test_tiny_cavity_odd_grid_zero_normal_velocity(fused):
     """Odd Newton-Cotes grid: xi_d = 0 exists, so xi.Sf = 0 on axis-aligned faces and the
     averaging branch of the internal reconstruction (fvDVM.C:847-863) is exercised."""
     c = _cases().cavity_case(5, "NC", 9, 4.0, 0.3)
-    o, g, ho, hg = _run_pair(c, 20, 0)
+    o, g, ho, hg = _run_pair(c, 20, 0, fused)
     e = _errs(o, g)
     print("odd grid", e)
     for k, v in e.items():
         assert v < 1e-12, (k, v)
 
 
-def test_config0_fp64_1000_steps():
+@pytest.mark.parametrize("fused", [0, 1])
+def test_config0_fp64_1000_steps(fused):
     c = _cases().config0()
-    o, g, ho, hg = _run_pair(c, 1000, 0)
+    o, g, ho, hg = _run_pair(c, 1000, 0, fused)
     e = _errs(o, g)
     print("config0 fp64 1000 steps", e, "dt rel", np.max(np.abs(hg - ho) / ho))
     assert np.max(np.abs(hg - ho) / ho) < 1e-10
@@ -78,9 +80,10 @@ def test_config0_fp64_1000_steps():
         assert v < 1e-10, (k, v)
 
 
-def test_config0_mixed_200_steps():
+@pytest.mark.parametrize("fused", [0, 1])
+def test_config0_mixed_200_steps(fused):
     c = _cases().config0()
-    o, g, ho, hg = _run_pair(c, 200, 1)
+    o, g, ho, hg = _run_pair(c, 200, 1, fused)
     e = _errs(o, g)
     print("config0 mixed 200 steps", e)
     for k in ("rho", "U", "T"):
@@ -88,17 +91,18 @@ def test_config0_mixed_200_steps():
     assert e["q"] < 1e-3
 
 
-def test_determinism_and_checkpoint_resume(tmp_path):
+@pytest.mark.parametrize("fused", [0, 1])
+def test_determinism_and_checkpoint_resume(tmp_path, fused):
     from dugks_amd import hip
-    c = _cases().cavity_case(8, "NC38", 10, 4.0, 0.1)
-    a = hip.FvDVM(c)
+    c = _cases().cavity_case(20, "NC38", 10, 4.0, 0.1)
+    a = hip.FvDVM(c, fused=fused)
     a.run(12)
-    b = hip.FvDVM(c)
+    b = hip.FvDVM(c, fused=fused)
     b.run(5)
     path = tmp_path / "ck.bin"
     b.checkpoint_save(path)
     dt_b = b.dt
-    d = hip.FvDVM(c)
+    d = hip.FvDVM(c, fused=fused)
     d.checkpoint_load(path)
     d.dt = dt_b
     d.run(7)
@@ -108,3 +112,31 @@ def test_determinism_and_checkpoint_resume(tmp_path):
     ma, md = a.cell_macros(), d.cell_macros()
     for k in ma:
         assert np.array_equal(ma[k], md[k]), k
+
+
+def test_fused_and_staged_agree_on_multiblock_mesh():
+    """A 48x48 cavity gives nine RCB blocks, so block halos, ring-2 cells and faces shared
+    between blocks are all exercised; FP64 fused vs staged after 50 steps."""
+    from dugks_amd import hip
+    c = _cases().cavity_case(48, "NC38", 13, 4.0, 0.05)
+    a = hip.FvDVM(c, fused=0)
+    b = hip.FvDVM(c, fused=1)
+    a.run(50)
+    b.run(50)
+    ma, mb = a.cell_macros(), b.cell_macros()
+    for k in ("rho", "U", "T", "q"):
+        assert field_rel_err(mb[k], ma[k]) < 1e-12, k
+    fa, fb = a.face_macros(), b.face_macros()
+    for k in ("rho", "U", "T", "q"):
+        assert field_rel_err(fb[k], fa[k]) < 1e-12, k
+
+
+def test_fused_fp64_is_bitwise_deterministic():
+    from dugks_amd import hip
+    c = _cases().cavity_case(40, "NC38", 16, 4.0, 0.1)
+    runs = []
+    for _ in range(2):
+        s = hip.FvDVM(c, fused=1)
+        s.run(20)
+        runs.append(s.df())
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
