@@ -219,6 +219,47 @@ def smoothed_wall_profile(n: int, seed: int = 12345, amp: float = 0.05, width: f
     return 1.0 + amp * zs
 
 
+def sphere_case(mesh_kw: dict, rule: str, n_dv: int, vel_scale: float, Kn: float, Ma: float = 0.2,
+                wall_T=1.0, name: str | None = None) -> Case:
+    """Flow past a sphere (diameter 1), nondimensional (R = T0 = rho0 = 1, L = D).
+
+    Freestream Ma = U / sqrt(gamma R T0), gamma = 5/3, on the inlet / outlet / side patches
+    (rho fixedValue -> DF mixed: incoming half frozen at the freestream Maxwellian), and a
+    diffuse wall (rho calculatedMaxwell) at temperature ``wall_T`` (scalar or per face) on
+    the sphere.  Uniform freestream initial state.
+    """
+    from . import sphere
+    m = sphere.sphere_box_mesh(**mesh_kw)
+    geo = meshmod.geometry(m)
+    R = T0 = 1.0
+    U_inf = Ma * math.sqrt(5.0 / 3.0 * R * T0)
+    gas = Gas(R=R, omega=0.81, Tref=T0, muRef=mu_ref_from_kn(Kn, omega=0.81, L=1.0), Pr=2.0 / 3.0, KInner=0)
+    scale = vel_scale * math.sqrt((2.0 if rule.startswith("GH") else 1.0) * R * T0)
+    xis, wts = quadrature.make_rule(rule, scale, n_dv)
+    nC = m.n_cells
+    bcs = []
+    for p in m.patches:
+        if p.name == "sphere":
+            bcs.append(PatchBC(p.name, "calculatedMaxwell", np.ones(p.size), np.zeros((p.size, 3)),
+                               np.broadcast_to(np.asarray(wall_T, float), (p.size,)).copy()))
+        else:
+            bcs.append(PatchBC(p.name, "fixedValue", np.ones(p.size), np.tile([U_inf, 0.0, 0.0], (p.size, 1)),
+                               np.full(p.size, T0)))
+    return Case(name=name or f"sphere{nC}", mesh=m, geo=geo, D=3, xis=xis, wts=wts,
+                xi_max=float(np.max(np.abs(xis))), xi_min=float(np.min(xis)), gas=gas,
+                rho0=np.ones(nC), U0=np.tile([U_inf, 0.0, 0.0], (nC, 1)), T0=np.full(nC, T0), bcs=bcs,
+                meta=dict(Kn=Kn, Ma=Ma, U_inf=U_inf, rule=rule, n_dv=n_dv))
+
+
+SPHERE_200K = dict(nc=24, nr=12, n_up=14, n_down=30, n_side=14)
+SPHERE_50K = dict(nc=16, nr=8, n_up=8, n_down=18, n_side=8)
+
+
+def config2(n_dv: int = 24) -> Case:
+    """Sphere, ~211k hex cells, 24^3 half-range Gauss-Hermite set, Kn = 0.1, Ma = 0.2."""
+    return sphere_case(SPHERE_200K, "GH", n_dv, 1.0, 0.1, name=f"sphere211k_gh{n_dv}_kn0.1")
+
+
 def config3() -> Case:
     """100x100 cavity, 32x32 trapezoid on [-4, 4], Kn = 0.1, seeded bottom-wall profile."""
     Tw = smoothed_wall_profile(100)

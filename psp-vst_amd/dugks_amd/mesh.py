@@ -339,6 +339,81 @@ def cavity(N: int, L: float = 1.0, dz: float = 0.1) -> PolyMesh:
                     patches=patches, n_cells=N * N)
 
 
+# OpenFOAM hex model faces (vertex order 0-3 bottom, 4-7 top), outward for a right-handed hex
+_HEX_FACES = np.array([[0, 4, 7, 3], [1, 2, 6, 5], [0, 1, 5, 4], [3, 7, 6, 2], [0, 3, 2, 1], [4, 5, 6, 7]])
+
+
+def hex_mesh(points: np.ndarray, cells: np.ndarray, patch_names, patch_types, classify) -> PolyMesh:
+    """Vectorised polyMesh assembly from hexahedra.
+
+    ``cells`` [nC, 8] are point labels in OpenFOAM hex order.  Shared faces are matched by
+    their sorted point labels.  ``classify(face_xyz [k, 4, 3]) -> int array`` assigns every
+    boundary face to a patch index (into ``patch_names``), or -1 to raise.  Internal faces
+    are ordered upper-triangularly, boundary faces patch by patch and by owner; every face
+    is oriented out of its owner.
+    """
+    points = np.asarray(points, dtype=np.float64)
+    cells = np.asarray(cells, dtype=np.int64)
+    nC = len(cells)
+    fp = cells[:, _HEX_FACES].reshape(-1, 4)                 # [6nC, 4]
+    fc = np.repeat(np.arange(nC), 6)
+    key = np.sort(fp, axis=1)
+    order = np.lexsort(key.T[::-1])
+    ks = key[order]
+    same = np.all(ks[1:] == ks[:-1], axis=1)
+    first = np.nonzero(same)[0]
+    if np.any(same[1:] & same[:-1]):
+        raise ValueError("a face is shared by more than two cells")
+    a, b = order[first], order[first + 1]
+    internal_mask = np.zeros(len(fp), dtype=bool)
+    internal_mask[a] = True
+    internal_mask[b] = True
+    ca, cb = fc[a], fc[b]
+    own = np.minimum(ca, cb)
+    nei = np.maximum(ca, cb)
+    fpts = np.where((ca < cb)[:, None], fp[a], fp[b])
+    cavg = points[cells].mean(axis=1)
+
+    def normals(q):
+        xyz = points[q]
+        ctr = xyz.mean(axis=1)
+        n = np.zeros((len(q), 3))
+        for k in range(4):
+            n += np.cross(xyz[:, (k + 1) % 4] - xyz[:, k], ctr - xyz[:, k])
+        return n, ctr
+
+    n, _ = normals(fpts)
+    flip = np.sum(n * (cavg[nei] - cavg[own]), axis=1) < 0
+    fpts[flip] = fpts[flip][:, ::-1]
+    io = np.lexsort((nei, own))
+    own, nei, fpts = own[io], nei[io], fpts[io]
+
+    bidx = np.nonzero(~internal_mask)[0]
+    bpts = fp[bidx]
+    bown = fc[bidx]
+    n, ctr = normals(bpts)
+    flip = np.sum(n * (ctr - cavg[bown]), axis=1) < 0
+    bpts[flip] = bpts[flip][:, ::-1]
+    pid = np.asarray(classify(points[bpts]))
+    if np.any(pid < 0):
+        raise ValueError(f"{int(np.sum(pid < 0))} boundary faces match no patch")
+    face_list = [fpts]
+    owner = [own]
+    patches = []
+    start = len(own)
+    for k, (name, typ) in enumerate(zip(patch_names, patch_types)):
+        sel = np.nonzero(pid == k)[0]
+        sel = sel[np.argsort(bown[sel], kind="stable")]
+        face_list.append(bpts[sel])
+        owner.append(bown[sel])
+        patches.append(Patch(name, typ, start, len(sel)))
+        start += len(sel)
+    allf = np.concatenate(face_list)
+    return PolyMesh(points=points, face_offsets=np.arange(len(allf) + 1, dtype=np.int64) * 4,
+                    face_points=allf.ravel().astype(np.int64), owner=np.concatenate(owner).astype(np.int64),
+                    neighbour=nei.astype(np.int64), patches=patches, n_cells=nC)
+
+
 def cell_faces(mesh: PolyMesh):
     """CSR list of the faces of each cell, faces in ascending label order."""
     nC = mesh.n_cells

@@ -140,3 +140,27 @@ def test_fused_fp64_is_bitwise_deterministic():
         s.run(20)
         runs.append(s.df())
     assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+
+
+SMALL_SPHERE = dict(nc=4, nr=2, n_up=2, n_down=3, n_side=2)
+
+
+@pytest.mark.parametrize("fused", [0, 1])
+def test_small_sphere_fp64_matches_oracle(fused):
+    """3-D hex mesh with a diffuse sphere wall and mixed inlet/outlet/side patches."""
+    c = _cases().sphere_case(SMALL_SPHERE, "GH", 6, 1.0, 0.1)
+    o, g, ho, hg = _run_pair(c, 15, 0, fused)
+    e = _errs(o, g)
+    print("small sphere fp64", fused, c.n_cells, e)
+    for k, v in e.items():
+        assert v < 1e-12, (k, v)
+    assert field_rel_err(g.boundary_rho(), o.wall_rho()) < 1e-12
+
+
+def test_small_sphere_mixed_matches_oracle():
+    c = _cases().sphere_case(SMALL_SPHERE, "GH", 6, 1.0, 0.1)
+    o, g, ho, hg = _run_pair(c, 15, 1, 1)
+    e = _errs(o, g)
+    print("small sphere mixed", e)
+    for k in ("rho", "U", "T"):
+        assert e[k] < 1e-5, (k, e[k])
