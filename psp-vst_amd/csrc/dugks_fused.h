@@ -27,6 +27,27 @@ namespace dugks {
 constexpr int kFusedThreadsA = 512;   // pass A threads
 constexpr int kFusedSlotsA = 2;       // owned faces per pass-A thread
 constexpr int kFusedThreadsB = 256;   // pass B threads = max cells per block
+constexpr int kFusedMaxNb = 8;        // max internal faces per gradient cell / faces per block cell
+constexpr int kFusedXcds = 8;         // accelerator dies; work-group ids are dealt round-robin over them
+// local cells per thread in stage 1 (pass A / pass B) and block faces per thread in pass B
+constexpr int fused_cells_per_thread_a(int D) { return D == 2 ? 1 : 2; }
+constexpr int fused_cells_per_thread_b(int D) { return D == 2 ? 2 : 4; }
+constexpr int fused_faces_per_thread_b(int D) { return D == 2 ? 3 : 4; }
+
+// Equilibrium coefficients of one cell or face, formed once per step by the macro kernels
+// so that the velocity loops need no division:
+//   A = rho / (2 pi R T)^(D/2), U, invRT = 1 / (R T), RTh = R T (faces on patches: the
+//   temperature that multiplies h, see q1_compat), q5 = q / (5 rho (R T)^2),
+//   tr = tau (cells) or the face relaxation weight h/(2 tau + h) (faces).
+struct alignas(16) EqCoef {
+    double A, ux, uy, uz, invRT, RTh, qx, qy, qz, tr, pad0, pad1;
+};
+
+// Incoming half of wall / inlet faces: g = s * A * exp(-|U - xi|^2 invRT / 2), h = g * hfac,
+// s = rho_w for diffuse walls and 1 for inlets (A then includes the fixed patch rho).
+struct alignas(16) BndCoef {
+    double A, ux, uy, uz, invRT, hfac, pad0, pad1;
+};
 
 struct BlockInfo {
     int c0, nc;        // block cells: permuted ids [c0, c0 + nc)
@@ -41,8 +62,13 @@ struct BlockInfo {
 
 struct FusedPlan {
     bool ready = false;
-    int nblk = 0, nvg = 0, vper = 0;   // blocks, velocity groups, velocities per group
+    int nblk = 0, nvg = 0;             // blocks, velocity groups
     int vc = 0;                        // velocities per LDS chunk
+    // Velocities are visited in chunks (x-stripe of vc abscissae, row = (iy, iz)); chunk id
+    // ch -> stripe = ch / nrow, row = ch % nrow; a velocity group is a contiguous chunk range.
+    int n1 = 0, nrow = 0, nstripe = 0, nchunk = 0, cpg = 0;
+    int b8 = 0;                        // blocks per XCD in the work-group mapping
+    int wg_target = 4096;              // work-groups per pass the velocity grouping aims at
     int max_nl = 0, max_ng = 0, max_nfa = 0, max_nfb = 0, max_nc = 0;
     int nm = 0, nma = 0;               // raw moments per cell / per face (+ outgoing wall flux)
     size_t lds_a = 0, lds_b = 0;       // dynamic LDS bytes of the two passes
@@ -61,7 +87,8 @@ struct FusedPlan {
     double *fpart = nullptr;           // [nvg][nF][nma]
     double *cpart = nullptr;           // [nvg][nC][nm]
     double *bin = nullptr;             // [nB][nm] incoming moments (walls: per unit density)
-    double *cA = nullptr, *fA = nullptr;  // rho / (2 pi R T)^(D/2) of cells / faces
+    EqCoef *cc = nullptr, *fc = nullptr;  // equilibrium coefficients of cells / faces
+    BndCoef *bc = nullptr;                // incoming-half coefficients of boundary faces
     void *gT2 = nullptr, *hT2 = nullptr;  // second distribution buffer
     std::vector<void *> allocs;
 };
@@ -72,6 +99,8 @@ void rcb_partition(int nC, const double *C, int max_block, std::vector<int> &per
 
 bool fused_plan_build(FusedPlan &p, const Geo &g, const HostPrep &prep, const std::vector<int> &block_start,
                       size_t es, int nvg_hint, std::string &err);
+// velocity chunking: vc velocities per chunk, chunk groups sized for about wg_target work-groups
+void fused_velocity_groups(FusedPlan &p, int D, int n1, int vc);
 bool fused_plan_upload(FusedPlan &p, const Geo &g, size_t es, std::string &err);
 bool fused_init_boundary(FusedPlan &p, const Geo &g, const Macros &cm, hipStream_t st, std::string &err);
 void fused_plan_free(FusedPlan &p);

@@ -173,6 +173,20 @@ bool fused_plan_build(FusedPlan &p, const Geo &g, const HostPrep &prep, const st
         p.max_nfb = std::max(p.max_nfb, bi.nfb);
         p.max_nc = std::max(p.max_nc, bi.nc);
     }
+    int max_st = 0, max_cf = 0;
+    for (size_t i = 0; i + 1 < p.gs_beg.size(); ++i) max_st = std::max(max_st, p.gs_beg[i + 1] - p.gs_beg[i]);
+    for (size_t i = 0; i + 1 < p.fbc_beg.size(); ++i) max_cf = std::max(max_cf, p.fbc_beg[i + 1] - p.fbc_beg[i]);
+    if (max_st > kFusedMaxNb || max_cf > kFusedMaxNb) {
+        err = "cells with more than " + std::to_string(kFusedMaxNb) + " faces";
+        return false;
+    }
+    if (p.max_nl > kFusedThreadsA * fused_cells_per_thread_a(g.D) ||
+        p.max_nl > kFusedThreadsB * fused_cells_per_thread_b(g.D) ||
+        p.max_nfb > kFusedThreadsB * fused_faces_per_thread_b(g.D)) {
+        err = "block halos exceed the work-group shape (local cells " + std::to_string(p.max_nl) + ", faces " +
+              std::to_string(p.max_nfb) + ")";
+        return false;
+    }
     if (p.max_nc > kFusedThreadsB || p.max_nfa > kFusedThreadsA * kFusedSlotsA) {
         err = "block tables exceed the work-group shape (cells " + std::to_string(p.max_nc) + ", owned faces " +
               std::to_string(p.max_nfa) + ")";
@@ -180,11 +194,7 @@ bool fused_plan_build(FusedPlan &p, const Geo &g, const HostPrep &prep, const st
     }
     p.nm = g.D == 3 ? 17 : 11;
     p.nma = p.nm + 1;
-    // velocity groups: enough work-groups to fill 256 CUs several times, few partials
-    int nvg = nvg_hint > 0 ? nvg_hint : std::max(1, std::min(64, (2048 + nblk - 1) / nblk));
-    nvg = std::min(nvg, g.Nv);
-    p.vper = (g.Nv + nvg - 1) / nvg;
-    p.nvg = (g.Nv + p.vper - 1) / p.vper;
+    p.wg_target = nvg_hint > 0 ? nvg_hint * nblk : 4096;
     (void)es;
     p.ready = true;
     return true;

@@ -19,6 +19,19 @@ struct d3 {
 };
 
 __host__ __device__ inline double dot(const d3 &a, const d3 &b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// xi . Sf with each product and sum rounded on its own, (x x' + y y') + z z', exactly as
+// OpenFOAM's vector inner product evaluates it on the host.  The upwind tests branch on its
+// sign (fvDVM.C:829, 840, 950, 1079, 1431), and velocities tangent to a face give
+// sigma = +-1e-19 round-off: a fused multiply-add there would flip the branch.
+__device__ inline double sigma_dot(const d3 &xi, const d3 &Sf)
+{
+#pragma clang fp contract(off)
+    const double px = xi.x * Sf.x;
+    const double py = xi.y * Sf.y;
+    const double pz = xi.z * Sf.z;
+    return (px + py) + pz;
+}
 __host__ __device__ inline double magsqr(const d3 &a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
 
 enum PatchType : int { kMaxwellWall = 1, kMixed = 2 };

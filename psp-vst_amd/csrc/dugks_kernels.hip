@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(kBlock) k_init_incoming(Geo g, double *in_by_r
         const double Tb = g.bf_T[b];
         for (int v = 0; v < g.Nv; ++v) {
             const d3 xi = g.xi[v];
-            const double xs = dot(xi, Sf);
+            const double xs = sigma_dot(xi, Sf);
             if (xs < 0) acc += -g.w[v] * xs * maxwell_by_rho(g, xi, Ub, Tb);
         }
     }
@@ -173,7 +173,7 @@ __global__ void __launch_bounds__(kBlock)
     const double dt = *dtp;
     const d3 xi = g.xi[v];
     const d3 Cf = g.Cf[f];
-    const double xs = dot(xi, g.Sf[f]);
+    const double xs = sigma_dot(xi, g.Sf[f]);
     const S *gv = gB + (size_t)v * g.nC;
     const S *hv = hB + (size_t)v * g.nC;
     const size_t k = (size_t)v * g.nF + f;
@@ -214,7 +214,7 @@ __global__ void __launch_bounds__(kBlock) k_wall_rho(Geo g, const S *__restrict_
     const d3 Sf = g.Sf[f];
     double out = 0.0;
     for (int v = 0; v < g.Nv; ++v) {
-        const double xs = dot(g.xi[v], Sf);
+        const double xs = sigma_dot(g.xi[v], Sf);
         if (xs > 0) out += g.w[v] * xs * (double)gS[(size_t)v * g.nF + f];
     }
     rho_b[b] = out / fabs(g.in_by_rho[b]);
@@ -229,7 +229,7 @@ __global__ void __launch_bounds__(kBlock) k_wall_in(Geo g, const double *rho_b, 
     if (b >= g.nB || g.bf_type[b] != kMaxwellWall) return;
     const int f = g.nIF + b;
     const d3 xi = g.xi[v];
-    if (dot(xi, g.Sf[f]) <= 0) {
+    if (sigma_dot(xi, g.Sf[f]) <= 0) {
         const double Tb = g.bf_T[b];
         const double gw = rho_b[b] * maxwell_by_rho(g, xi, g.bf_U[b], Tb);
         const size_t k = (size_t)v * g.nF + f;
@@ -291,7 +291,7 @@ __global__ void __launch_bounds__(kBlock) k_face_relax(Geo g, Macros fm, S *gS, 
     if (f >= g.nF) return;
     const d3 xi = g.xi[v];
     const bool bnd = f >= g.nIF;
-    if (bnd && !(dot(xi, g.Sf[f]) > 0)) return;  // patches: outgoing only (fvDVM.C:1431)
+    if (bnd && !(sigma_dot(xi, g.Sf[f]) > 0)) return;  // patches: outgoing only (fvDVM.C:1431)
     const double dt = *dtp;
     const double T = fm.T[f];
     const double Th = (bnd && g.q1) ? fm.T[g.bf_local[f - g.nIF]] : T;
@@ -322,7 +322,7 @@ __global__ void __launch_bounds__(kBlock)
     const S *hv = hS + (size_t)v * g.nF;
     for (int k = g.cf_off[c]; k < g.cf_off[c + 1]; ++k) {
         const int f = g.cf_face[k];
-        const double xs = dot(xi, g.Sf[f]);
+        const double xs = sigma_dot(xi, g.Sf[f]);
         const double fg = xs * (double)gv[f] * dt / V;
         const double fh = xs * (double)hv[f] * dt / V;
         if (f < g.nIF && g.owner[f] != c) {

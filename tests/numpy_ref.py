@@ -40,7 +40,10 @@ class NumpyDUGKS:
         for t, s, z in zip(a["patch_type"], a["patch_start"], a["patch_size"]):
             self.btype[s:s + z] = t
             self.blocal[s:s + z] = np.arange(z)
-        self.sigma = self.xi @ self.Sf.T            # [Nv, nF]
+        # xi . Sf as (x x' + y y') + z z', each operation rounded on its own (the sign of
+        # round-off-sized values decides the upwind side for velocities tangent to a face)
+        X, S = self.xi, self.Sf
+        self.sigma = (X[:, None, 0] * S[None, :, 0] + X[:, None, 1] * S[None, :, 1]) + X[:, None, 2] * S[None, :, 2]
         self._ls_matrix()
 
     # ---- least squares: grad_c = A_c^-1 sum_f wt_f d_f (phi_nb - phi_c)

@@ -54,6 +54,29 @@ def test_oracle_matches_independent_numpy_restatement():
     assert field_rel_err(o.wall_rho(), n.rho_w) < 1e-12
 
 
+def test_oracle_matches_numpy_on_small_sphere():
+    """3-D hex mesh, diffuse sphere wall, frozen-incoming inlet/outlet/side patches, and
+    velocities tangent to shell faces (xi.Sf = +-1e-19 round-off decides the upwind side)."""
+    from numpy_ref import NumpyDUGKS
+    from oracle import Oracle
+    from dugks_amd.timecontrol import next_delta_t
+    c = _cases().sphere_case(dict(nc=4, nr=2, n_up=2, n_down=3, n_side=2), "GH", 6, 1.0, 0.1)
+    o = Oracle(c)
+    n = NumpyDUGKS(c)
+    n.init()
+    dt = c.controls.deltaT
+    for _ in range(5):
+        dt = next_delta_t(dt, n.co_num(dt), c.controls)
+        n.step(dt)
+        o.step()
+    m = o.cell_macros()
+    for k, v in (("rho", n.rho), ("U", n.U), ("T", n.T)):
+        assert field_rel_err(m[k], v) < 1e-13, k
+    assert field_rel_err(m["q"], n.q) < 1e-12
+    wall = n.btype == 1
+    assert field_rel_err(o.wall_rho()[wall], n.rho_w[wall]) < 1e-13
+
+
 def _distorted_cavity(N, seed=3):
     from dugks_amd import mesh as meshmod
     m = meshmod.cavity(N)
