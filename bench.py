@@ -12,6 +12,17 @@ HBM roofline of the dominant kernel and a CPU baseline (the FP64 oracle on the h
 cores, a bounded sample) ride along.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-steps S] [--no-cpu]
+
+HBM traffic of the dominant kernel (``roofline.traffic``) comes from two rocprofv3 counter
+passes of this script with ``--copy-probe-mib M`` (FETCH_SIZE, then WRITE_SIZE); the probe
+is a plain device-to-device copy of M MiB whose counter readings calibrate the counters
+(bytes moved / bytes counted).  ``--pmc-summary`` turns the two passes into
+profiles/hbm_traffic_<workload>.json, which later runs read:
+
+    rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT/fetch -o run -- python3 bench.py \
+        --steps 2 --warmup 1 --no-cpu --no-fp64 --copy-probe-mib 1024
+    rocprofv3 --pmc WRITE_SIZE ... (same)
+    python bench.py --pmc-summary OUT/fetch/run_counter_collection.csv OUT/write/run_counter_collection.csv
 """
 from __future__ import annotations
 
@@ -25,6 +36,69 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "psp-vst_amd"))
 
 HBM_PEAK_GBS = 8000.0
+KERNEL_SYMBOLS = {"fused_update": "k_pass_b", "fused_face": "k_pass_a", "face_recon": "face_recon",
+                  "update": "k_update", "bar": "k_bar", "cell_moments": "k_cell_moments"}
+COPY_SYMBOL = "__amd_rocclr_copyBuffer"
+
+
+def workload_tag(cells: int, ndv: int, kn: float) -> str:
+    return f"cavity{cells}_v{ndv}_kn{kn:g}"
+
+
+def traffic_file(tag: str) -> str:
+    return os.path.join(ROOT, "profiles", f"hbm_traffic_{tag}.json")
+
+
+def copy_probe(mib: int) -> None:
+    """A plain device-to-device copy of mib MiB (counter calibration for the traffic figure),
+    issued through the HIP runtime libdugks_hip.so is linked against (no second runtime)."""
+    import ctypes as ct
+    from dugks_amd import hip
+    hip.load_library()
+    rt = ct.CDLL("libamdhip64.so.7")
+    n = ct.c_size_t(mib << 20)
+    src, dst = ct.c_void_p(), ct.c_void_p()
+    if rt.hipMalloc(ct.byref(src), n) or rt.hipMalloc(ct.byref(dst), n):
+        raise RuntimeError("copy probe: hipMalloc failed")
+    try:
+        if rt.hipMemset(src, 1, n) or rt.hipDeviceSynchronize():
+            raise RuntimeError("copy probe: hipMemset failed")
+        if rt.hipMemcpy(dst, src, n, 3) or rt.hipDeviceSynchronize():  # 3 = device to device
+            raise RuntimeError("copy probe: hipMemcpy failed")
+    finally:
+        rt.hipFree(src)
+        rt.hipFree(dst)
+
+
+def pmc_summary(fetch_csv: str, write_csv: str, tag: str, mib: int) -> dict:
+    """Average FETCH_SIZE / WRITE_SIZE (KiB per dispatch) per kernel, scaled by the copy probe."""
+    import csv
+    import collections
+
+    def read(path, counter):
+        vals = collections.defaultdict(list)
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] == counter:
+                vals[r["Kernel_Name"]].append(float(r["Counter_Value"]) * 1024.0)
+        return vals
+
+    f, w = read(fetch_csv, "FETCH_SIZE"), read(write_csv, "WRITE_SIZE")
+    probe = float(mib << 20)
+    # the probe's copy dispatches (the copy may be split); the step itself copies only small arrays
+    fcopy = sum(v for k, vs in f.items() if COPY_SYMBOL in k for v in vs if v > probe / 64)
+    wcopy = sum(v for k, vs in w.items() if COPY_SYMBOL in k for v in vs if v > probe / 64)
+    if not (0.5 * probe < fcopy < 2 * probe and 0.5 * probe < wcopy < 2 * probe):
+        raise SystemExit(f"copy probe dispatches not found in the counter files ({fcopy:.3e}, {wcopy:.3e} B)")
+    kf, kw = probe / fcopy, probe / wcopy
+    out = {"workload": tag, "copy_probe_bytes": probe, "fetch_scale": kf, "write_scale": kw, "kernels": {}}
+    for short, sym in KERNEL_SYMBOLS.items():
+        names = [k for k in f if sym in k and k in w]
+        for k in names:
+            fb = sum(f[k]) / len(f[k]) * kf
+            wb = sum(w[k]) / len(w[k]) * kw
+            out["kernels"][short] = {"symbol": k, "fetch_bytes": fb, "write_bytes": wb,
+                                     "traffic_bytes": fb + wb, "dispatches": len(f[k])}
+    return out
 
 
 def algorithmic_bytes(kernel: str, nC: int, nF: int, Nv: int, s: int) -> float | None:
@@ -96,7 +170,17 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0)
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
     a = ap.parse_args()
+    tag = workload_tag(a.cells, a.ndv, a.kn)
+    if a.pmc_summary:
+        summ = pmc_summary(a.pmc_summary[0], a.pmc_summary[1], tag, a.copy_probe_mib or 1024)
+        os.makedirs(os.path.dirname(traffic_file(tag)), exist_ok=True)
+        with open(traffic_file(tag), "w") as fh:
+            json.dump(summ, fh, indent=1)
+        print(json.dumps(summ))
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs the DUGKS step on one GPU; --gpus must be 1 "
                                    "(velocity-space sharding across GPUs is a design, not a build)"}))
@@ -122,7 +206,14 @@ def main():
         ach = alg / dom_avg / 1e9
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                    "avg_launch_ms": round(dom_avg * 1e3, 4)}
+                    "algorithmic_bytes": alg, "avg_launch_ms": round(dom_avg * 1e3, 4)}
+        tf = traffic_file(tag)
+        if os.path.exists(tf):
+            rec = json.load(open(tf)).get("kernels", {}).get(dom)
+            if rec:
+                roofline["traffic"] = round(rec["traffic_bytes"])
+                roofline["traffic_over_algorithmic"] = round(rec["traffic_bytes"] / alg, 3)
+                roofline["traffic_source"] = os.path.relpath(tf, ROOT)
     step_frac = value * 24.0 / (HBM_PEAK_GBS * 1e9)
 
     out = {
@@ -161,7 +252,9 @@ def main():
                                "kind": "port",
                                "sample": f"{a.cpu_steps} FP64 steps of the same workload (after 1 untimed step), "
                                          f"OpenMP over velocities, {cb['wall']:.1f} s"}
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
+    if a.copy_probe_mib:
+        copy_probe(a.copy_probe_mib)
 
 
 if __name__ == "__main__":
