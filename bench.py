@@ -158,6 +158,21 @@ def cpu_baseline(case, steps):
     return dict(value=case.n_cells * case.nv * steps / wall, cores=threads, wall=wall)
 
 
+def kn_sweep(a):
+    """Mixed (FP32 storage) against FP64 on the GPU for each Kn, after a.steps steps each."""
+    from dugks_amd import cases
+    for kn in [float(x) for x in a.kn_sweep.split(",")]:
+        case = cases.config1(kn, N=a.cells, n_dv=a.ndv)
+        mixed = run_gpu(case, 1, a.steps, 0, a.fused)
+        fp64 = run_gpu(case, 0, a.steps, 0, a.fused)
+        mo, mf = mixed["macros"], fp64["macros"]
+        upd = case.n_cells * case.nv * a.steps
+        print(json.dumps({"kn": kn, "steps": a.steps, "lid_speed": case.meta.get("lid_speed"), "lid_jitter": case.meta.get("lid_jitter"),
+                          "mixed_rel_err_vs_fp64": {k: rel(mo[k], mf[k]) for k in ("rho", "U", "T", "q")},
+                          "mixed_value": upd / mixed["wall"], "fp64_value": upd / fp64["wall"],
+                          "mixed_over_fp64": fp64["wall"] / mixed["wall"]}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,6 +186,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0)
+    ap.add_argument("--workload", choices=("cavity", "sphere"), default="cavity")
+    ap.add_argument("--kn-sweep", type=str, default="",
+                    help="comma-separated Kn values: mixed vs FP64 error table on the cavity, one JSON line each")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
     a = ap.parse_args()
     tag = workload_tag(a.cells, a.ndv, a.kn)
@@ -187,10 +205,20 @@ def main():
         sys.exit(2)
 
     from dugks_amd import cases
-    case = cases.config1(a.kn, N=a.cells, n_dv=a.ndv)
+    if a.workload == "sphere":
+        case = cases.config2(n_dv=a.ndv if a.ndv != 64 else 24)
+        tag = f"sphere{case.n_cells}_v{case.nv}"
+        wl = (f"flow past a sphere, {case.n_cells} hex cells, {round(case.nv ** (1 / 3))}^3 half-range "
+              f"Gauss-Hermite velocities, Kn=0.1, Ma=0.2")
+    else:
+        case = cases.config1(a.kn, N=a.cells, n_dv=a.ndv)
+        wl = f"lid-driven cavity {a.cells}x{a.cells}, {a.ndv}x{a.ndv} trapezoid on [-5,5], Kn={a.kn}"
     nC, Nv = case.n_cells, case.nv
     nF = case.mesh.n_internal_faces + sum(p.size for p in case.mesh.patches if p.geometric_type != "empty")
     upd_per_step = nC * Nv
+    if a.kn_sweep:
+        kn_sweep(a)
+        return
 
     mixed = run_gpu(case, 1, a.steps, a.warmup, a.fused)
     ms_per_step = mixed["wall"] / a.steps * 1e3
@@ -229,8 +257,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 storage / f64 moments+equilibrium+update",
         "data": "synthetic (uniform rest equilibrium, nondimensional)",
-        "config": {"workload": f"lid-driven cavity {a.cells}x{a.cells}, {a.ndv}x{a.ndv} trapezoid on [-5,5], "
-                               f"Kn={a.kn}", "cells": nC, "velocities": Nv, "faces": nF,
+        "config": {"workload": wl, "cells": nC, "velocities": Nv, "faces": nF,
                    "path": "fused" if mixed["fused"] else "staged"},
         "step_roofline_frac": round(step_frac, 4),
         "kernels_ms": {k: round(v[0] / v[1] * 1e3, 4) for k, v in kt.items()},
@@ -246,7 +273,7 @@ def main():
         out["mixed_over_fp64"] = value / v64
         out["mixed_rel_err_vs_fp64"] = {k: rel(mo[k], mf[k]) for k in ("rho", "U", "T", "q")}
         out["mixed_rel_err_note"] = f"after {a.warmup + a.steps} steps, same dt control"
-    if not a.no_cpu:
+    if not a.no_cpu and a.workload == "cavity":
         cb = cpu_baseline(case, a.cpu_steps)
         out["cpu_baseline"] = {"value": cb["value"], "unit": "cell*velocity updates/s", "cores": cb["cores"],
                                "kind": "port",

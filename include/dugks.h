@@ -143,6 +143,33 @@ int dugks_set_cell_macros(dugks_handle *h, const double *rho, const double *U, c
 int dugks_checkpoint_save(dugks_handle *h, const char *path);
 int dugks_checkpoint_load(dugks_handle *h, const char *path);
 
+/* Objective of the adjoint, evaluated on the last step's face values:
+ *   J = sum_f q_f . q_weight_f  +  sum_f sum_v w_v (xi_v . mom_weight_f) (xi_v . S_f) g_f,v
+ * with q_f the face heat flux (qSurf, fvDVM.C:1300) and g_f,v the face distribution after
+ * the face relaxation (the value the update uses, fvDVM.C:1362-1460): the first term gives
+ * wall heat flux, the second the momentum flux through faces (force on a wall = drag).
+ * Arrays are [nIF + nB][3] in the caller's face numbering; either may be NULL. */
+typedef struct dugks_objective {
+    const double *q_weight;
+    const double *mom_weight;
+} dugks_objective;
+
+/* No counterpart in the reference (it has no adjoint).  Hand-written reverse of n_steps
+ * DUGKS steps with the given time steps dts[n_steps] (held fixed: the gradient is that of
+ * the fixed-dt map), starting from the handle's current state and leaving it advanced by
+ * n_steps.  The forward keeps g~, h~ every checkpoint_every steps and recomputes the steps
+ * in between during the reverse sweep.  Outputs (any may be NULL):
+ *   J                 objective value;
+ *   dJ_dmuRef         derivative with respect to the reference viscosity muRef (Kn);
+ *   dJ_dTw, dJ_dUw    [nB], [nB][3]: with respect to each diffuse-wall face's temperature
+ *                     and velocity (they enter the incoming Maxwellian and inComingByRho);
+ *   dJ_dg0, dJ_dh0    [Nv][n_cells]: with respect to the initial g~, h~ (initial cell macros
+ *                     held fixed).
+ * Adjoint arithmetic is FP64; the forward storage follows the handle's precision. */
+int dugks_adjoint(dugks_handle *h, int n_steps, const double *dts, int checkpoint_every,
+                  const dugks_objective *obj, double *J, double *dJ_dmuRef, double *dJ_dTw,
+                  double *dJ_dUw, double *dJ_dg0, double *dJ_dh0);
+
 /* sizes */
 int dugks_num_velocities(const dugks_handle *h);
 int dugks_velocity_set(const dugks_handle *h, double *xi, double *w);  /* [Nv][3], [Nv] */

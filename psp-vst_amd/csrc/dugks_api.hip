@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/dugks.h"
+#include "dugks_adjoint.h"
 #include "dugks_fused.h"
 #include "dugks_internal.h"
 
@@ -256,21 +257,24 @@ int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patch
     for (int b = 0; b < nB; ++b)
         if (bf_type[b] == -1) return fail("dugks_create: boundary face " + std::to_string(b) + " is in no patch");
 
-    // Internal cell numbering: recursive coordinate bisection into blocks of <= 256 cells
-    // (contiguous in the internal numbering).  Faces keep the caller's numbering; every
+    for (int f = 0; f < nF; ++f)
+        if (mesh->owner[f] < 0 || mesh->owner[f] >= nC) return fail("dugks_create: owner label out of range");
+    for (int f = 0; f < nIF; ++f)
+        if (mesh->neighbour[f] < 0 || mesh->neighbour[f] >= nC) return fail("dugks_create: neighbour label out of range");
+
+    // Internal cell numbering: recursive coordinate bisection into blocks (contiguous in the
+    // internal numbering) of the largest size whose two halo rings fit the fused passes'
+    // work-group shapes (2-D meshes: 256 cells).  Faces keep the caller's numbering; every
     // cell-indexed array crossing the C-ABI is permuted on the way in and out.
     std::vector<int> perm, block_start, inv(nC);
-    rcb_partition(nC, mesh->C, 256, perm, block_start);
+    for (int max_block : {256, 192, 128, 96, 64}) {
+        rcb_partition(nC, mesh->C, max_block, perm, block_start);
+        if (o.fused == 0 || block_tables_fit(nC, nIF, nF, mesh->owner, mesh->neighbour, D, perm, block_start)) break;
+    }
     for (int i = 0; i < nC; ++i) inv[perm[i]] = i;
     std::vector<int> own_p(nF), nei_p(nIF);
-    for (int f = 0; f < nF; ++f) {
-        if (mesh->owner[f] < 0 || mesh->owner[f] >= nC) return fail("dugks_create: owner label out of range");
-        own_p[f] = inv[mesh->owner[f]];
-    }
-    for (int f = 0; f < nIF; ++f) {
-        if (mesh->neighbour[f] < 0 || mesh->neighbour[f] >= nC) return fail("dugks_create: neighbour label out of range");
-        nei_p[f] = inv[mesh->neighbour[f]];
-    }
+    for (int f = 0; f < nF; ++f) own_p[f] = inv[mesh->owner[f]];
+    for (int f = 0; f < nIF; ++f) nei_p[f] = inv[mesh->neighbour[f]];
     std::vector<double> C_p(3 * (size_t)nC), V_p(nC);
     for (int i = 0; i < nC; ++i) {
         for (int k = 0; k < 3; ++k) C_p[3 * i + k] = mesh->C[3 * (size_t)perm[i] + k];
@@ -688,6 +692,53 @@ int ckpt_io(dugks_handle *h, const char *path, bool save)
     return rc;
 }
 }  // namespace
+
+int dugks_adjoint(dugks_handle *h, int n_steps, const double *dts, int checkpoint_every,
+                  const dugks_objective *obj, double *J, double *dJ_dmuRef, double *dJ_dTw, double *dJ_dUw,
+                  double *dJ_dg0, double *dJ_dh0)
+{
+    if (check_handle(h)) return 1;
+    if (n_steps < 1 || !dts) return fail("dugks_adjoint: need n_steps >= 1 and dts");
+    if (checkpoint_every < 1) return fail("dugks_adjoint: checkpoint_every must be >= 1");
+    for (int n = 0; n < n_steps; ++n)
+        if (!(dts[n] > 0)) return fail("dugks_adjoint: non-positive time step " + std::to_string(n));
+    const Geo &g = h->g;
+    AdjointIO io;
+    io.n_steps = n_steps;
+    io.dts = dts;
+    io.ckpt = std::min(checkpoint_every, n_steps);
+    io.q_weight = obj ? obj->q_weight : nullptr;
+    io.mom_weight = obj ? obj->mom_weight : nullptr;
+    io.J = J;
+    io.dmu = dJ_dmuRef;
+    io.dTw = dJ_dTw;
+    io.dUw = dJ_dUw;
+    std::vector<double> lg, lh;
+    if (dJ_dg0) {
+        lg.resize((size_t)g.Nv * g.nC);
+        io.dg0 = lg.data();
+    }
+    if (dJ_dh0) {
+        lh.resize((size_t)g.Nv * g.nC);
+        io.dh0 = lh.data();
+    }
+    HIPCHK(hipSetDevice(h->device));
+    std::string err;
+    const int rc = h->es == 4 ? run_adjoint<float>(g, h->d, h->stream, io, err)
+                              : run_adjoint<double>(g, h->d, h->stream, io, err);
+    if (rc) return fail("dugks_adjoint: " + err);
+    if (h->fused) Fused<double>::cell_coef(g, h->plan, h->d.cell, h->stream);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    // distributions back to the caller's cell order
+    for (int which = 0; which < 2; ++which) {
+        double *out = which ? dJ_dh0 : dJ_dg0;
+        const std::vector<double> &src = which ? lh : lg;
+        if (!out) continue;
+        for (int v = 0; v < g.Nv; ++v)
+            for (int i = 0; i < g.nC; ++i) out[(size_t)v * g.nC + h->perm[i]] = src[(size_t)v * g.nC + i];
+    }
+    return 0;
+}
 
 int dugks_checkpoint_save(dugks_handle *h, const char *path) { return ckpt_io(h, path, true); }
 int dugks_checkpoint_load(dugks_handle *h, const char *path)

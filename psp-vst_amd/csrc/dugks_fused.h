@@ -96,6 +96,10 @@ struct FusedPlan {
 // Recursive coordinate bisection of the cells into blocks of at most max_block cells.
 // Returns the permutation (new -> old) and the block starts in the new numbering.
 void rcb_partition(int nC, const double *C, int max_block, std::vector<int> &perm, std::vector<int> &block_start);
+// True when every block with its two halo rings fits the work-group shapes of the passes
+// (owner / neighbour in the caller's numbering, perm new -> old).
+bool block_tables_fit(int nC, int nIF, int nF, const int *owner, const int *neighbour, int D,
+                      const std::vector<int> &perm, const std::vector<int> &block_start);
 
 bool fused_plan_build(FusedPlan &p, const Geo &g, const HostPrep &prep, const std::vector<int> &block_start,
                       size_t es, int nvg_hint, std::string &err);

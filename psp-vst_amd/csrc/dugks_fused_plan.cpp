@@ -57,6 +57,64 @@ void rcb_partition(int nC, const double *C, int max_block, std::vector<int> &per
     block_start.push_back(nC);
 }
 
+bool block_tables_fit(int nC, int nIF, int nF, const int *owner, const int *neighbour, int D,
+                      const std::vector<int> &perm, const std::vector<int> &block_start)
+{
+    // cell -> internal-face neighbours and cell -> faces, in the caller's numbering
+    std::vector<int> deg(nC + 1, 0), fdeg(nC + 1, 0);
+    for (int f = 0; f < nIF; ++f) {
+        ++deg[owner[f] + 1];
+        ++deg[neighbour[f] + 1];
+    }
+    for (int f = 0; f < nF; ++f) {
+        ++fdeg[owner[f] + 1];
+        if (f < nIF) ++fdeg[neighbour[f] + 1];
+    }
+    for (int c = 0; c < nC; ++c) {
+        deg[c + 1] += deg[c];
+        fdeg[c + 1] += fdeg[c];
+    }
+    std::vector<int> nb(deg[nC]), fc(fdeg[nC]), pos(deg.begin(), deg.end() - 1), fpos(fdeg.begin(), fdeg.end() - 1);
+    for (int f = 0; f < nIF; ++f) {
+        nb[pos[owner[f]]++] = neighbour[f];
+        nb[pos[neighbour[f]]++] = owner[f];
+    }
+    for (int f = 0; f < nF; ++f) {
+        fc[fpos[owner[f]]++] = f;
+        if (f < nIF) fc[fpos[neighbour[f]]++] = f;
+    }
+    const int max_nl = std::min(kFusedThreadsA * fused_cells_per_thread_a(D), kFusedThreadsB * fused_cells_per_thread_b(D));
+    const int max_nfb = kFusedThreadsB * fused_faces_per_thread_b(D);
+    std::vector<int> mark(nC, -1), fmark(nF, -1);
+    const int nblk = (int)block_start.size() - 1;
+    for (int b = 0; b < nblk; ++b) {
+        std::vector<int> ring(perm.begin() + block_start[b], perm.begin() + block_start[b + 1]);
+        if ((int)ring.size() > kFusedThreadsB) return false;
+        int nl = (int)ring.size(), nfb = 0;
+        for (int c : ring) {
+            mark[c] = b;
+            for (int k = fdeg[c]; k < fdeg[c + 1]; ++k)
+                if (fmark[fc[k]] != b) {
+                    fmark[fc[k]] = b;
+                    ++nfb;
+                }
+        }
+        for (int r = 0; r < 2; ++r) {
+            std::vector<int> next;
+            for (int c : ring)
+                for (int k = deg[c]; k < deg[c + 1]; ++k)
+                    if (mark[nb[k]] != b) {
+                        mark[nb[k]] = b;
+                        next.push_back(nb[k]);
+                    }
+            nl += (int)next.size();
+            ring.swap(next);
+        }
+        if (nl > max_nl || nfb > max_nfb) return false;
+    }
+    return true;
+}
+
 bool fused_plan_build(FusedPlan &p, const Geo &g, const HostPrep &prep, const std::vector<int> &block_start,
                       size_t es, int nvg_hint, std::string &err)
 {

@@ -97,8 +97,13 @@ def load_library(path: str = LIB_PATH):
     L.dugks_last_error.restype = ct.c_char_p
     L.dugks_destroy.argtypes = [vp]
     L.dugks_destroy.restype = None
+    L.dugks_adjoint.argtypes = [vp, ct.c_int, _D, ct.c_int, ct.POINTER(Objective), _D, _D, _D, _D, _D, _D]
     _lib = L
     return L
+
+
+class Objective(ct.Structure):
+    _fields_ = [("q_weight", _D), ("mom_weight", _D)]
 
 
 def _p(a, typ=_D):
@@ -207,6 +212,36 @@ class FvDVM:
             hist.append(self.dt)
             self.evolution()
         return np.array(hist)
+
+    def run_fixed(self, dts):
+        """evolution() with the given time steps (no Courant-number control)."""
+        for dt in np.asarray(dts, dtype=np.float64):
+            self.evolution(float(dt))
+
+    def adjoint(self, dts, checkpoint_every: int = 1, q_weight=None, mom_weight=None, want_df0: bool = False):
+        """Objective and gradients of len(dts) steps with fixed time steps (dugks_adjoint).
+
+        q_weight / mom_weight: [nF, 3] face weights of the heat-flux and momentum-flux terms
+        of the objective on the last step.  Returns dict(J, dmuRef, dTw [nB], dUw [nB, 3]
+        and, with want_df0, dg0 / dh0 [Nv, nC]); the solver is left after the steps.
+        """
+        dts = np.ascontiguousarray(dts, dtype=np.float64)
+        qw = None if q_weight is None else np.ascontiguousarray(q_weight, dtype=np.float64).reshape(self.nF, 3)
+        mw = None if mom_weight is None else np.ascontiguousarray(mom_weight, dtype=np.float64).reshape(self.nF, 3)
+        obj = Objective(q_weight=_p(qw), mom_weight=_p(mw))
+        J, dmu = ct.c_double(), ct.c_double()
+        dTw = np.zeros(self.nB)
+        dUw = np.zeros((self.nB, 3))
+        dg0 = np.empty((self.Nv, self.nC)) if want_df0 else None
+        dh0 = np.empty((self.Nv, self.nC)) if want_df0 else None
+        self._check(self.L.dugks_adjoint(self._h, len(dts), _p(dts), int(checkpoint_every), ct.byref(obj),
+                                         ct.byref(J), ct.byref(dmu), _p(dTw), _p(dUw), _p(dg0), _p(dh0)))
+        self.step_index += len(dts)
+        self.time += float(dts.sum())
+        r = dict(J=J.value, dmuRef=dmu.value, dTw=dTw, dUw=dUw)
+        if want_df0:
+            r["dg0"], r["dh0"] = dg0, dh0
+        return r
 
     def synchronize(self):
         self._check(self.L.dugks_synchronize(self._h))
