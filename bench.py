@@ -158,19 +158,104 @@ def cpu_baseline(case, steps):
     return dict(value=case.n_cells * case.nv * steps / wall, cores=threads, wall=wall)
 
 
+def rel2(a, b):
+    import numpy as np
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def compare_precisions(case, steps, every, fused=-1, label=None):
+    """Mixed (FP32 storage) and FP64 runs side by side on the GPU, compared every `every`
+    steps up to `steps` (max-norm relative error max|a-b|/max|b| and 2-norm relative error);
+    a run whose fields stop being finite ends the comparison (the step is reported)."""
+    import numpy as np
+    from dugks_amd import hip
+    sm = hip.FvDVM(case, precision=hip.MIXED, fused=fused)
+    sf = hip.FvDVM(case, precision=hip.FP64, fused=fused)
+    rows, done, diverged = [], 0, None
+    tm = tf = 0.0
+    while done < steps:
+        n = min(every, steps - done)
+        t0 = time.perf_counter()
+        sm.run(n)
+        sm.synchronize()
+        t1 = time.perf_counter()
+        sf.run(n)
+        sf.synchronize()
+        t2 = time.perf_counter()
+        tm, tf = tm + t1 - t0, tf + t2 - t1
+        done += n
+        mo, mf = sm.cell_macros(), sf.cell_macros()
+        finite = {k: bool(np.all(np.isfinite(v))) for k, v in (("mixed", mo["rho"]), ("fp64", mf["rho"]))}
+        if not all(finite.values()):
+            diverged = {"step": done, "finite": finite}
+            break
+        row = {"step": done, **{k: rel(mo[k], mf[k]) for k in ("rho", "U", "T", "q")},
+               "l2": {k: rel2(mo[k], mf[k]) for k in ("rho", "U", "T", "q")},
+               "rho_range": [float(mf["rho"].min()), float(mf["rho"].max())],
+               "T_range": [float(mf["T"].min()), float(mf["T"].max())]}
+        rows.append(row)
+        print(json.dumps({"progress": label, **row}), flush=True)
+    sm.close()
+    sf.close()
+    upd = case.n_cells * case.nv * done
+    return {"mixed_rel_err_vs_fp64": rows, "diverged": diverged, "mixed_value": upd / tm if tm else None,
+            "fp64_value": upd / tf if tf else None, "mixed_over_fp64": tf / tm if tm else None}
+
+
 def kn_sweep(a):
-    """Mixed (FP32 storage) against FP64 on the GPU for each Kn, after a.steps steps each."""
+    """Mixed against FP64 on the cavity for each Kn of a.kn_sweep (a.steps steps)."""
     from dugks_amd import cases
     for kn in [float(x) for x in a.kn_sweep.split(",")]:
         case = cases.config1(kn, N=a.cells, n_dv=a.ndv)
-        mixed = run_gpu(case, 1, a.steps, 0, a.fused)
-        fp64 = run_gpu(case, 0, a.steps, 0, a.fused)
-        mo, mf = mixed["macros"], fp64["macros"]
-        upd = case.n_cells * case.nv * a.steps
-        print(json.dumps({"kn": kn, "steps": a.steps, "lid_speed": case.meta.get("lid_speed"), "lid_jitter": case.meta.get("lid_jitter"),
-                          "mixed_rel_err_vs_fp64": {k: rel(mo[k], mf[k]) for k in ("rho", "U", "T", "q")},
-                          "mixed_value": upd / mixed["wall"], "fp64_value": upd / fp64["wall"],
-                          "mixed_over_fp64": fp64["wall"] / mixed["wall"]}), flush=True)
+        r = compare_precisions(case, a.steps, a.sweep_every, a.fused, label=f"kn={kn}")
+        print(json.dumps({"kn": kn, "steps": a.steps, "max_co": case.controls.maxCo,
+                          "lid_speed": case.meta.get("lid_speed"), "lid_jitter": case.meta.get("lid_jitter"),
+                          **r}), flush=True)
+
+
+def adjoint_check(a):
+    """Adjoint gradient against central differences of the FP64 GPU forward (fixed time
+    steps) on the adjoint cavity (100x100, 32x32 trapezoid on [-4,4], Kn 0.1, 500 steps,
+    checkpoints every 50) or the adjoint sphere (~51k cells, 16^3 Gauss-Hermite, Kn 0.1,
+    Ma 0.2, 300 steps, checkpoints every 30, mixed precision, 64 wall patches)."""
+    import numpy as np
+    from dugks_amd import adjoint, hip
+    eps = 1e-6
+    if a.adjoint == "cavity":
+        prob = adjoint.CavityHeatFlux(steps=a.adjoint_steps or 500, checkpoint_every=50, precision=hip.FP64)
+        th = prob.nominal()
+        fd_idx = [0, 1, 2 + 10, 2 + 50, 2 + 90]
+        names = ["lid_speed", "Kn", "T_w[10]", "T_w[50]", "T_w[90]"]
+    else:
+        prob = adjoint.SphereDrag(steps=a.adjoint_steps or 300, checkpoint_every=30, precision=hip.MIXED)
+        th = prob.nominal()
+        th[1:] = 1.0 + 0.05 * np.sin(np.arange(prob.n_patch))
+        fd_idx = [0, 1, 17, 40]
+        names = ["Kn", "T_patch[0]", "T_patch[16]", "T_patch[39]"]
+    t0 = time.perf_counter()
+    prob.time_steps(th)
+    t1 = time.perf_counter()
+    res = prob.evaluate(th)
+    t2 = time.perf_counter()
+    out = {"problem": a.adjoint, "steps": prob.steps, "checkpoint_every": prob.ckpt, "J": res.J,
+           "n_params": len(th), "adjoint_s": t2 - t1, "forward_dt_run_s": t1 - t0, "checks": []}
+    print(json.dumps({"progress": "adjoint done", "J": res.J, "s": t2 - t1}), flush=True)
+    for i, nm in zip(fd_idx, names):
+        h = eps * max(abs(th[i]), 1.0) if i > 1 or a.adjoint == "sphere" else eps * th[i]
+        tp, tm = th.copy(), th.copy()
+        tp[i] += h
+        tm[i] -= h
+        fd = (prob.value(tp, precision=hip.FP64) - prob.value(tm, precision=hip.FP64)) / (2 * h)
+        out["checks"].append({"param": nm, "adjoint": float(res.grad[i]), "fd_fp64": fd,
+                              "rel_err": abs(res.grad[i] - fd) / max(abs(fd), 1e-300)})
+        print(json.dumps({"progress": nm, **out["checks"][-1]}), flush=True)
+    if a.adjoint == "sphere":
+        p64 = adjoint.SphereDrag(steps=prob.steps, checkpoint_every=30, precision=hip.FP64, dts=prob.dts)
+        r64 = p64.evaluate(th)
+        out["fp64_adjoint_J"] = r64.J
+        out["mixed_vs_fp64_grad_rel_err"] = float(np.max(np.abs(res.grad - r64.grad)) / np.max(np.abs(r64.grad)))
+        out["mixed_vs_fp64_J_rel_err"] = abs(res.J - r64.J) / abs(r64.J)
+    print(json.dumps(out), flush=True)
 
 
 def main():
@@ -189,6 +274,12 @@ def main():
     ap.add_argument("--workload", choices=("cavity", "sphere"), default="cavity")
     ap.add_argument("--kn-sweep", type=str, default="",
                     help="comma-separated Kn values: mixed vs FP64 error table on the cavity, one JSON line each")
+    ap.add_argument("--sweep-every", type=int, default=100)
+    ap.add_argument("--compare", action="store_true",
+                    help="mixed against FP64 side by side for --steps steps (error every --sweep-every)")
+    ap.add_argument("--adjoint", choices=("cavity", "sphere"), default=None,
+                    help="adjoint gradient check against central differences instead of the benchmark")
+    ap.add_argument("--adjoint-steps", type=int, default=0)
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
     a = ap.parse_args()
     tag = workload_tag(a.cells, a.ndv, a.kn)
@@ -218,6 +309,13 @@ def main():
     upd_per_step = nC * Nv
     if a.kn_sweep:
         kn_sweep(a)
+        return
+    if a.adjoint:
+        adjoint_check(a)
+        return
+    if a.compare:
+        r = compare_precisions(case, a.steps, a.sweep_every, a.fused, label=wl)
+        print(json.dumps({"workload": wl, "steps": a.steps, "max_co": case.controls.maxCo, **r}), flush=True)
         return
 
     mixed = run_gpu(case, 1, a.steps, a.warmup, a.fused)
@@ -261,6 +359,8 @@ def main():
                    "path": "fused" if mixed["fused"] else "staged"},
         "step_roofline_frac": round(step_frac, 4),
         "kernels_ms": {k: round(v[0] / v[1] * 1e3, 4) for k, v in kt.items()},
+        "kernels_gbs": {k: round(algorithmic_bytes(k, nC, nF, Nv, 4) / (v[0] / v[1]) / 1e9, 1)
+                        for k, v in kt.items() if algorithmic_bytes(k, nC, nF, Nv, 4)},
         "roofline": roofline,
     }
     if not a.no_fp64:

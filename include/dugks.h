@@ -165,6 +165,7 @@ typedef struct dugks_objective {
  *                     and velocity (they enter the incoming Maxwellian and inComingByRho);
  *   dJ_dg0, dJ_dh0    [Nv][n_cells]: with respect to the initial g~, h~ (initial cell macros
  *                     held fixed).
+ * With every derivative output NULL only the forward runs (objective evaluation).
  * Adjoint arithmetic is FP64; the forward storage follows the handle's precision. */
 int dugks_adjoint(dugks_handle *h, int n_steps, const double *dts, int checkpoint_every,
                   const dugks_objective *obj, double *J, double *dJ_dmuRef, double *dJ_dTw,

@@ -782,7 +782,9 @@ int run_adjoint(const Geo &g, DevState &d, hipStream_t st, const AdjointIO &io, 
     ADJ_CHK(hipMemcpyAsync(rho_b_save, d.rho_b, sizeof(double) * nB, hipMemcpyDeviceToDevice, st) == hipSuccess);
     ADJ_CHK(hipMemcpyAsync(cfl_save, d.cfl_part, sizeof(double) * ncfl, hipMemcpyDeviceToDevice, st) == hipSuccess);
 
-    // reverse sweep
+    // reverse sweep (skipped when no derivative is asked for: objective only)
+    const bool want_grad = io.dmu || io.dTw || io.dUw || io.dg0 || io.dh0;
+    if (!want_grad) return 0;
     FaceAdjArgs fa{thb, d_pstart, d_psize, (int)p_start.size(), nullptr, mu_f, Tw_bar, Uw_bar};
     for (int s = nseg - 1; s >= 0; --s) {
         const int n0 = s * K, n1 = std::min(N, n0 + K);

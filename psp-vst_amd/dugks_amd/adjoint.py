@@ -91,7 +91,8 @@ class CavityHeatFlux:
     def case(self, theta):
         theta = np.asarray(theta, float)
         return cases.cavity_case(self.N, self.rule, self.n_dv, self.vel_extent, float(theta[1]),
-                                 lid_speed=float(theta[0]), wall_T=theta[2:2 + self.N])
+                                 lid_speed=float(theta[0]), wall_T=theta[2:2 + self.N],
+                                 controls=cases.Controls(maxCo=cases.STABLE_MAX_CO[2]))
 
     def time_steps(self, theta):
         if self.dts is None:
@@ -100,7 +101,17 @@ class CavityHeatFlux:
             s.close()
         return self.dts
 
-    def evaluate(self, theta, want_grad: bool = True) -> AdjointResult:
+    def value(self, theta, precision=None) -> float:
+        """J(theta) from the forward only (same fixed time steps)."""
+        theta = np.asarray(theta, float)
+        dts = self.time_steps(theta)
+        c = self.case(theta)
+        s = hip.FvDVM(c, precision=self.precision if precision is None else precision)
+        J = s.objective(dts, q_weight=bottom_wall_q_weight(c))
+        s.close()
+        return J
+
+    def evaluate(self, theta) -> AdjointResult:
         theta = np.asarray(theta, float)
         dts = self.time_steps(theta)
         c = self.case(theta)
@@ -145,7 +156,17 @@ class SphereDrag:
             s.close()
         return self.dts
 
-    def evaluate(self, theta, want_grad: bool = True) -> AdjointResult:
+    def value(self, theta, precision=None) -> float:
+        """J(theta) from the forward only (same fixed time steps)."""
+        theta = np.asarray(theta, float)
+        dts = self.time_steps(theta)
+        c = self.case(theta)
+        s = hip.FvDVM(c, precision=self.precision if precision is None else precision)
+        J = s.objective(dts, mom_weight=drag_mom_weight(c))
+        s.close()
+        return J
+
+    def evaluate(self, theta) -> AdjointResult:
         theta = np.asarray(theta, float)
         dts = self.time_steps(theta)
         c = self.case(theta)

@@ -197,12 +197,25 @@ def config0() -> Case:
     return cavity_case(40, "NC38", 28, 4.0, 0.075, name="cavity40_nc28_kn0.075")
 
 
+# Courant limit for long runs.  The step reconstructs at x_f - xi dt/2 from the upwind cell
+# with its full gradient (transverse terms included), an unsplit scheme whose von Neumann
+# factor for free transport exceeds 1 once sum_d |xi_d| dt / dx > 1 (checkerboard mode;
+# 1.20 per step at |xi_d| dt / dx = 0.55 in 2-D).  dugksFoam's Courant number bounds
+# (|U| + sqrt(D) xi_max) dt / dx, so maxCo 0.8 allows sum_d |xi_d| dt / dx up to 0.8 sqrt(D):
+# corner velocities of the set grow and, on meshes fine enough for the growth to outrun
+# the walls (256^2, 100^2 with 2000 / 500 steps), the run diverges.  maxCo <= 1 / sqrt(D)
+# keeps every velocity stable.
+STABLE_MAX_CO = {2: 0.65, 3: 0.55}
+
+
 def config1(Kn: float, N: int = 256, n_dv: int = 64) -> Case:
-    """256x256 cavity, 64x64 trapezoid on [-5, 5], lid jittered by +-5 % per (config, Kn)."""
+    """256x256 cavity, 64x64 trapezoid on [-5, 5], lid jittered by +-5 % per (config, Kn),
+    Courant limit STABLE_MAX_CO[2]."""
     tag = f"cavity{N}_trap{n_dv}_Kn{Kn!r}"
     jitter = seeded_uniform(tag, -0.05, 0.05)
     lid = 0.16 * math.sqrt(2.0) * (1.0 + jitter)
-    c = cavity_case(N, "trapezoid", n_dv, 5.0, Kn, lid_speed=lid, name=tag, seed_tag=tag)
+    c = cavity_case(N, "trapezoid", n_dv, 5.0, Kn, lid_speed=lid, name=tag, seed_tag=tag,
+                    controls=Controls(maxCo=STABLE_MAX_CO[2]))
     c.meta["lid_jitter"] = jitter
     return c
 
@@ -220,7 +233,7 @@ def smoothed_wall_profile(n: int, seed: int = 12345, amp: float = 0.05, width: f
 
 
 def sphere_case(mesh_kw: dict, rule: str, n_dv: int, vel_scale: float, Kn: float, Ma: float = 0.2,
-                wall_T=1.0, name: str | None = None) -> Case:
+                wall_T=1.0, name: str | None = None, controls: Controls | None = None) -> Case:
     """Flow past a sphere (diameter 1), nondimensional (R = T0 = rho0 = 1, L = D).
 
     Freestream Ma = U / sqrt(gamma R T0), gamma = 5/3, on the inlet / outlet / side patches
@@ -248,6 +261,7 @@ def sphere_case(mesh_kw: dict, rule: str, n_dv: int, vel_scale: float, Kn: float
     return Case(name=name or f"sphere{nC}", mesh=m, geo=geo, D=3, xis=xis, wts=wts,
                 xi_max=float(np.max(np.abs(xis))), xi_min=float(np.min(xis)), gas=gas,
                 rho0=np.ones(nC), U0=np.tile([U_inf, 0.0, 0.0], (nC, 1)), T0=np.full(nC, T0), bcs=bcs,
+                controls=controls or Controls(maxCo=STABLE_MAX_CO[3]),
                 meta=dict(Kn=Kn, Ma=Ma, U_inf=U_inf, rule=rule, n_dv=n_dv))
 
 
@@ -263,4 +277,5 @@ def config2(n_dv: int = 24) -> Case:
 def config3() -> Case:
     """100x100 cavity, 32x32 trapezoid on [-4, 4], Kn = 0.1, seeded bottom-wall profile."""
     Tw = smoothed_wall_profile(100)
-    return cavity_case(100, "trapezoid", 32, 4.0, 0.1, wall_T=Tw, name="cavity100_trap32_kn0.1")
+    return cavity_case(100, "trapezoid", 32, 4.0, 0.1, wall_T=Tw, name="cavity100_trap32_kn0.1",
+                       controls=Controls(maxCo=STABLE_MAX_CO[2]))

@@ -243,6 +243,19 @@ class FvDVM:
             r["dg0"], r["dh0"] = dg0, dh0
         return r
 
+    def objective(self, dts, q_weight=None, mom_weight=None):
+        """Objective of the adjoint after len(dts) fixed steps, forward only."""
+        dts = np.ascontiguousarray(dts, dtype=np.float64)
+        qw = None if q_weight is None else np.ascontiguousarray(q_weight, dtype=np.float64).reshape(self.nF, 3)
+        mw = None if mom_weight is None else np.ascontiguousarray(mom_weight, dtype=np.float64).reshape(self.nF, 3)
+        obj = Objective(q_weight=_p(qw), mom_weight=_p(mw))
+        J = ct.c_double()
+        self._check(self.L.dugks_adjoint(self._h, len(dts), _p(dts), len(dts), ct.byref(obj), ct.byref(J),
+                                         None, None, None, None, None))
+        self.step_index += len(dts)
+        self.time += float(dts.sum())
+        return J.value
+
     def synchronize(self):
         self._check(self.L.dugks_synchronize(self._h))
 

@@ -164,3 +164,23 @@ def test_small_sphere_mixed_matches_oracle():
     print("small sphere mixed", e)
     for k in ("rho", "U", "T"):
         assert e[k] < 1e-5, (k, e[k])
+
+
+def test_courant_limit_for_long_runs():
+    """dugksFoam's Courant rule with maxCo 0.8 lets diagonal velocities exceed the unsplit
+    reconstruction's stability bound (sum_d |xi_d| dt/dx <= 1): a 128^2 cavity with the
+    64^2 trapezoid set on [-5, 5] diverges within 300 steps, the same run at maxCo 0.65
+    (the limit used for the long synthetic runs) stays bounded.  Both precisions agree."""
+    from dugks_amd import cases, hip
+    out = {}
+    for co in (0.8, cases.STABLE_MAX_CO[2]):
+        c = cases.cavity_case(128, "trapezoid", 64, 5.0, 0.1, controls=cases.Controls(maxCo=co))
+        s = hip.FvDVM(c, precision=hip.FP64)
+        s.run(300)
+        m = s.cell_macros()
+        s.close()
+        out[co] = m
+    assert not np.all(np.isfinite(out[0.8]["rho"]))
+    m = out[cases.STABLE_MAX_CO[2]]
+    assert np.all(np.isfinite(m["rho"])) and 0.8 < m["rho"].min() and m["rho"].max() < 1.25
+    assert 0.95 < m["T"].min() and m["T"].max() < 1.05
