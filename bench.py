@@ -135,7 +135,8 @@ def run_gpu(case, precision, steps, warmup, fused):
     ktimes = solver.kernel_times()
     solver.profile(False)
     macros = solver.cell_macros()
-    info = dict(wall=wall, ktimes=ktimes, macros=macros, fused="fused_update" in ktimes)
+    pinfo = solver.path_info()
+    info = dict(wall=wall, ktimes=ktimes, macros=macros, fused=bool(pinfo["fused"]), path_info=pinfo)
     del solver
     return info
 
@@ -356,7 +357,11 @@ def main():
         "dtype": "f32 storage / f64 moments+equilibrium+update",
         "data": "synthetic (uniform rest equilibrium, nondimensional)",
         "config": {"workload": wl, "cells": nC, "velocities": Nv, "faces": nF,
-                   "path": "fused" if mixed["fused"] else "staged"},
+                   "path": {1: "fused, stored face values", 2: "fused, recomputed halo rings"}.get(
+                       mixed["path_info"]["fused"], "staged"),
+                   "block_cells": mixed["path_info"]["block_cells"],
+                   "chunk_velocities": mixed["path_info"]["chunk_velocities"],
+                   "velocity_groups": mixed["path_info"]["velocity_groups"]},
         "step_roofline_frac": round(step_frac, 4),
         "kernels_ms": {k: round(v[0] / v[1] * 1e3, 4) for k, v in kt.items()},
         "kernels_gbs": {k: round(algorithmic_bytes(k, nC, nF, Nv, 4) / (v[0] / v[1]) / 1e9, 1)

@@ -175,6 +175,13 @@ int dugks_adjoint(dugks_handle *h, int n_steps, const double *dts, int checkpoin
 int dugks_num_velocities(const dugks_handle *h);
 int dugks_velocity_set(const dugks_handle *h, double *xi, double *w);  /* [Nv][3], [Nv] */
 
+/* Diagnostics (no reference counterpart): the path a handle runs -- 0 staged, 1 fused with
+ * stored face values (3-D meshes), 2 fused recomputing two halo rings (2-D meshes) -- and,
+ * for the fused paths, the largest cell block, velocities per LDS chunk and velocity
+ * groups per block.  Any pointer may be NULL. */
+int dugks_path_info(const dugks_handle *h, int *fused, int *block_cells, int *chunk_velocities,
+                    int *velocity_groups);
+
 /* per-kernel timing of the last evolution (HIP events), seconds; names via index */
 int dugks_kernel_count(const dugks_handle *h);
 const char *dugks_kernel_name(const dugks_handle *h, int i);

@@ -21,7 +21,8 @@ LIB = os.path.join(OUT_DIR, "libdugks_hip.so")
 BUILD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("DUGKS_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["dugks_setup.cpp", "dugks_fused_plan.cpp", "dugks_kernels.hip", "dugks_fused.hip", "dugks_adjoint.hip",
+SOURCES = ["dugks_setup.cpp", "dugks_fused_plan.cpp", "dugks_fused_rc_plan.cpp", "dugks_kernels.hip",
+           "dugks_fused.hip", "dugks_fused_rc.hip", "dugks_adjoint.hip",
            "dugks_api.hip"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
           f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

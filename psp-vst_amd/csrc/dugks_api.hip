@@ -12,6 +12,7 @@
 #include "../../include/dugks.h"
 #include "dugks_adjoint.h"
 #include "dugks_fused.h"
+#include "dugks_fused_rc.h"
 #include "dugks_internal.h"
 
 using namespace dugks;
@@ -34,6 +35,14 @@ int fail(const std::string &m)
 
 __global__ void k_set_scalar(double *p, double v) { *p = v; }
 
+// Fused schedules.  Face values (dugks_fused.h): one halo ring, face values stored once
+// per velocity by the upwind block, in-place update.  Recompute (dugks_fused_rc.h): two
+// halo rings rebuilt by both passes, no face values.  Recomputing the rings costs
+// (ring cells / block cells) equilibria per update: about 1.5x in 2-D, where it beats
+// storing face values (measured 15.9 vs 18.3 ms per step on the 256^2 cavity), and about
+// 5x in 3-D, where storing face values wins (356 vs 506 ms on the 211k-cell sphere).
+enum Schedule { kSchedFaceValues = 1, kSchedRecompute = 2 };
+
 enum Phase {
     kPhBar, kPhFaceRecon, kPhWallRho, kPhWallIn, kPhFaceMom, kPhFaceRelax, kPhUpdate, kPhCellMom, kPhDt,
     kPhFusedFace, kPhFaceMacro, kPhFusedUpdate, kPhCellMacro, kNumPhases
@@ -53,7 +62,10 @@ struct dugks_handle {
     size_t es = 8;                 // bytes per stored distribution value
     bool initialised = false;
     bool fused = false;
-    FusedPlan plan;                // fused-path data
+    // fused schedule: kSchedFaceValues (dugks_fused.h) or kSchedRecompute (dugks_fused_rc.h)
+    int sched = 0;
+    FusedPlan plan;                // face-value schedule data
+    rc::FusedPlan plan_rc;         // recompute schedule data
     std::vector<int> perm;         // internal cell -> caller's cell label
     int hist_cap = 0;
     std::vector<void *> allocs;
@@ -164,16 +176,34 @@ int fused_step(dugks_handle *h)
     const Geo &g = h->g;
     DevState &d = h->d;
     hipStream_t st = h->stream;
-    FusedPlan &p = h->plan;
-    PHASE(kPhFusedFace, Fused<S>::face_pass(g, p, (const S *)d.gT, (const S *)d.hT, d.cell, d.dt, st));
-    PHASE(kPhFaceMacro, Fused<S>::face_macros(g, p, d.face, d.rho_b, d.dt, d.cfl_part, st));
-    PHASE(kPhFusedUpdate, Fused<S>::update_pass(g, p, (const S *)d.gT, (const S *)d.hT, (S *)p.gT2, (S *)p.hT2,
-                                                d.cell, d.face, d.rho_b, d.dt, st));
-    std::swap(d.gT, p.gT2);
-    std::swap(d.hT, p.hT2);
-    PHASE(kPhCellMacro, Fused<S>::cell_macros(g, p, d.cell, d.dt, st));
+    if (h->sched == kSchedRecompute) {
+        rc::FusedPlan &p = h->plan_rc;
+        PHASE(kPhFusedFace, rc::Fused<S>::face_pass(g, p, (const S *)d.gT, (const S *)d.hT, d.cell, d.dt, st));
+        PHASE(kPhFaceMacro, rc::Fused<S>::face_macros(g, p, d.face, d.rho_b, d.dt, d.cfl_part, st));
+        PHASE(kPhFusedUpdate, rc::Fused<S>::update_pass(g, p, (const S *)d.gT, (const S *)d.hT, (S *)p.gT2,
+                                                        (S *)p.hT2, d.cell, d.face, d.rho_b, d.dt, st));
+        std::swap(d.gT, p.gT2);
+        std::swap(d.hT, p.hT2);
+        PHASE(kPhCellMacro, rc::Fused<S>::cell_macros(g, p, d.cell, d.dt, st));
+    } else {
+        FusedPlan &p = h->plan;
+        PHASE(kPhFusedFace, Fused<S>::face_pass(g, p, (const S *)d.gT, (const S *)d.hT, d.dt, st));
+        PHASE(kPhFaceMacro, Fused<S>::face_macros(g, p, d.face, d.rho_b, d.dt, d.cfl_part, st));
+        PHASE(kPhFusedUpdate, Fused<S>::update_pass(g, p, (S *)d.gT, (S *)d.hT, d.rho_b, d.dt, st));
+        PHASE(kPhCellMacro, Fused<S>::cell_macros(g, p, d.cell, d.dt, st));
+    }
     HIPCHK(hipGetLastError());
     return 0;
+}
+
+// cell equilibrium records of the fused schedules after the cell macros changed outside a step
+void refresh_cell_coef(dugks_handle *h)
+{
+    if (!h->fused) return;
+    if (h->sched == kSchedRecompute)
+        rc::Fused<double>::cell_coef(h->g, h->plan_rc, h->d.cell, h->stream);
+    else
+        Fused<double>::cell_coef(h->g, h->plan, h->d.cell, h->stream);
 }
 
 int do_step(dugks_handle *h)
@@ -263,13 +293,18 @@ int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patch
         if (mesh->neighbour[f] < 0 || mesh->neighbour[f] >= nC) return fail("dugks_create: neighbour label out of range");
 
     // Internal cell numbering: recursive coordinate bisection into blocks (contiguous in the
-    // internal numbering) of the largest size whose two halo rings fit the fused passes'
+    // internal numbering) of the largest size whose halo fits the chosen fused schedule's
     // work-group shapes (2-D meshes: 256 cells).  Faces keep the caller's numbering; every
     // cell-indexed array crossing the C-ABI is permuted on the way in and out.
+    const int sched = D == 2 ? kSchedRecompute : kSchedFaceValues;
     std::vector<int> perm, block_start, inv(nC);
     for (int max_block : {256, 192, 128, 96, 64}) {
         rcb_partition(nC, mesh->C, max_block, perm, block_start);
-        if (o.fused == 0 || block_tables_fit(nC, nIF, nF, mesh->owner, mesh->neighbour, D, perm, block_start)) break;
+        if (o.fused == 0) break;
+        const bool fit = sched == kSchedRecompute
+                             ? rc::block_tables_fit(nC, nIF, nF, mesh->owner, mesh->neighbour, D, perm, block_start)
+                             : block_tables_fit(nC, nIF, nF, mesh->owner, mesh->neighbour, D, perm, block_start);
+        if (fit) break;
     }
     for (int i = 0; i < nC; ++i) inv[perm[i]] = i;
     std::vector<int> own_p(nF), nei_p(nIF);
@@ -384,11 +419,17 @@ int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patch
     const size_t ncell = (size_t)Nv * nC, nface = (size_t)Nv * nF;
     if (dalloc_bytes(h, &d.gT, ncell * h->es) || dalloc_bytes(h, &d.hT, ncell * h->es)) return bail(1);
     h->fused = o.fused != 0;
+    h->sched = sched;
     if (h->fused) {
-        if (!fused_plan_build(h->plan, g, prep, block_start, h->es, 0, err) ||
-            !fused_plan_upload(h->plan, g, h->es, err)) {
+        const bool ok = sched == kSchedRecompute
+                            ? rc::fused_plan_build(h->plan_rc, g, prep, block_start, h->es, 0, err) &&
+                                  rc::fused_plan_upload(h->plan_rc, g, h->es, err)
+                            : fused_plan_build(h->plan, g, prep, block_start, h->es, 0, err) &&
+                                  fused_plan_upload(h->plan, g, h->es, err);
+        if (!ok) {
             if (o.fused == 1) return bail(fail("dugks_create: fused path unavailable: " + err));
             fused_plan_free(h->plan);
+            rc::fused_plan_free(h->plan_rc);
             h->fused = false;
         }
     }
@@ -445,7 +486,9 @@ int dugks_init_from_macros(dugks_handle *h, const double *rho, const double *U, 
             launch_init_mixed_faces<double>(g, (double *)d.gS, (double *)d.hS, st);
     } else {
         std::string err;
-        if (!fused_init_boundary(h->plan, g, d.cell, st, err)) return fail("dugks_init_from_macros: " + err);
+        const bool ok = h->sched == kSchedRecompute ? rc::fused_init_boundary(h->plan_rc, g, d.cell, st, err)
+                                                    : fused_init_boundary(h->plan, g, d.cell, st, err);
+        if (!ok) return fail("dugks_init_from_macros: " + err);
     }
     HIPCHK(hipMemcpyAsync(d.rho_b, g.bf_rho, sizeof(double) * g.nB, hipMemcpyDeviceToDevice, st));
     launch_init_usurf(g, d.cell, d.face, st);
@@ -628,7 +671,7 @@ int dugks_set_cell_macros(dugks_handle *h, const double *rho, const double *U, c
         upload(d.cell.T, Tp.data(), g.nC, h->stream) || upload(d.cell.q, qp.data(), g.nC, h->stream))
         return 1;
     launch_tau(g, d.cell, g.nC, h->stream);
-    if (h->fused) Fused<double>::cell_coef(g, h->plan, d.cell, h->stream);
+    refresh_cell_coef(h);
     HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
 }
@@ -727,7 +770,7 @@ int dugks_adjoint(dugks_handle *h, int n_steps, const double *dts, int checkpoin
     const int rc = h->es == 4 ? run_adjoint<float>(g, h->d, h->stream, io, err)
                               : run_adjoint<double>(g, h->d, h->stream, io, err);
     if (rc) return fail("dugks_adjoint: " + err);
-    if (h->fused) Fused<double>::cell_coef(g, h->plan, h->d.cell, h->stream);
+    refresh_cell_coef(h);
     HIPCHK(hipStreamSynchronize(h->stream));
     // distributions back to the caller's cell order
     for (int which = 0; which < 2; ++which) {
@@ -744,14 +787,24 @@ int dugks_checkpoint_save(dugks_handle *h, const char *path) { return ckpt_io(h,
 int dugks_checkpoint_load(dugks_handle *h, const char *path)
 {
     if (ckpt_io(h, path, false)) return 1;
-    if (h->fused) {
-        Fused<double>::cell_coef(h->g, h->plan, h->d.cell, h->stream);
-        HIPCHK(hipStreamSynchronize(h->stream));
-    }
+    refresh_cell_coef(h);
+    HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
 }
 
 int dugks_num_velocities(const dugks_handle *h) { return h ? h->g.Nv : 0; }
+
+int dugks_path_info(const dugks_handle *h, int *fused, int *block_cells, int *chunk_velocities,
+                    int *velocity_groups)
+{
+    if (!h) return fail("null handle");
+    const bool rc_s = h->sched == kSchedRecompute;
+    if (fused) *fused = h->fused ? h->sched : 0;
+    if (block_cells) *block_cells = !h->fused ? 0 : rc_s ? h->plan_rc.max_nc : h->plan.max_nc;
+    if (chunk_velocities) *chunk_velocities = !h->fused ? 0 : rc_s ? h->plan_rc.vc : h->plan.vc;
+    if (velocity_groups) *velocity_groups = !h->fused ? 0 : rc_s ? h->plan_rc.nvg : h->plan.nvg;
+    return 0;
+}
 
 int dugks_velocity_set(const dugks_handle *h, double *xi, double *w)
 {
@@ -805,6 +858,7 @@ void dugks_destroy(dugks_handle *h)
     for (auto &e : h->ev_pool)
         if (e) (void)hipEventDestroy(e);
     fused_plan_free(h->plan);
+    rc::fused_plan_free(h->plan_rc);
     for (void *p : h->allocs)
         if (p) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);

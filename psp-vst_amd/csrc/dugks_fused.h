@@ -1,18 +1,21 @@
-// Fused DUGKS step: two LDS-blocked gather passes over the distributions per step.
+// Fused DUGKS step: two LDS-blocked passes over the distributions per step.
 //
-//   pass A (face moments): per (cell block, velocity group) work-group, stage f-bar-plus
-//     of the block and its two halo rings in LDS, form least-squares gradients of the block
-//     and the first ring, reconstruct every face the block owns at x_f - xi dt/2 and
-//     accumulate FP64 raw velocity moments per face in registers;
-//   face macros: sum the per-group partials in a fixed order, add the incoming half of the
-//     wall / inlet faces, form rho, U, T, tau, q of every face;
-//   pass B (update): same staging, face fluxes relaxed towards the face equilibrium in
-//     LDS, cell-wise gather of the fluxes, new g~, h~ written to the second buffer, FP64
-//     raw moments of the new distributions per cell;
+//   pass A (faces): per (cell block, velocity group) work-group, stage f-bar-plus of the
+//     block and its halo ring in LDS, form least-squares gradients of the block cells,
+//     reconstruct every face of the block's cells from the side whose upwind cell is in
+//     the block, write the face value (storage precision) and accumulate FP64 raw velocity
+//     moments per (face, side) in registers;
+//   face macros: sum the per-group, per-side partials in a fixed order, add the incoming
+//     half of the wall / inlet faces, form rho, U, T, tau, q of every face;
+//   pass B (update): per (cell block, velocity group): f-bar-plus of the block cells in
+//     registers, face values relaxed towards the face equilibrium and turned into fluxes in
+//     LDS, cell-wise gather of the fluxes, new g~, h~ written in place, FP64 raw moments of
+//     the new distributions per cell;
 //   cell macros: sum the cell partials, form rho, U, T, tau, q of every cell.
 //
-// No atomics: every sum has a fixed association, so the FP64 path is bitwise
-// reproducible run to run.
+// A face value is written once per velocity, by the block holding its upwind cell; pass B
+// reads it for both adjacent cells.  No atomics: every sum has a fixed association, so the
+// FP64 path is bitwise reproducible run to run.
 #pragma once
 
 #include <string>
@@ -25,13 +28,16 @@ namespace dugks {
 
 // work-group shapes of the two passes (the planner checks the block tables against them)
 constexpr int kFusedThreadsA = 512;   // pass A threads
-constexpr int kFusedSlotsA = 2;       // owned faces per pass-A thread
+constexpr int kFusedSlotsA = 2;       // face slots per pass-A thread
 constexpr int kFusedThreadsB = 256;   // pass B threads = max cells per block
 constexpr int kFusedMaxNb = 8;        // max internal faces per gradient cell / faces per block cell
 constexpr int kFusedXcds = 8;         // accelerator dies; work-group ids are dealt round-robin over them
-// local cells per thread in stage 1 (pass A / pass B) and block faces per thread in pass B
+#ifndef DUGKS_PASS_B_VC
+#define DUGKS_PASS_B_VC 2
+#endif
+constexpr int kFusedVcB = DUGKS_PASS_B_VC;  // max velocities per pass-B chunk (register budget)
+// local cells (block + halo ring) per pass-A thread in stage 1, face slots per pass-B thread
 constexpr int fused_cells_per_thread_a(int D) { return D == 2 ? 1 : 2; }
-constexpr int fused_cells_per_thread_b(int D) { return D == 2 ? 2 : 4; }
 constexpr int fused_faces_per_thread_b(int D) { return D == 2 ? 3 : 4; }
 
 // Equilibrium coefficients of one cell or face, formed once per step by the macro kernels
@@ -51,14 +57,16 @@ struct alignas(16) BndCoef {
 
 struct BlockInfo {
     int c0, nc;        // block cells: permuted ids [c0, c0 + nc)
-    int l_off, nl;     // local cell table: lcell[l_off + l], l < nl (block, ring 1, ring 2)
-    int ng;            // local cells l < ng (block + ring 1) get a gradient
-    int gsb_off;       // gs_beg[gsb_off + l], l <= ng: stencil ranges (absolute)
-    int fa_off, nfa;   // faces owned by the block (owner in block), pass A
-    int fb_off, nfb;   // faces of the block cells, pass B
+    int l_off, nl;     // local cell table: lcell[l_off + l], l < nl (block cells first, then ring 1)
+    int gsb_off;       // gs_beg[gsb_off + l], l <= nc: gradient stencil ranges of the block cells
+    int fs_off, nfs;   // face slots: the faces of the block cells
     int fbc_off;       // fbc_beg[fbc_off + i], i <= nc: per block cell ranges into fbc_*
     int pad;
 };
+
+// face slot flags
+constexpr int kSlotOwnIn = 1;   // owner cell in the block
+constexpr int kSlotNeiIn = 2;   // neighbour cell in the block (internal faces)
 
 struct FusedPlan {
     bool ready = false;
@@ -67,36 +75,38 @@ struct FusedPlan {
     // Velocities are visited in chunks (x-stripe of vc abscissae, row = (iy, iz)); chunk id
     // ch -> stripe = ch / nrow, row = ch % nrow; a velocity group is a contiguous chunk range.
     int n1 = 0, nrow = 0, nstripe = 0, nchunk = 0, cpg = 0;
+    int vc_b = 0, nchunk_b = 0, cpg_b = 0, nvg_b = 0;  // the same for pass B (own chunk size)
     int b8 = 0;                        // blocks per XCD in the work-group mapping
     int wg_target = 4096;              // work-groups per pass the velocity grouping aims at
-    int max_nl = 0, max_ng = 0, max_nfa = 0, max_nfb = 0, max_nc = 0;
+    int max_nl = 0, max_nfs = 0, max_nc = 0;
     int nm = 0, nma = 0;               // raw moments per cell / per face (+ outgoing wall flux)
     size_t lds_a = 0, lds_b = 0;       // dynamic LDS bytes of the two passes
     // host tables
     std::vector<BlockInfo> blk;
-    std::vector<int> lcell, gs_beg, gs_nb, fa_face, fa_own, fa_nei, fb_face, fb_own, fb_nei, fbc_beg, fbc_idx;
-    std::vector<signed char> fbc_sgn;
+    std::vector<int> lcell, gs_beg, gs_nb, fs_face, fs_twin, fs_lo, fs_ln, fbc_beg, fbc_idx;
+    std::vector<signed char> fs_flag, fbc_sgn;
     std::vector<d3> gs_ls;
     // device tables
     BlockInfo *d_blk = nullptr;
-    int *d_lcell = nullptr, *d_gs_beg = nullptr, *d_gs_nb = nullptr, *d_fa_face = nullptr, *d_fa_own = nullptr,
-        *d_fa_nei = nullptr, *d_fb_face = nullptr, *d_fb_own = nullptr, *d_fb_nei = nullptr, *d_fbc_beg = nullptr,
-        *d_fbc_idx = nullptr;
-    signed char *d_fbc_sgn = nullptr;
+    int *d_lcell = nullptr, *d_gs_beg = nullptr, *d_gs_nb = nullptr, *d_fs_face = nullptr, *d_fs_twin = nullptr,
+        *d_fs_lo = nullptr, *d_fs_ln = nullptr, *d_fbc_beg = nullptr, *d_fbc_idx = nullptr;
+    signed char *d_fs_flag = nullptr, *d_fbc_sgn = nullptr;
     void *d_gs_ls = nullptr;           // d3 (FP64 path) or float3-as-3-floats (mixed)
-    double *fpart = nullptr;           // [nvg][nF][nma]
+    double *fpart = nullptr;           // [nvg][nF][2][nma]: per (face, side) partial moments
     double *cpart = nullptr;           // [nvg][nC][nm]
     double *bin = nullptr;             // [nB][nm] incoming moments (walls: per unit density)
     EqCoef *cc = nullptr, *fc = nullptr;  // equilibrium coefficients of cells / faces
     BndCoef *bc = nullptr;                // incoming-half coefficients of boundary faces
-    void *gT2 = nullptr, *hT2 = nullptr;  // second distribution buffer
+    // face values [Nv][slots]: written by the block that reconstructs them into its own slot
+    void *gF = nullptr, *hF = nullptr;
+    int n_slots = 0;
     std::vector<void *> allocs;
 };
 
 // Recursive coordinate bisection of the cells into blocks of at most max_block cells.
 // Returns the permutation (new -> old) and the block starts in the new numbering.
 void rcb_partition(int nC, const double *C, int max_block, std::vector<int> &perm, std::vector<int> &block_start);
-// True when every block with its two halo rings fits the work-group shapes of the passes
+// True when every block with its halo ring fits the work-group shapes of the passes
 // (owner / neighbour in the caller's numbering, perm new -> old).
 bool block_tables_fit(int nC, int nIF, int nF, const int *owner, const int *neighbour, int D,
                       const std::vector<int> &perm, const std::vector<int> &block_start);
@@ -111,12 +121,12 @@ void fused_plan_free(FusedPlan &p);
 
 template <class S>
 struct Fused {
-    static void face_pass(const Geo &g, FusedPlan &p, const S *gT, const S *hT, const Macros &cm, const double *dt,
-                          hipStream_t st);
+    static void face_pass(const Geo &g, FusedPlan &p, const S *gT, const S *hT, const double *dt, hipStream_t st);
     static void face_macros(const Geo &g, FusedPlan &p, Macros fm, double *rho_b, const double *dt, double *cfl_part,
                             hipStream_t st);
-    static void update_pass(const Geo &g, FusedPlan &p, const S *gT, const S *hT, S *gN, S *hN, const Macros &cm,
-                            const Macros &fm, const double *rho_b, const double *dt, hipStream_t st);
+    // g~, h~ are updated in place
+    static void update_pass(const Geo &g, FusedPlan &p, S *gT, S *hT, const double *rho_b, const double *dt,
+                            hipStream_t st);
     static void cell_macros(const Geo &g, FusedPlan &p, Macros cm, const double *dt, hipStream_t st);
     static void cell_coef(const Geo &g, FusedPlan &p, const Macros &cm, hipStream_t st);
 };

@@ -2,23 +2,30 @@
 //
 // Work-group = (cell block, velocity group).  Per chunk of up to VC velocities (one x-stripe
 // of a velocity row):
-//   stage 1  f-bar-plus of every block / ring-1 / ring-2 cell -> LDS
-//            (fvDVM::updateGHbarPvol, fvDVM.C:648-683; equilibrium in FP64)
-//   stage 2  least-squares gradient of every block / ring-1 cell -> LDS
-//            (leastSquaresGrad through discreteVelocity::updateGrad, discreteVelocity.C:366-369)
-//   stage 3A reconstruction of owned faces at x_f - xi dt/2 (fvDVM.C:824-963), FP64 raw
-//            moments accumulated per face in registers (fvDVM.C:1181-1297)
-//   stage 3B face values relaxed to the face equilibrium (fvDVM.C:1362-1460), incoming
-//            wall / inlet values (fvDVM.C:1054-1098, discreteVelocity.C:244-257), flux in LDS
-//   stage 4B g~ <- 4/3 f-bar-plus - 1/3 g~ - dt/V sum(+-flux) in ascending face order
-//            (fvDVM.C:1462-1519), FP64 raw moments of the new g~, h~ per cell
+//   pass A
+//     stage 1  f-bar-plus of the block cells and their halo ring -> LDS
+//              (fvDVM::updateGHbarPvol, fvDVM.C:648-683; equilibrium in FP64)
+//     stage 2  least-squares gradient of the block cells -> LDS
+//              (leastSquaresGrad through discreteVelocity::updateGrad, discreteVelocity.C:366-369)
+//     stage 3  faces of the block cells: reconstruction at x_f - xi dt/2 from the upwind cell
+//              when it is a block cell (fvDVM.C:824-963), value written for pass B, FP64 raw
+//              moments accumulated per (face, side) in registers (fvDVM.C:1181-1297)
+//   pass B
+//     stage 1  f-bar-plus of the block cells in registers
+//     stage 2  faces of the block cells: stored value relaxed to the face equilibrium
+//              (fvDVM.C:1362-1460) or the incoming wall / inlet value (fvDVM.C:1054-1098,
+//              discreteVelocity.C:244-257); flux sigma f in LDS (FP64)
+//     stage 3  g~ <- 4/3 f-bar-plus - 1/3 g~ - dt/V sum(+-flux) in ascending face order
+//              (fvDVM.C:1462-1519), in place; FP64 raw moments of the new g~, h~ per cell
 //
 // Each thread owns fixed cells / faces for the whole work-group.  The velocity set is a
 // tensor product, so the Maxwellian factor A exp(-(xi_x - U_x)^2 / 2RT) of an entity is
 // formed once per x-stripe and kept in registers; a chunk then needs one exp for the y/z
 // part per entity.  Work-group ids are mapped so that the velocity groups of a block run on
 // the same accelerator die and share its L2.  Reconstruction arithmetic runs in the storage
-// type S (FP32 for the mixed path); equilibria and moments in FP64.
+// type S (FP32 for the mixed path); equilibria, relaxation, update and moments in FP64.
+// Entries with |xi.S| < VSMALL carry no flux (sigma f = 0 in double precision); their face
+// value is the average of both sides' reconstructions, which only the face moments use.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,28 +49,39 @@ struct V3 {
 
 struct FArgs {
     const BlockInfo *blk;
-    const int *lcell, *gs_beg, *gs_nb, *fa_face, *fa_own, *fa_nei, *fb_face, *fb_own, *fb_nei, *fbc_beg, *fbc_idx;
-    const signed char *fbc_sgn;
+    const int *lcell, *gs_beg, *gs_nb, *fs_face, *fs_twin, *fs_lo, *fs_ln, *fbc_beg, *fbc_idx;
+    const signed char *fs_flag, *fbc_sgn;
     const void *gs_ls;
     double *fpart, *cpart;
     const double *bin;
     EqCoef *cc, *fc;
     const BndCoef *bc;
-    int nblk, b8, nvg, n1, nrow, nchunk, cpg;
-    int max_nl, max_ng, max_nfb, max_nc, nm, nma;
+    int nblk, b8, nvg, n1, nrow, nchunk, cpg, n_slots;
+    int max_nl, max_nfs, max_nc, nm, nma;
 };
 
 FArgs fargs(const FusedPlan &p)
 {
     FArgs a;
     a.blk = p.d_blk; a.lcell = p.d_lcell; a.gs_beg = p.d_gs_beg; a.gs_nb = p.d_gs_nb;
-    a.fa_face = p.d_fa_face; a.fa_own = p.d_fa_own; a.fa_nei = p.d_fa_nei;
-    a.fb_face = p.d_fb_face; a.fb_own = p.d_fb_own; a.fb_nei = p.d_fb_nei;
-    a.fbc_beg = p.d_fbc_beg; a.fbc_idx = p.d_fbc_idx; a.fbc_sgn = p.d_fbc_sgn; a.gs_ls = p.d_gs_ls;
+    a.fs_face = p.d_fs_face; a.fs_twin = p.d_fs_twin; a.fs_lo = p.d_fs_lo; a.fs_ln = p.d_fs_ln;
+    a.n_slots = p.n_slots;
+    a.fbc_beg = p.d_fbc_beg; a.fbc_idx = p.d_fbc_idx; a.fs_flag = p.d_fs_flag; a.fbc_sgn = p.d_fbc_sgn;
+    a.gs_ls = p.d_gs_ls;
     a.fpart = p.fpart; a.cpart = p.cpart; a.bin = p.bin; a.cc = p.cc; a.fc = p.fc; a.bc = p.bc;
     a.nblk = p.nblk; a.b8 = p.b8; a.nvg = p.nvg; a.n1 = p.n1; a.nrow = p.nrow; a.nchunk = p.nchunk; a.cpg = p.cpg;
-    a.max_nl = p.max_nl; a.max_ng = p.max_ng; a.max_nfb = p.max_nfb; a.max_nc = p.max_nc;
+    a.max_nl = p.max_nl; a.max_nfs = p.max_nfs; a.max_nc = p.max_nc;
     a.nm = p.nm; a.nma = p.nma;
+    return a;
+}
+
+// pass B (and the cell partials it writes) use their own velocity chunking
+FArgs fargs_b(const FusedPlan &p)
+{
+    FArgs a = fargs(p);
+    a.nvg = p.nvg_b;
+    a.cpg = p.cpg_b;
+    a.nchunk = p.nchunk_b;
     return a;
 }
 
@@ -234,52 +252,62 @@ __device__ inline Chunk chunk_of(const FArgs &a, int ch, int vc, int &stripe)
     return c;
 }
 
-// stage 1: f-bar-plus of local cells for one chunk; cex[s] holds the x-factors of the
-// thread's slot-s cell for the current stripe
+// f-bar-plus of cell c for the chunk's velocities (FP64), x-factors cex of the stripe kept
+// by the caller
+template <class S, int VC>
+__device__ inline void cell_bar(const Geo &g, const FArgs &a, int c, const S *__restrict__ gT,
+                                const S *__restrict__ hT, double dt, const Chunk &ck, const d3 &xr, double *cex,
+                                S *gt, S *ht, double *gb, double *hb)
+{
+    const EqCoef k = a.cc[c];
+#pragma unroll
+    for (int vv = 0; vv < VC; ++vv) {
+        const size_t kk = (size_t)(ck.vrow + ck.ix0 + min(vv, ck.nvv - 1)) * g.nC + c;
+        gt[vv] = gT[kk];
+        ht[vv] = hT[kk];
+    }
+    if (ck.new_stripe) stripe_factors<VC>(g, k, ck.ix0, a.n1, cex);
+    const double eyz = row_factor(k, xr);
+    const double af = 1.5 * dt / (2.0 * k.tr + dt);
+#pragma unroll
+    for (int vv = 0; vv < VC; ++vv) {
+        double gEq, hEq;
+        shakhov_poly(g, k, g.xi[ck.vrow + ck.ix0 + min(vv, ck.nvv - 1)], cex[vv] * eyz, gEq, hEq);
+        gb[vv] = (1.0 - af) * (double)gt[vv] + af * gEq;
+        hb[vv] = (1.0 - af) * (double)ht[vv] + af * hEq;
+    }
+}
+
+// pass A stage 1: f-bar-plus of the local cells (block + ring) into LDS
 template <class S, int VC, int NT, int CPT>
 __device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &bi, const S *__restrict__ gT,
                                  const S *__restrict__ hT, double dt, const Chunk &ck, double (&cex)[CPT][VC],
-                                 S *fbg, S *fbh, S *ftg, S *fth)
+                                 S *fbg, S *fbh)
 {
     const d3 xr = g.xi[ck.vrow];
 #pragma unroll
     for (int s = 0; s < CPT; ++s) {
         const int l = threadIdx.x + s * NT;
         if (l >= bi.nl) break;
-        const int c = a.lcell[bi.l_off + l];
-        const EqCoef k = a.cc[c];
         S gt[VC], ht[VC];
-#pragma unroll
-        for (int vv = 0; vv < VC; ++vv) {
-            const size_t kk = (size_t)(ck.vrow + ck.ix0 + min(vv, ck.nvv - 1)) * g.nC + c;
-            gt[vv] = gT[kk];
-            ht[vv] = hT[kk];
-        }
-        if (ck.new_stripe) stripe_factors<VC>(g, k, ck.ix0, a.n1, cex[s]);
-        const double eyz = row_factor(k, xr);
-        const double af = 1.5 * dt / (2.0 * k.tr + dt);
+        double gb[VC], hb[VC];
+        cell_bar<S, VC>(g, a, a.lcell[bi.l_off + l], gT, hT, dt, ck, xr, cex[s], gt, ht, gb, hb);
 #pragma unroll
         for (int vv = 0; vv < VC; ++vv) {
             if (vv >= ck.nvv) break;
-            double gEq, hEq;
-            shakhov_poly(g, k, g.xi[ck.vrow + ck.ix0 + vv], cex[s][vv] * eyz, gEq, hEq);
-            fbg[vv * a.max_nl + l] = (S)((1.0 - af) * (double)gt[vv] + af * gEq);
-            fbh[vv * a.max_nl + l] = (S)((1.0 - af) * (double)ht[vv] + af * hEq);
-            if (ftg && l < bi.nc) {
-                ftg[vv * a.max_nc + l] = gt[vv];
-                fth[vv * a.max_nc + l] = ht[vv];
-            }
+            fbg[vv * a.max_nl + l] = (S)gb[vv];
+            fbh[vv * a.max_nl + l] = (S)hb[vv];
         }
     }
 }
 
-// stage 2: gradients of block + ring-1 cells
+// pass A stage 2: gradients of the block cells
 template <class S, int D, int VC, int NT>
 __device__ inline void stage_grad(const FArgs &a, const BlockInfo &bi, int nvv, const S *fbg, const S *fbh, S *grg,
                                   S *grh)
 {
     const V3<S> *ls = (const V3<S> *)a.gs_ls;
-    for (int l = threadIdx.x; l < bi.ng; l += NT) {
+    for (int l = threadIdx.x; l < bi.nc; l += NT) {
         const int k0 = a.gs_beg[bi.gsb_off + l];
         const int n = a.gs_beg[bi.gsb_off + l + 1] - k0;
         int nb[kNb];
@@ -308,7 +336,7 @@ __device__ inline void stage_grad(const FArgs &a, const BlockInfo &bi, int nvv, 
                     }
                 }
             }
-            S *og = grg + (vv * a.max_ng + l) * D, *oh = grh + (vv * a.max_ng + l) * D;
+            S *og = grg + (vv * a.max_nc + l) * D, *oh = grh + (vv * a.max_nc + l) * D;
             og[0] = gx; og[1] = gy; oh[0] = hx; oh[1] = hy;
             if (D == 3) {
                 og[2] = gz;
@@ -329,7 +357,8 @@ __device__ inline S recon_l(const S *fb, const S *gr, int l, double ax, double a
 // ------------------------------------------------------------------------- pass A
 template <class S, int D, int VC>
 __global__ void __launch_bounds__(kNTA)
-    k_pass_a(Geo g, FArgs a, const S *__restrict__ gT, const S *__restrict__ hT, const double *dtp)
+    k_pass_a(Geo g, FArgs a, const S *__restrict__ gT, const S *__restrict__ hT, S *__restrict__ gF,
+             S *__restrict__ hF, const double *dtp)
 {
     constexpr int NMA = (D == 3 ? 17 : 11) + 1;
     constexpr int CPT = fused_cells_per_thread_a(D);
@@ -341,12 +370,12 @@ __global__ void __launch_bounds__(kNTA)
     S *fbg = (S *)smem;
     S *fbh = fbg + VC * a.max_nl;
     S *grg = fbh + VC * a.max_nl;
-    S *grh = grg + VC * a.max_ng * D;
+    S *grh = grg + VC * a.max_nc * D;
     const double dt = *dtp;
     const double hx = 0.5 * dt;
 
     double acc[kSlotsA][NMA];
-    int fid[kSlotsA], fown[kSlotsA], fnei[kSlotsA], ftyp[kSlotsA];
+    int fid[kSlotsA], fpos[kSlotsA], flo[kSlotsA], fln[kSlotsA], fflag[kSlotsA], ftyp[kSlotsA];
     d3 fS[kSlotsA], armO[kSlotsA], armN[kSlotsA];
 #pragma unroll
     for (int s = 0; s < kSlotsA; ++s) {
@@ -354,20 +383,22 @@ __global__ void __launch_bounds__(kNTA)
         for (int k = 0; k < NMA; ++k) acc[s][k] = 0.0;
         const int j = threadIdx.x + s * kNTA;
         fid[s] = -1;
-        fown[s] = fnei[s] = ftyp[s] = 0;
+        fpos[s] = flo[s] = fln[s] = fflag[s] = ftyp[s] = 0;
         fS[s] = armO[s] = armN[s] = d3{0, 0, 0};
-        if (j < bi.nfa) {
-            const int f = a.fa_face[bi.fa_off + j];
+        if (j < bi.nfs) {
+            const int f = a.fs_face[bi.fs_off + j];
             fid[s] = f;
-            fown[s] = a.fa_own[bi.fa_off + j];
-            fnei[s] = a.fa_nei[bi.fa_off + j];
+            fpos[s] = bi.fs_off + j;   // the block's own slot of the face
+            flo[s] = a.fs_lo[bi.fs_off + j];
+            fln[s] = a.fs_ln[bi.fs_off + j];
+            fflag[s] = a.fs_flag[bi.fs_off + j];
             ftyp[s] = f < g.nIF ? 0 : g.bf_type[f - g.nIF];
             fS[s] = g.Sf[f];
             const d3 Cf = g.Cf[f];
-            const d3 Co = g.C[a.lcell[bi.l_off + fown[s]]];
+            const d3 Co = g.C[a.lcell[bi.l_off + flo[s]]];
             armO[s] = d3{Cf.x - Co.x, Cf.y - Co.y, Cf.z - Co.z};
-            if (fnei[s] >= 0) {
-                const d3 Cn = g.C[a.lcell[bi.l_off + fnei[s]]];
+            if (fln[s] >= 0) {
+                const d3 Cn = g.C[a.lcell[bi.l_off + fln[s]]];
                 armN[s] = d3{Cf.x - Cn.x, Cf.y - Cn.y, Cf.z - Cn.z};
             } else {
                 armN[s] = armO[s];
@@ -379,13 +410,14 @@ __global__ void __launch_bounds__(kNTA)
     int stripe = -1;
     for (int ch = ch0; ch < ch1; ++ch) {
         const Chunk ck = chunk_of(a, ch, VC, stripe);
-        stage_bar<S, VC, kNTA, CPT>(g, a, bi, gT, hT, dt, ck, cex, fbg, fbh, (S *)nullptr, (S *)nullptr);
+        stage_bar<S, VC, kNTA, CPT>(g, a, bi, gT, hT, dt, ck, cex, fbg, fbh);
         __syncthreads();
         stage_grad<S, D, VC, kNTA>(a, bi, ck.nvv, fbg, fbh, grg, grh);
         __syncthreads();
 #pragma unroll
         for (int s = 0; s < kSlotsA; ++s) {
             if (fid[s] < 0) continue;
+            const bool own_in = fflag[s] & kSlotOwnIn, nei_in = fflag[s] & kSlotNeiIn;
 #pragma unroll 1
             for (int vv = 0; vv < ck.nvv; ++vv) {
                 const int v = ck.vrow + ck.ix0 + vv;
@@ -393,28 +425,49 @@ __global__ void __launch_bounds__(kNTA)
                 const double w = g.w[v];
                 const double xs = sigma_dot(xi, fS[s]);
                 const S *pg = fbg + vv * a.max_nl, *ph = fbh + vv * a.max_nl;
-                const S *qg = grg + vv * a.max_ng * D, *qh = grh + vv * a.max_ng * D;
-                const double ox = armO[s].x - xi.x * hx, oy = armO[s].y - xi.y * hx, oz = armO[s].z - xi.z * hx;
+                const S *qg = grg + vv * a.max_nc * D, *qh = grh + vv * a.max_nc * D;
                 double gf, hf;
+                bool store = true;
                 if (ftyp[s] == 0) {
-                    const double nx = armN[s].x - xi.x * hx, ny = armN[s].y - xi.y * hx, nz = armN[s].z - xi.z * hx;
                     if (xs >= kVSmall) {
-                        gf = (double)recon_l<S, D>(pg, qg, fown[s], ox, oy, oz);
-                        hf = (double)recon_l<S, D>(ph, qh, fown[s], ox, oy, oz);
+                        if (!own_in) continue;
+                        const double ox = armO[s].x - xi.x * hx, oy = armO[s].y - xi.y * hx, oz = armO[s].z - xi.z * hx;
+                        gf = (double)recon_l<S, D>(pg, qg, flo[s], ox, oy, oz);
+                        hf = (double)recon_l<S, D>(ph, qh, flo[s], ox, oy, oz);
                     } else if (xs < -kVSmall) {
-                        gf = (double)recon_l<S, D>(pg, qg, fnei[s], nx, ny, nz);
-                        hf = (double)recon_l<S, D>(ph, qh, fnei[s], nx, ny, nz);
+                        if (!nei_in) continue;
+                        const double nx = armN[s].x - xi.x * hx, ny = armN[s].y - xi.y * hx, nz = armN[s].z - xi.z * hx;
+                        gf = (double)recon_l<S, D>(pg, qg, fln[s], nx, ny, nz);
+                        hf = (double)recon_l<S, D>(ph, qh, fln[s], nx, ny, nz);
                     } else {
-                        gf = 0.5 * ((double)recon_l<S, D>(pg, qg, fnei[s], nx, ny, nz) +
-                                    (double)recon_l<S, D>(pg, qg, fown[s], ox, oy, oz));
-                        hf = 0.5 * ((double)recon_l<S, D>(ph, qh, fnei[s], nx, ny, nz) +
-                                    (double)recon_l<S, D>(ph, qh, fown[s], ox, oy, oz));
+                        // no flux; each side adds half of its reconstruction to the moments
+                        store = false;
+                        const double ox = armO[s].x - xi.x * hx, oy = armO[s].y - xi.y * hx, oz = armO[s].z - xi.z * hx;
+                        const double nx = armN[s].x - xi.x * hx, ny = armN[s].y - xi.y * hx, nz = armN[s].z - xi.z * hx;
+                        if (own_in && nei_in) {
+                            gf = 0.5 * ((double)recon_l<S, D>(pg, qg, fln[s], nx, ny, nz) +
+                                        (double)recon_l<S, D>(pg, qg, flo[s], ox, oy, oz));
+                            hf = 0.5 * ((double)recon_l<S, D>(ph, qh, fln[s], nx, ny, nz) +
+                                        (double)recon_l<S, D>(ph, qh, flo[s], ox, oy, oz));
+                        } else if (own_in) {
+                            gf = 0.5 * (double)recon_l<S, D>(pg, qg, flo[s], ox, oy, oz);
+                            hf = 0.5 * (double)recon_l<S, D>(ph, qh, flo[s], ox, oy, oz);
+                        } else {
+                            gf = 0.5 * (double)recon_l<S, D>(pg, qg, fln[s], nx, ny, nz);
+                            hf = 0.5 * (double)recon_l<S, D>(ph, qh, fln[s], nx, ny, nz);
+                        }
                     }
                 } else {
                     if (!(xs > 0)) continue;  // incoming half added from constants
-                    gf = (double)recon_l<S, D>(pg, qg, fown[s], ox, oy, oz);
-                    hf = (double)recon_l<S, D>(ph, qh, fown[s], ox, oy, oz);
+                    const double ox = armO[s].x - xi.x * hx, oy = armO[s].y - xi.y * hx, oz = armO[s].z - xi.z * hx;
+                    gf = (double)recon_l<S, D>(pg, qg, flo[s], ox, oy, oz);
+                    hf = (double)recon_l<S, D>(ph, qh, flo[s], ox, oy, oz);
                     if (ftyp[s] == kMaxwellWall) acc[s][NMA - 1] += w * xs * gf;
+                }
+                if (store) {
+                    const size_t kf = (size_t)v * a.n_slots + fpos[s];
+                    gF[kf] = (S)gf;
+                    hF[kf] = (S)hf;
                 }
                 add_moments<D>(acc[s], w * gf, w * hf, xi);
             }
@@ -424,7 +477,8 @@ __global__ void __launch_bounds__(kNTA)
 #pragma unroll
     for (int s = 0; s < kSlotsA; ++s) {
         if (fid[s] < 0) continue;
-        double *o = a.fpart + ((size_t)vg * g.nF + fid[s]) * NMA;
+        const int side = (fflag[s] & kSlotOwnIn) ? 0 : 1;
+        double *o = a.fpart + (((size_t)vg * g.nF + fid[s]) * 2 + side) * NMA;
 #pragma unroll
         for (int k = 0; k < NMA; ++k) o[k] = acc[s][k];
     }
@@ -443,9 +497,11 @@ __global__ void __launch_bounds__(kBlk)
 #pragma unroll
     for (int k = 0; k < NMA; ++k) m[k] = 0.0;
     for (int vg = 0; vg < a.nvg; ++vg) {
-        const double *p = a.fpart + ((size_t)vg * g.nF + f) * NMA;
+        const double *p = a.fpart + ((size_t)vg * g.nF + f) * 2 * NMA;
 #pragma unroll
         for (int k = 0; k < NMA; ++k) m[k] += p[k];
+#pragma unroll
+        for (int k = 0; k < NMA; ++k) m[k] += p[NMA + k];
     }
     if (f >= g.nIF) {
         const int b = f - g.nIF;
@@ -486,27 +542,19 @@ __global__ void __launch_bounds__(kBlk) k_face_coef(Geo g, Macros fm, EqCoef *fc
 // ------------------------------------------------------------------------- pass B
 template <class S, int D, int VC>
 __global__ void __launch_bounds__(kNTB)
-    k_pass_b(Geo g, FArgs a, const S *__restrict__ gT, const S *__restrict__ hT, S *__restrict__ gN,
-             S *__restrict__ hN, const double *rho_b, const double *dtp)
+    k_pass_b(Geo g, FArgs a, S *gT, S *hT, const S *__restrict__ gF, const S *__restrict__ hF, const double *rho_b,
+             const double *dtp)
 {
     constexpr int NM = D == 3 ? 17 : 11;
-    constexpr int CPT = fused_cells_per_thread_b(D);
     constexpr int FPT = fused_faces_per_thread_b(D);
     int blkid, vg;
     if (!wg_map(a, blkid, vg)) return;
     extern __shared__ __align__(16) unsigned char smem[];
     const BlockInfo bi = a.blk[blkid];
     const int ch0 = vg * a.cpg, ch1 = min(a.nchunk, ch0 + a.cpg);
-    double *flg = (double *)smem;                   // [VC][max_nfb]
-    double *flh = flg + VC * a.max_nfb;
-    S *fbg = (S *)(flh + VC * a.max_nfb);
-    S *fbh = fbg + VC * a.max_nl;
-    S *grg = fbh + VC * a.max_nl;
-    S *grh = grg + VC * a.max_ng * D;
-    S *ftg = grh + VC * a.max_ng * D;
-    S *fth = ftg + VC * a.max_nc;
+    double *flg = (double *)smem;                   // [VC][max_nfs]
+    double *flh = flg + VC * a.max_nfs;
     const double dt = *dtp;
-    const double hx = 0.5 * dt;
 
     double mom[NM];
 #pragma unroll
@@ -531,33 +579,26 @@ __global__ void __launch_bounds__(kNTB)
         }
     }
 
-    double cex[CPT][VC], fex[FPT][VC];
+    double cex[VC], fex[FPT][VC];
     int stripe = -1;
     for (int ch = ch0; ch < ch1; ++ch) {
         const Chunk ck = chunk_of(a, ch, VC, stripe);
-        stage_bar<S, VC, kNTB, CPT>(g, a, bi, gT, hT, dt, ck, cex, fbg, fbh, ftg, fth);
-        __syncthreads();
-        stage_grad<S, D, VC, kNTB>(a, bi, ck.nvv, fbg, fbh, grg, grh);
-        __syncthreads();
-        // stage 3: face values -> flux sigma * f_f in LDS
         const d3 xr = g.xi[ck.vrow];
+        // stage 1: f-bar-plus of the thread's cell
+        S gt[VC], ht[VC];
+        double gb[VC], hb[VC];
+        if (has_cell) cell_bar<S, VC>(g, a, cell, gT, hT, dt, ck, xr, cex, gt, ht, gb, hb);
+        // stage 2: face values -> flux sigma * f_f in LDS
 #pragma unroll
         for (int s = 0; s < FPT; ++s) {
             const int j = threadIdx.x + s * kNTB;
-            if (j >= bi.nfb) break;
-            const int f = a.fb_face[bi.fb_off + j];
-            const int lo = a.fb_own[bi.fb_off + j], ln = a.fb_nei[bi.fb_off + j];
+            if (j >= bi.nfs) break;
+            const int f = a.fs_face[bi.fs_off + j];
+            // the value sits in the slot of the block holding the upwind cell: this block's
+            // slot, or the twin slot of the adjacent block
+            const int own_slot = bi.fs_off + j, twin = a.fs_twin[own_slot];
+            const int flag = a.fs_flag[own_slot];
             const d3 Sf = g.Sf[f];
-            const d3 Cf = g.Cf[f];
-            const d3 Co = g.C[a.lcell[bi.l_off + lo]];
-            const double aox = Cf.x - Co.x, aoy = Cf.y - Co.y, aoz = Cf.z - Co.z;
-            double anx = aox, any = aoy, anz = aoz;
-            if (ln >= 0) {
-                const d3 Cn = g.C[a.lcell[bi.l_off + ln]];
-                anx = Cf.x - Cn.x;
-                any = Cf.y - Cn.y;
-                anz = Cf.z - Cn.z;
-            }
             const EqCoef k = a.fc[f];
             if (ck.new_stripe) stripe_factors<VC>(g, k, ck.ix0, a.n1, fex[s]);
             const double eyz = row_factor(k, xr);
@@ -565,61 +606,51 @@ __global__ void __launch_bounds__(kNTB)
 #pragma unroll
             for (int vv = 0; vv < VC; ++vv) {
                 if (vv >= ck.nvv) break;
-                const d3 xi = g.xi[ck.vrow + ck.ix0 + vv];
+                const int v = ck.vrow + ck.ix0 + vv;
+                const d3 xi = g.xi[v];
                 const double xs = sigma_dot(xi, Sf);
-                const S *pg = fbg + vv * a.max_nl, *ph = fbh + vv * a.max_nl;
-                const S *qg = grg + vv * a.max_ng * D, *qh = grh + vv * a.max_ng * D;
-                const double ox = aox - xi.x * hx, oy = aoy - xi.y * hx, oz = aoz - xi.z * hx;
-                double gv, hv;
-                bool relax = true;
+                double fg = 0.0, fh = 0.0;
                 if (b < 0) {
-                    const double nx = anx - xi.x * hx, ny = any - xi.y * hx, nz = anz - xi.z * hx;
-                    if (xs >= kVSmall) {
-                        gv = (double)recon_l<S, D>(pg, qg, lo, ox, oy, oz);
-                        hv = (double)recon_l<S, D>(ph, qh, lo, ox, oy, oz);
-                    } else if (xs < -kVSmall) {
-                        gv = (double)recon_l<S, D>(pg, qg, ln, nx, ny, nz);
-                        hv = (double)recon_l<S, D>(ph, qh, ln, nx, ny, nz);
-                    } else {
-                        gv = 0.5 * ((double)recon_l<S, D>(pg, qg, ln, nx, ny, nz) +
-                                    (double)recon_l<S, D>(pg, qg, lo, ox, oy, oz));
-                        hv = 0.5 * ((double)recon_l<S, D>(ph, qh, ln, nx, ny, nz) +
-                                    (double)recon_l<S, D>(ph, qh, lo, ox, oy, oz));
+                    if (xs >= kVSmall || xs < -kVSmall) {
+                        const bool mine = xs >= kVSmall ? (flag & kSlotOwnIn) : (flag & kSlotNeiIn);
+                        const size_t kf = (size_t)v * a.n_slots + (mine ? own_slot : twin);
+                        double gEq, hEq;
+                        shakhov_poly(g, k, xi, fex[s][vv] * eyz, gEq, hEq);
+                        fg = xs * ((1.0 - k.tr) * (double)gF[kf] + k.tr * gEq);
+                        fh = xs * ((1.0 - k.tr) * (double)hF[kf] + k.tr * hEq);
                     }
                 } else if (xs > 0) {
-                    gv = (double)recon_l<S, D>(pg, qg, lo, ox, oy, oz);
-                    hv = (double)recon_l<S, D>(ph, qh, lo, ox, oy, oz);
+                    const size_t kf = (size_t)v * a.n_slots + own_slot;
+                    double gEq, hEq;
+                    shakhov_poly(g, k, xi, fex[s][vv] * eyz, gEq, hEq);
+                    fg = xs * ((1.0 - k.tr) * (double)gF[kf] + k.tr * gEq);
+                    fh = xs * ((1.0 - k.tr) * (double)hF[kf] + k.tr * hEq);
                 } else {
-                    relax = false;
                     const BndCoef kb = a.bc[b];
                     const double scale = g.bf_type[b] == kMaxwellWall ? rho_b[b] : 1.0;
                     const double cx = xi.x - kb.ux, cy = xi.y - kb.uy, cz = xi.z - kb.uz;
-                    gv = scale * (kb.A * exp(-((cx * cx + cy * cy + cz * cz) * kb.invRT) / 2.0));
-                    hv = gv * kb.hfac;
+                    const double gv = scale * (kb.A * exp(-((cx * cx + cy * cy + cz * cz) * kb.invRT) / 2.0));
+                    fg = xs * gv;
+                    fh = xs * (gv * kb.hfac);
                 }
-                if (relax) {
-                    double gEq, hEq;
-                    shakhov_poly(g, k, xi, fex[s][vv] * eyz, gEq, hEq);
-                    gv = (1.0 - k.tr) * gv + k.tr * gEq;
-                    hv = (1.0 - k.tr) * hv + k.tr * hEq;
-                }
-                flg[vv * a.max_nfb + j] = xs * gv;
-                flh[vv * a.max_nfb + j] = xs * hv;
+                flg[vv * a.max_nfs + j] = fg;
+                flh[vv * a.max_nfs + j] = fh;
             }
         }
         __syncthreads();
-        // stage 4: cell update in ascending face order, new values to the other buffer
+        // stage 3: cell update in ascending face order, in place
         if (has_cell) {
-#pragma unroll 1
-            for (int vv = 0; vv < ck.nvv; ++vv) {
+#pragma unroll
+            for (int vv = 0; vv < VC; ++vv) {
+                if (vv >= ck.nvv) break;
                 const int v = ck.vrow + ck.ix0 + vv;
-                double ag = -1.0 / 3 * (double)ftg[vv * a.max_nc + my] + 4.0 / 3 * (double)fbg[vv * a.max_nl + my];
-                double ah = -1.0 / 3 * (double)fth[vv * a.max_nc + my] + 4.0 / 3 * (double)fbh[vv * a.max_nl + my];
+                double ag = -1.0 / 3 * (double)gt[vv] + 4.0 / 3 * gb[vv];
+                double ah = -1.0 / 3 * (double)ht[vv] + 4.0 / 3 * hb[vv];
 #pragma unroll
                 for (int j = 0; j < kNb; ++j) {
                     if (j < nf) {
-                        const double tg = flg[vv * a.max_nfb + fidx[j]] * dtV;
-                        const double th = flh[vv * a.max_nfb + fidx[j]] * dtV;
+                        const double tg = flg[vv * a.max_nfs + fidx[j]] * dtV;
+                        const double th = flh[vv * a.max_nfs + fidx[j]] * dtV;
                         if (fsg[j] > 0) {
                             ag += tg;
                             ah += th;
@@ -631,8 +662,8 @@ __global__ void __launch_bounds__(kNTB)
                 }
                 const S sg = (S)ag, sh = (S)ah;
                 const size_t kk = (size_t)v * g.nC + cell;
-                gN[kk] = sg;
-                hN[kk] = sh;
+                gT[kk] = sg;
+                hT[kk] = sh;
                 const double w = g.w[v];
                 add_moments<D>(mom, w * (double)sg, w * (double)sh, g.xi[v]);
             }
@@ -714,14 +745,10 @@ __global__ void __launch_bounds__(kBlk) k_incoming(Geo g, double *bin, BndCoef *
 template <class S>
 size_t lds_pass_a(const FusedPlan &p, int D, int VC)
 {
-    return (size_t)VC * (2 * p.max_nl + 2 * p.max_ng * D) * sizeof(S);
+    return (size_t)VC * (2 * p.max_nl + 2 * p.max_nc * D) * sizeof(S);
 }
-template <class S>
-size_t lds_pass_b(const FusedPlan &p, int D, int VC)
-{
-    return (size_t)VC * 2 * p.max_nfb * sizeof(double) + lds_pass_a<S>(p, D, VC) +
-           (size_t)VC * 2 * p.max_nc * sizeof(S);
-}
+size_t lds_pass_b(const FusedPlan &p, int VC) { return (size_t)VC * 2 * p.max_nfs * sizeof(double); }
+inline int vc_for_b(int vc) { return std::min(vc, kFusedVcB); }
 
 constexpr size_t kLdsBudget = 80 * 1024;
 
@@ -729,26 +756,27 @@ template <class S>
 int choose_vc(const FusedPlan &p, int D)
 {
     for (int vc : {4, 2, 1})
-        if (lds_pass_b<S>(p, D, vc) <= kLdsBudget) return vc;
+        if (lds_pass_a<S>(p, D, vc) <= kLdsBudget && lds_pass_b(p, vc_for_b(vc)) <= kLdsBudget) return vc;
     return 0;
 }
 
-inline int grid_ab(const FusedPlan &p) { return kFusedXcds * p.b8 * p.nvg; }
+inline int grid_a(const FusedPlan &p) { return kFusedXcds * p.b8 * p.nvg; }
+inline int grid_b(const FusedPlan &p) { return kFusedXcds * p.b8 * p.nvg_b; }
 
 template <class S, int D, int VC>
 void launch_a(const Geo &g, FusedPlan &p, const S *gT, const S *hT, const double *dt, hipStream_t st)
 {
     const size_t lds = lds_pass_a<S>(p, D, VC);
-    hipLaunchKernelGGL((k_pass_a<S, D, VC>), dim3(grid_ab(p)), dim3(kNTA), lds, st, g, fargs(p), gT, hT, dt);
+    hipLaunchKernelGGL((k_pass_a<S, D, VC>), dim3(grid_a(p)), dim3(kNTA), lds, st, g, fargs(p), gT, hT, (S *)p.gF,
+                       (S *)p.hF, dt);
 }
 
 template <class S, int D, int VC>
-void launch_b(const Geo &g, FusedPlan &p, const S *gT, const S *hT, S *gN, S *hN, const double *rho_b,
-              const double *dt, hipStream_t st)
+void launch_b(const Geo &g, FusedPlan &p, S *gT, S *hT, const double *rho_b, const double *dt, hipStream_t st)
 {
-    const size_t lds = lds_pass_b<S>(p, D, VC);
-    hipLaunchKernelGGL((k_pass_b<S, D, VC>), dim3(grid_ab(p)), dim3(kNTB), lds, st, g, fargs(p), gT, hT, gN, hN,
-                       rho_b, dt);
+    const size_t lds = lds_pass_b(p, VC);
+    hipLaunchKernelGGL((k_pass_b<S, D, VC>), dim3(grid_b(p)), dim3(kNTB), lds, st, g, fargs_b(p), gT, hT,
+                       (const S *)p.gF, (const S *)p.hF, rho_b, dt);
 }
 
 template <class S>
@@ -759,16 +787,20 @@ bool set_lds_attrs(int D, int vc, const FusedPlan &p)
     };
     bool ok = true;
 #define DUGKS_SET(DD, VV)                                                                              \
-    if (D == DD && vc == VV) {                                                                         \
-        ok &= set((const void *)k_pass_a<S, DD, VV>, lds_pass_a<S>(p, DD, VV));                        \
-        ok &= set((const void *)k_pass_b<S, DD, VV>, lds_pass_b<S>(p, DD, VV));                        \
-    }
+    if (D == DD && vc == VV) ok &= set((const void *)k_pass_a<S, DD, VV>, lds_pass_a<S>(p, DD, VV));   \
+    if (D == DD && vc_for_b(vc) == VV) ok &= set((const void *)k_pass_b<S, DD, VV>, lds_pass_b(p, VV));
     DUGKS_SET(2, 1) DUGKS_SET(2, 2) DUGKS_SET(2, 4) DUGKS_SET(3, 1) DUGKS_SET(3, 2) DUGKS_SET(3, 4)
 #undef DUGKS_SET
     return ok;
 }
 
 inline dim3 grid1(int n) { return dim3((n + kBlk - 1) / kBlk); }
+
+__global__ void __launch_bounds__(kBlk) k_zero_d(double *p, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * kBlk + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
 
 }  // namespace
 
@@ -815,6 +847,11 @@ void fused_velocity_groups(FusedPlan &p, int D, int n1, int vc)
     const int want = std::max(1, std::min(p.nchunk, (p.wg_target + p.nblk - 1) / p.nblk));
     p.cpg = (p.nchunk + want - 1) / want;
     p.nvg = (p.nchunk + p.cpg - 1) / p.cpg;
+    p.vc_b = vc_for_b(vc);
+    p.nchunk_b = ((n1 + p.vc_b - 1) / p.vc_b) * p.nrow;
+    const int want_b = std::max(1, std::min(p.nchunk_b, (p.wg_target + p.nblk - 1) / p.nblk));
+    p.cpg_b = (p.nchunk_b + want_b - 1) / want_b;
+    p.nvg_b = (p.nchunk_b + p.cpg_b - 1) / p.cpg_b;
     p.b8 = (p.nblk + kFusedXcds - 1) / kFusedXcds;
 }
 
@@ -835,10 +872,9 @@ bool fused_plan_upload(FusedPlan &p, const Geo &g, size_t es, std::string &err)
         return false;
     }
     if (!up(p, &p.d_blk, p.blk, err) || !up(p, &p.d_lcell, p.lcell, err) || !up(p, &p.d_gs_beg, p.gs_beg, err) ||
-        !up(p, &p.d_gs_nb, p.gs_nb, err) || !up(p, &p.d_fa_face, p.fa_face, err) ||
-        !up(p, &p.d_fa_own, p.fa_own, err) || !up(p, &p.d_fa_nei, p.fa_nei, err) ||
-        !up(p, &p.d_fb_face, p.fb_face, err) || !up(p, &p.d_fb_own, p.fb_own, err) ||
-        !up(p, &p.d_fb_nei, p.fb_nei, err) || !up(p, &p.d_fbc_beg, p.fbc_beg, err) ||
+        !up(p, &p.d_gs_nb, p.gs_nb, err) || !up(p, &p.d_fs_face, p.fs_face, err) ||
+        !up(p, &p.d_fs_twin, p.fs_twin, err) || !up(p, &p.d_fs_lo, p.fs_lo, err) || !up(p, &p.d_fs_ln, p.fs_ln, err) ||
+        !up(p, &p.d_fs_flag, p.fs_flag, err) || !up(p, &p.d_fbc_beg, p.fbc_beg, err) ||
         !up(p, &p.d_fbc_idx, p.fbc_idx, err) || !up(p, &p.d_fbc_sgn, p.fbc_sgn, err))
         return false;
     if (es == 4) {
@@ -853,10 +889,11 @@ bool fused_plan_upload(FusedPlan &p, const Geo &g, size_t es, std::string &err)
         p.d_gs_ls = d;
     }
     void *fp = nullptr, *cp = nullptr, *bi = nullptr, *cc = nullptr, *fc = nullptr, *bc = nullptr;
-    if (!dbuf(p, &fp, (size_t)p.nvg * g.nF * p.nma * 8, err) || !dbuf(p, &cp, (size_t)p.nvg * g.nC * p.nm * 8, err) ||
+    const size_t nfpart = (size_t)p.nvg * g.nF * 2 * p.nma;
+    if (!dbuf(p, &fp, nfpart * 8, err) || !dbuf(p, &cp, (size_t)p.nvg_b * g.nC * p.nm * 8, err) ||
         !dbuf(p, &bi, (size_t)g.nB * p.nm * 8, err) || !dbuf(p, &cc, (size_t)g.nC * sizeof(EqCoef), err) ||
         !dbuf(p, &fc, (size_t)g.nF * sizeof(EqCoef), err) || !dbuf(p, &bc, (size_t)g.nB * sizeof(BndCoef), err) ||
-        !dbuf(p, &p.gT2, (size_t)g.Nv * g.nC * es, err) || !dbuf(p, &p.hT2, (size_t)g.Nv * g.nC * es, err))
+        !dbuf(p, &p.gF, (size_t)g.Nv * p.n_slots * es, err) || !dbuf(p, &p.hF, (size_t)g.Nv * p.n_slots * es, err))
         return false;
     p.fpart = (double *)fp;
     p.cpart = (double *)cp;
@@ -864,8 +901,14 @@ bool fused_plan_upload(FusedPlan &p, const Geo &g, size_t es, std::string &err)
     p.cc = (EqCoef *)cc;
     p.fc = (EqCoef *)fc;
     p.bc = (BndCoef *)bc;
+    // (face, side) slots no block writes (the second side of a face inside one block) stay 0
+    hipLaunchKernelGGL(k_zero_d, dim3((unsigned)((nfpart + kBlk - 1) / kBlk)), dim3(kBlk), 0, 0, p.fpart, nfpart);
+    if (hipDeviceSynchronize() != hipSuccess) {
+        err = "clearing the face partials failed";
+        return false;
+    }
     p.lds_a = es == 4 ? lds_pass_a<float>(p, g.D, vc) : lds_pass_a<double>(p, g.D, vc);
-    p.lds_b = es == 4 ? lds_pass_b<float>(p, g.D, vc) : lds_pass_b<double>(p, g.D, vc);
+    p.lds_b = lds_pass_b(p, p.vc_b);
     return true;
 }
 
@@ -896,8 +939,7 @@ void fused_plan_free(FusedPlan &p)
 // ------------------------------------------------------------------- launchers
 
 template <class S>
-void Fused<S>::face_pass(const Geo &g, FusedPlan &p, const S *gT, const S *hT, const Macros &, const double *dt,
-                         hipStream_t st)
+void Fused<S>::face_pass(const Geo &g, FusedPlan &p, const S *gT, const S *hT, const double *dt, hipStream_t st)
 {
 #define DUGKS_A(DD, VV) \
     if (g.D == DD && p.vc == VV) return launch_a<S, DD, VV>(g, p, gT, hT, dt, st);
@@ -918,11 +960,11 @@ void Fused<S>::face_macros(const Geo &g, FusedPlan &p, Macros fm, double *rho_b,
 }
 
 template <class S>
-void Fused<S>::update_pass(const Geo &g, FusedPlan &p, const S *gT, const S *hT, S *gN, S *hN, const Macros &,
-                           const Macros &, const double *rho_b, const double *dt, hipStream_t st)
+void Fused<S>::update_pass(const Geo &g, FusedPlan &p, S *gT, S *hT, const double *rho_b, const double *dt,
+                           hipStream_t st)
 {
 #define DUGKS_B(DD, VV) \
-    if (g.D == DD && p.vc == VV) return launch_b<S, DD, VV>(g, p, gT, hT, gN, hN, rho_b, dt, st);
+    if (g.D == DD && p.vc_b == VV) return launch_b<S, DD, VV>(g, p, gT, hT, rho_b, dt, st);
     DUGKS_B(2, 1) DUGKS_B(2, 2) DUGKS_B(2, 4) DUGKS_B(3, 1) DUGKS_B(3, 2) DUGKS_B(3, 4)
 #undef DUGKS_B
 }
@@ -931,9 +973,9 @@ template <class S>
 void Fused<S>::cell_macros(const Geo &g, FusedPlan &p, Macros cm, const double *dt, hipStream_t st)
 {
     if (g.D == 3)
-        hipLaunchKernelGGL(k_cell_macros<3>, grid1(g.nC), dim3(kBlk), 0, st, g, fargs(p), cm, dt);
+        hipLaunchKernelGGL(k_cell_macros<3>, grid1(g.nC), dim3(kBlk), 0, st, g, fargs_b(p), cm, dt);
     else
-        hipLaunchKernelGGL(k_cell_macros<2>, grid1(g.nC), dim3(kBlk), 0, st, g, fargs(p), cm, dt);
+        hipLaunchKernelGGL(k_cell_macros<2>, grid1(g.nC), dim3(kBlk), 0, st, g, fargs_b(p), cm, dt);
 }
 
 template <class S>

@@ -1,6 +1,7 @@
-// Host-side planning of the fused path: cell blocks by recursive coordinate bisection,
-// the two halo rings of each block, and the block-local tables (gradient stencils, owned
-// faces, faces of the block cells with their gather order) that the kernels follow.
+// Host-side planning of the fused path: cell blocks by recursive coordinate bisection, the
+// halo ring of each block, and the block-local tables (gradient stencils of the block
+// cells, the faces of the block cells with their gather order, the twin slot of every face
+// two blocks share) that the kernels follow.
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -46,6 +47,16 @@ void rcb(std::vector<int> &ids, int lo, int hi, const double *C, int max_block, 
     rcb(ids, mid, hi, C, max_block, starts);
 }
 
+int max_local_cells(int D)
+{
+    return kFusedThreadsA * fused_cells_per_thread_a(D);
+}
+
+int max_face_slots(int D)
+{
+    return std::min(kFusedThreadsA * kFusedSlotsA, kFusedThreadsB * fused_faces_per_thread_b(D));
+}
+
 }  // namespace
 
 void rcb_partition(int nC, const double *C, int max_block, std::vector<int> &perm, std::vector<int> &block_start)
@@ -83,34 +94,27 @@ bool block_tables_fit(int nC, int nIF, int nF, const int *owner, const int *neig
         fc[fpos[owner[f]]++] = f;
         if (f < nIF) fc[fpos[neighbour[f]]++] = f;
     }
-    const int max_nl = std::min(kFusedThreadsA * fused_cells_per_thread_a(D), kFusedThreadsB * fused_cells_per_thread_b(D));
-    const int max_nfb = kFusedThreadsB * fused_faces_per_thread_b(D);
     std::vector<int> mark(nC, -1), fmark(nF, -1);
     const int nblk = (int)block_start.size() - 1;
     for (int b = 0; b < nblk; ++b) {
-        std::vector<int> ring(perm.begin() + block_start[b], perm.begin() + block_start[b + 1]);
-        if ((int)ring.size() > kFusedThreadsB) return false;
-        int nl = (int)ring.size(), nfb = 0;
-        for (int c : ring) {
-            mark[c] = b;
+        const int c0 = block_start[b], c1 = block_start[b + 1];
+        if (c1 - c0 > kFusedThreadsB) return false;
+        int nl = c1 - c0, nfs = 0;
+        for (int i = c0; i < c1; ++i) mark[perm[i]] = b;
+        for (int i = c0; i < c1; ++i) {
+            const int c = perm[i];
             for (int k = fdeg[c]; k < fdeg[c + 1]; ++k)
                 if (fmark[fc[k]] != b) {
                     fmark[fc[k]] = b;
-                    ++nfb;
+                    ++nfs;
+                }
+            for (int k = deg[c]; k < deg[c + 1]; ++k)
+                if (mark[nb[k]] != b) {
+                    mark[nb[k]] = b;
+                    ++nl;
                 }
         }
-        for (int r = 0; r < 2; ++r) {
-            std::vector<int> next;
-            for (int c : ring)
-                for (int k = deg[c]; k < deg[c + 1]; ++k)
-                    if (mark[nb[k]] != b) {
-                        mark[nb[k]] = b;
-                        next.push_back(nb[k]);
-                    }
-            nl += (int)next.size();
-            ring.swap(next);
-        }
-        if (nl > max_nl || nfb > max_nfb) return false;
+        if (nl > max_local_cells(D) || nfs > max_face_slots(D)) return false;
     }
     return true;
 }
@@ -123,17 +127,21 @@ bool fused_plan_build(FusedPlan &p, const Geo &g, const HostPrep &prep, const st
         err = "fused path needs a 2-D or 3-D mesh";
         return false;
     }
-    const int nC = g.nC, nIF = g.nIF;
+    const int nC = g.nC, nIF = g.nIF, nF = g.nF;
     const int nblk = (int)block_start.size() - 1;
-    // cell adjacency comes from the gradient stencil (internal faces)
     const std::vector<int> &cg_off = prep.cg_off, &cg_nb = prep.cg_nb;
     const std::vector<int> &cf_off = prep.cf_off, &cf_face = prep.cf_face;
     const std::vector<int> &owner = prep.owner_h, &neighbour = prep.neighbour_h;
 
+    std::vector<int> block_of(nC);
+    for (int b = 0; b < nblk; ++b)
+        for (int c = block_start[b]; c < block_start[b + 1]; ++c) block_of[c] = b;
+    // a face shared by two blocks has a slot in each; twin links them
+    std::vector<int> first_slot(nF, -1);
+
     p.nblk = nblk;
     p.blk.resize(nblk);
-    std::vector<int> mark(nC, -1);
-    std::vector<int> local(nC, -1);
+    std::vector<int> mark(nC, -1), local(nC, -1);
     for (int b = 0; b < nblk; ++b) {
         BlockInfo bi{};
         const int c0 = block_start[b], c1 = block_start[b + 1];
@@ -145,7 +153,7 @@ bool fused_plan_build(FusedPlan &p, const Geo &g, const HostPrep &prep, const st
             L.push_back(c);
             mark[c] = b;
         }
-        std::vector<int> h1, h2;
+        std::vector<int> h1;
         for (int c = c0; c < c1; ++c)
             for (int k = cg_off[c]; k < cg_off[c + 1]; ++k) {
                 const int n = cg_nb[k];
@@ -155,29 +163,18 @@ bool fused_plan_build(FusedPlan &p, const Geo &g, const HostPrep &prep, const st
                 }
             }
         std::sort(h1.begin(), h1.end());
-        for (int c : h1)
-            for (int k = cg_off[c]; k < cg_off[c + 1]; ++k) {
-                const int n = cg_nb[k];
-                if (mark[n] != b) {
-                    mark[n] = b;
-                    h2.push_back(n);
-                }
-            }
-        std::sort(h2.begin(), h2.end());
         L.insert(L.end(), h1.begin(), h1.end());
-        L.insert(L.end(), h2.begin(), h2.end());
         bi.nl = (int)L.size();
-        bi.ng = bi.nc + (int)h1.size();
         for (int l = 0; l < bi.nl; ++l) local[L[l]] = l;
         p.lcell.insert(p.lcell.end(), L.begin(), L.end());
-        // gradient stencils of block + ring 1, ascending face label (the order of cg)
+        // gradient stencils of the block cells, ascending face label (the order of cg)
         bi.gsb_off = (int)p.gs_beg.size();
-        for (int l = 0; l < bi.ng; ++l) {
+        for (int l = 0; l < bi.nc; ++l) {
             const int c = L[l];
             p.gs_beg.push_back((int)p.gs_nb.size());
             for (int k = cg_off[c]; k < cg_off[c + 1]; ++k) {
                 const int n = cg_nb[k];
-                if (mark[n] != b || local[n] < 0 || L[local[n]] != n) {
+                if (local[n] < 0) {
                     err = "internal error: stencil cell outside the halo";
                     return false;
                 }
@@ -186,33 +183,35 @@ bool fused_plan_build(FusedPlan &p, const Geo &g, const HostPrep &prep, const st
             }
         }
         p.gs_beg.push_back((int)p.gs_nb.size());
-        // owned faces (owner in block), ascending face label
-        bi.fa_off = (int)p.fa_face.size();
-        for (int c = c0; c < c1; ++c)
-            for (int k = cf_off[c]; k < cf_off[c + 1]; ++k) {
-                const int f = cf_face[k];
-                if (owner[f] != c) continue;
-                p.fa_face.push_back(f);
-                p.fa_own.push_back(local[c]);
-                p.fa_nei.push_back(f < nIF ? local[neighbour[f]] : -1);
-            }
-        bi.nfa = (int)p.fa_face.size() - bi.fa_off;
-        // faces of the block cells: unique list + per-cell gather order
-        bi.fb_off = (int)p.fb_face.size();
-        std::unordered_map<int, int> fb_index;
+        // faces of the block cells: unique slots + per-cell gather order (ascending face label)
+        bi.fs_off = (int)p.fs_face.size();
+        std::unordered_map<int, int> slot_of;
         bi.fbc_off = (int)p.fbc_beg.size();
         for (int c = c0; c < c1; ++c) {
             p.fbc_beg.push_back((int)p.fbc_idx.size());
             for (int k = cf_off[c]; k < cf_off[c + 1]; ++k) {
                 const int f = cf_face[k];
-                auto it = fb_index.find(f);
+                auto it = slot_of.find(f);
                 int idx;
-                if (it == fb_index.end()) {
-                    idx = (int)p.fb_face.size() - bi.fb_off;
-                    fb_index.emplace(f, idx);
-                    p.fb_face.push_back(f);
-                    p.fb_own.push_back(local[owner[f]]);
-                    p.fb_nei.push_back(f < nIF ? local[neighbour[f]] : -1);
+                if (it == slot_of.end()) {
+                    idx = (int)p.fs_face.size() - bi.fs_off;
+                    slot_of.emplace(f, idx);
+                    const int o = owner[f], n = f < nIF ? neighbour[f] : -1;
+                    int flag = 0;
+                    if (block_of[o] == b) flag |= kSlotOwnIn;
+                    if (n >= 0 && block_of[n] == b) flag |= kSlotNeiIn;
+                    const int gslot = (int)p.fs_face.size();
+                    p.fs_face.push_back(f);
+                    p.fs_twin.push_back(-1);
+                    if (first_slot[f] < 0) {
+                        first_slot[f] = gslot;
+                    } else {
+                        p.fs_twin[gslot] = first_slot[f];
+                        p.fs_twin[first_slot[f]] = gslot;
+                    }
+                    p.fs_lo.push_back(local[o]);
+                    p.fs_ln.push_back(n >= 0 ? local[n] : -1);
+                    p.fs_flag.push_back((signed char)flag);
                 } else {
                     idx = it->second;
                 }
@@ -222,13 +221,11 @@ bool fused_plan_build(FusedPlan &p, const Geo &g, const HostPrep &prep, const st
             }
         }
         p.fbc_beg.push_back((int)p.fbc_idx.size());
-        bi.nfb = (int)p.fb_face.size() - bi.fb_off;
+        bi.nfs = (int)p.fs_face.size() - bi.fs_off;
         for (int l = 0; l < bi.nl; ++l) local[L[l]] = -1;
         p.blk[b] = bi;
         p.max_nl = std::max(p.max_nl, bi.nl);
-        p.max_ng = std::max(p.max_ng, bi.ng);
-        p.max_nfa = std::max(p.max_nfa, bi.nfa);
-        p.max_nfb = std::max(p.max_nfb, bi.nfb);
+        p.max_nfs = std::max(p.max_nfs, bi.nfs);
         p.max_nc = std::max(p.max_nc, bi.nc);
     }
     int max_st = 0, max_cf = 0;
@@ -238,18 +235,12 @@ bool fused_plan_build(FusedPlan &p, const Geo &g, const HostPrep &prep, const st
         err = "cells with more than " + std::to_string(kFusedMaxNb) + " faces";
         return false;
     }
-    if (p.max_nl > kFusedThreadsA * fused_cells_per_thread_a(g.D) ||
-        p.max_nl > kFusedThreadsB * fused_cells_per_thread_b(g.D) ||
-        p.max_nfb > kFusedThreadsB * fused_faces_per_thread_b(g.D)) {
-        err = "block halos exceed the work-group shape (local cells " + std::to_string(p.max_nl) + ", faces " +
-              std::to_string(p.max_nfb) + ")";
+    if (p.max_nl > max_local_cells(g.D) || p.max_nfs > max_face_slots(g.D) || p.max_nc > kFusedThreadsB) {
+        err = "block tables exceed the work-group shape (cells " + std::to_string(p.max_nc) + ", local cells " +
+              std::to_string(p.max_nl) + ", faces " + std::to_string(p.max_nfs) + ")";
         return false;
     }
-    if (p.max_nc > kFusedThreadsB || p.max_nfa > kFusedThreadsA * kFusedSlotsA) {
-        err = "block tables exceed the work-group shape (cells " + std::to_string(p.max_nc) + ", owned faces " +
-              std::to_string(p.max_nfa) + ")";
-        return false;
-    }
+    p.n_slots = (int)p.fs_face.size();
     p.nm = g.D == 3 ? 17 : 11;
     p.nma = p.nm + 1;
     p.wg_target = nvg_hint > 0 ? nvg_hint * nblk : 4096;

@@ -1,0 +1,950 @@
+// Fused DUGKS step kernels, recompute schedule (see dugks_fused_rc.h for the data flow).
+//
+// Work-group = (cell block, velocity group).  Per chunk of up to VC velocities (one x-stripe
+// of a velocity row):
+//   stage 1  f-bar-plus of every block / ring-1 / ring-2 cell -> LDS
+//            (fvDVM::updateGHbarPvol, fvDVM.C:648-683; equilibrium in FP64)
+//   stage 2  least-squares gradient of every block / ring-1 cell -> LDS
+//            (leastSquaresGrad through discreteVelocity::updateGrad, discreteVelocity.C:366-369)
+//   stage 3A reconstruction of owned faces at x_f - xi dt/2 (fvDVM.C:824-963), FP64 raw
+//            moments accumulated per face in registers (fvDVM.C:1181-1297)
+//   stage 3B face values relaxed to the face equilibrium (fvDVM.C:1362-1460), incoming
+//            wall / inlet values (fvDVM.C:1054-1098, discreteVelocity.C:244-257), flux in LDS
+//   stage 4B g~ <- 4/3 f-bar-plus - 1/3 g~ - dt/V sum(+-flux) in ascending face order
+//            (fvDVM.C:1462-1519), FP64 raw moments of the new g~, h~ per cell
+//
+// Each thread owns fixed cells / faces for the whole work-group.  The velocity set is a
+// tensor product, so the Maxwellian factor A exp(-(xi_x - U_x)^2 / 2RT) of an entity is
+// formed once per x-stripe and kept in registers; a chunk then needs one exp for the y/z
+// part per entity.  Work-group ids are mapped so that the velocity groups of a block run on
+// the same accelerator die and share its L2.  Reconstruction arithmetic runs in the storage
+// type S (FP32 for the mixed path); equilibria and moments in FP64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "dugks_fused_rc.h"
+
+namespace dugks {
+namespace rc {
+
+namespace {
+
+constexpr int kNTA = kFusedThreadsA;
+constexpr int kSlotsA = kFusedSlotsA;
+constexpr int kNTB = kFusedThreadsB;
+constexpr int kNb = kFusedMaxNb;
+constexpr int kBlk = 256;
+
+template <class S>
+struct V3 {
+    S x, y, z;
+};
+
+struct FArgs {
+    const BlockInfo *blk;
+    const int *lcell, *gs_beg, *gs_nb, *fa_face, *fa_own, *fa_nei, *fb_face, *fb_own, *fb_nei, *fbc_beg, *fbc_idx;
+    const signed char *fbc_sgn;
+    const void *gs_ls;
+    double *fpart, *cpart;
+    const double *bin;
+    EqCoef *cc, *fc;
+    const BndCoef *bc;
+    int nblk, b8, nvg, n1, nrow, nchunk, cpg;
+    int max_nl, max_ng, max_nfb, max_nc, nm, nma;
+};
+
+FArgs fargs(const FusedPlan &p)
+{
+    FArgs a;
+    a.blk = p.d_blk; a.lcell = p.d_lcell; a.gs_beg = p.d_gs_beg; a.gs_nb = p.d_gs_nb;
+    a.fa_face = p.d_fa_face; a.fa_own = p.d_fa_own; a.fa_nei = p.d_fa_nei;
+    a.fb_face = p.d_fb_face; a.fb_own = p.d_fb_own; a.fb_nei = p.d_fb_nei;
+    a.fbc_beg = p.d_fbc_beg; a.fbc_idx = p.d_fbc_idx; a.fbc_sgn = p.d_fbc_sgn; a.gs_ls = p.d_gs_ls;
+    a.fpart = p.fpart; a.cpart = p.cpart; a.bin = p.bin; a.cc = p.cc; a.fc = p.fc; a.bc = p.bc;
+    a.nblk = p.nblk; a.b8 = p.b8; a.nvg = p.nvg; a.n1 = p.n1; a.nrow = p.nrow; a.nchunk = p.nchunk; a.cpg = p.cpg;
+    a.max_nl = p.max_nl; a.max_ng = p.max_ng; a.max_nfb = p.max_nfb; a.max_nc = p.max_nc;
+    a.nm = p.nm; a.nma = p.nma;
+    return a;
+}
+
+// Work-group id -> (block, velocity group).  Ids are dealt round-robin over the dies, so die
+// k receives ids k, k+8, ...; those are mapped to the velocity groups of blocks
+// [k*b8, (k+1)*b8), block-major, and a die's resident work-groups share a few blocks' data.
+__device__ inline bool wg_map(const FArgs &a, int &blk, int &vg)
+{
+    const int id = blockIdx.x;
+    const int die = id % kFusedXcds, s = id / kFusedXcds;
+    blk = die * a.b8 + s / a.nvg;
+    vg = s % a.nvg;
+    return blk < a.nblk;
+}
+
+__device__ inline double eq_prefactor(const Geo &g, double rho, double T)
+{
+    return rho / pow(sqrt(2.0 * kPi * g.R * T), (double)g.D);
+}
+
+// Coefficients of the Shakhov equilibrium (fvDVM.C:673-679) for rho, U, T, q.
+__device__ inline EqCoef make_coef(const Geo &g, double rho, const d3 &U, double T, const d3 &q, double RTh, double tr)
+{
+    EqCoef k;
+    const double RT = g.R * T;
+    k.A = eq_prefactor(g, rho, T);
+    k.ux = U.x; k.uy = U.y; k.uz = U.z;
+    k.invRT = 1.0 / RT;
+    k.RTh = RTh;
+    const double q5 = 1.0 / (5.0 * rho * RT * RT);
+    k.qx = q.x * q5; k.qy = q.y * q5; k.qz = q.z * q5;
+    k.tr = tr;
+    k.pad0 = k.pad1 = 0.0;
+    return k;
+}
+
+// Shakhov equilibrium (fvDVM.C:673-679) given gM = A exp(-|xi - U|^2 / 2RT):
+// c = xi - U, s = |c|^2/(R T), cq = c.q/(5 rho (R T)^2)
+__device__ inline void shakhov_poly(const Geo &g, const EqCoef &k, const d3 &xi, double gM, double &gEq, double &hEq)
+{
+    const double cx = xi.x - k.ux, cy = xi.y - k.uy, cz = xi.z - k.uz;
+    const double s = (cx * cx + cy * cy + cz * cz) * k.invRT;
+    const double cq = cx * k.qx + cy * k.qy + cz * k.qz;
+    const double omp = 1.0 - g.Pr;
+    const double kd = g.K + 3.0 - g.D;
+    gEq = (1.0 + omp * cq * (s - g.D - 2.0)) * gM;
+    hEq = (kd + omp * cq * ((s - g.D) * kd - 2 * g.K)) * gM * k.RTh;
+}
+
+// exp(-((xi_y - U_y)^2 + (xi_z - U_z)^2) / 2RT) of the chunk's row (xr = xi of the row)
+__device__ inline double row_factor(const EqCoef &k, const d3 &xr)
+{
+    const double dy = xr.y - k.uy, dz = xr.z - k.uz;
+    return exp(-((dy * dy + dz * dz) * k.invRT) / 2.0);
+}
+
+// A exp(-(xi_x - U_x)^2 / 2RT) for the abscissae ix0 .. ix0+VC-1 of a stripe; xi[ix] is the
+// velocity (x[ix], x[0], x[0]) of the tensor set (ix fastest)
+template <int VC>
+__device__ inline void stripe_factors(const Geo &g, const EqCoef &k, int ix0, int n1, double *ex)
+{
+#pragma unroll
+    for (int vv = 0; vv < VC; ++vv) {
+        const int ix = min(ix0 + vv, n1 - 1);
+        const double dx = g.xi[ix].x - k.ux;
+        ex[vv] = k.A * exp(-((dx * dx) * k.invRT) / 2.0);
+    }
+}
+
+// raw moments of one (value, velocity) pair: m[k] += w * g * phi_k(xi), h likewise
+template <int D>
+__device__ inline void add_moments(double *m, double wg, double wh, const d3 &xi)
+{
+    const double xx = xi.x * xi.x, yy = xi.y * xi.y;
+    if (D == 2) {
+        const double s2 = xx + yy;
+        m[0] += wg;
+        m[1] += wg * xi.x;
+        m[2] += wg * xi.y;
+        m[3] += wg * xx;
+        m[4] += wg * (xi.x * xi.y);
+        m[5] += wg * yy;
+        m[6] += wg * (s2 * xi.x);
+        m[7] += wg * (s2 * xi.y);
+        m[8] += wh;
+        m[9] += wh * xi.x;
+        m[10] += wh * xi.y;
+    } else {
+        const double zz = xi.z * xi.z;
+        const double s2 = xx + yy + zz;
+        m[0] += wg;
+        m[1] += wg * xi.x;
+        m[2] += wg * xi.y;
+        m[3] += wg * xi.z;
+        m[4] += wg * xx;
+        m[5] += wg * (xi.x * xi.y);
+        m[6] += wg * (xi.x * xi.z);
+        m[7] += wg * yy;
+        m[8] += wg * (xi.y * xi.z);
+        m[9] += wg * zz;
+        m[10] += wg * (s2 * xi.x);
+        m[11] += wg * (s2 * xi.y);
+        m[12] += wg * (s2 * xi.z);
+        m[13] += wh;
+        m[14] += wh * xi.x;
+        m[15] += wh * xi.y;
+        m[16] += wh * xi.z;
+    }
+}
+
+// rho, U, T, tau, q from raw moments (the conversions of fvDVM.C:1249-1253, 1683-1696;
+// q = 1/2 sum w c (|c|^2 g + h) with c = xi - U expanded in raw moments), then the
+// 2 tau / (2 tau + h Pr) correction with h = corr_dt.
+template <int D>
+__device__ inline void macros_from_moments(const Geo &g, const double *m, double corr_dt, double &rho, d3 &U,
+                                           double &T, double &tau, d3 &q)
+{
+    double G0, G1[3], G2[6], G3[3], H0, H1[3];
+    if (D == 2) {
+        G0 = m[0]; G1[0] = m[1]; G1[1] = m[2]; G1[2] = 0.0;
+        G2[0] = m[3]; G2[1] = m[4]; G2[2] = 0.0; G2[3] = m[5]; G2[4] = 0.0; G2[5] = 0.0;
+        G3[0] = m[6]; G3[1] = m[7]; G3[2] = 0.0;
+        H0 = m[8]; H1[0] = m[9]; H1[1] = m[10]; H1[2] = 0.0;
+    } else {
+        G0 = m[0]; G1[0] = m[1]; G1[1] = m[2]; G1[2] = m[3];
+        G2[0] = m[4]; G2[1] = m[5]; G2[2] = m[6]; G2[3] = m[7]; G2[4] = m[8]; G2[5] = m[9];
+        G3[0] = m[10]; G3[1] = m[11]; G3[2] = m[12];
+        H0 = m[13]; H1[0] = m[14]; H1[1] = m[15]; H1[2] = m[16];
+    }
+    rho = G0;
+    U = d3{G1[0] / rho, G1[1] / rho, G1[2] / rho};
+    const double trG2 = G2[0] + G2[3] + G2[5];
+    const double rhoE = 0.5 * (trG2 + H0);
+    T = (rhoE - 0.5 * rho * magsqr(U)) / ((g.K + 3) / 2.0 * g.R * rho);
+    tau = g.muRef * exp(g.omega * log(T / g.Tref)) / rho / T / g.R;
+    const double u[3] = {U.x, U.y, U.z};
+    const double uu = magsqr(U);
+    const double G1u = G1[0] * u[0] + G1[1] * u[1] + G1[2] * u[2];
+    const double G2u[3] = {G2[0] * u[0] + G2[1] * u[1] + G2[2] * u[2], G2[1] * u[0] + G2[3] * u[1] + G2[4] * u[2],
+                           G2[2] * u[0] + G2[4] * u[1] + G2[5] * u[2]};
+    double qq[3];
+    for (int k = 0; k < 3; ++k) {
+        const double qg = G3[k] - 2.0 * G2u[k] + uu * G1[k] - trG2 * u[k] + 2.0 * u[k] * G1u - uu * u[k] * G0;
+        qq[k] = 0.5 * (qg + H1[k] - H0 * u[k]);
+    }
+    const double corr = 2.0 * tau / (2.0 * tau + corr_dt * g.Pr);
+    q = d3{corr * qq[0], corr * qq[1], corr * qq[2]};
+}
+
+// ----------------------------------------------------------------- shared stages
+
+// Velocity chunk ch of the (x-stripe, row) order: stripe = ch / nrow, row = ch % nrow.
+struct Chunk {
+    int ix0, nvv, vrow;  // first abscissa of the stripe, valid velocities, v of (ix = 0, row)
+    bool new_stripe;     // stripe differs from the previous chunk's: x-factors must be formed
+};
+
+__device__ inline Chunk chunk_of(const FArgs &a, int ch, int vc, int &stripe)
+{
+    const int st = ch / a.nrow;
+    const int row = ch - st * a.nrow;
+    Chunk c;
+    c.ix0 = st * vc;
+    c.nvv = min(vc, a.n1 - c.ix0);
+    c.vrow = row * a.n1;
+    c.new_stripe = st != stripe;
+    stripe = st;
+    return c;
+}
+
+// stage 1: f-bar-plus of local cells for one chunk; cex[s] holds the x-factors of the
+// thread's slot-s cell for the current stripe
+template <class S, int VC, int NT, int CPT>
+__device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &bi, const S *__restrict__ gT,
+                                 const S *__restrict__ hT, double dt, const Chunk &ck, double (&cex)[CPT][VC],
+                                 S *fbg, S *fbh, S *ftg, S *fth)
+{
+    const d3 xr = g.xi[ck.vrow];
+#pragma unroll
+    for (int s = 0; s < CPT; ++s) {
+        const int l = threadIdx.x + s * NT;
+        if (l >= bi.nl) break;
+        const int c = a.lcell[bi.l_off + l];
+        const EqCoef k = a.cc[c];
+        S gt[VC], ht[VC];
+#pragma unroll
+        for (int vv = 0; vv < VC; ++vv) {
+            const size_t kk = (size_t)(ck.vrow + ck.ix0 + min(vv, ck.nvv - 1)) * g.nC + c;
+            gt[vv] = gT[kk];
+            ht[vv] = hT[kk];
+        }
+        if (ck.new_stripe) stripe_factors<VC>(g, k, ck.ix0, a.n1, cex[s]);
+        const double eyz = row_factor(k, xr);
+        const double af = 1.5 * dt / (2.0 * k.tr + dt);
+#pragma unroll
+        for (int vv = 0; vv < VC; ++vv) {
+            if (vv >= ck.nvv) break;
+            double gEq, hEq;
+            shakhov_poly(g, k, g.xi[ck.vrow + ck.ix0 + vv], cex[s][vv] * eyz, gEq, hEq);
+            fbg[vv * a.max_nl + l] = (S)((1.0 - af) * (double)gt[vv] + af * gEq);
+            fbh[vv * a.max_nl + l] = (S)((1.0 - af) * (double)ht[vv] + af * hEq);
+            if (ftg && l < bi.nc) {
+                ftg[vv * a.max_nc + l] = gt[vv];
+                fth[vv * a.max_nc + l] = ht[vv];
+            }
+        }
+    }
+}
+
+// stage 2: gradients of block + ring-1 cells
+template <class S, int D, int VC, int NT>
+__device__ inline void stage_grad(const FArgs &a, const BlockInfo &bi, int nvv, const S *fbg, const S *fbh, S *grg,
+                                  S *grh)
+{
+    const V3<S> *ls = (const V3<S> *)a.gs_ls;
+    for (int l = threadIdx.x; l < bi.ng; l += NT) {
+        const int k0 = a.gs_beg[bi.gsb_off + l];
+        const int n = a.gs_beg[bi.gsb_off + l + 1] - k0;
+        int nb[kNb];
+        V3<S> L[kNb];
+#pragma unroll
+        for (int j = 0; j < kNb; ++j) {
+            if (j < n) {
+                nb[j] = a.gs_nb[k0 + j];
+                L[j] = ls[k0 + j];
+            }
+        }
+#pragma unroll 1
+        for (int vv = 0; vv < nvv; ++vv) {
+            const S *pg = fbg + vv * a.max_nl, *ph = fbh + vv * a.max_nl;
+            const S cg = pg[l], ch = ph[l];
+            S gx = 0, gy = 0, gz = 0, hx = 0, hy = 0, hz = 0;
+#pragma unroll
+            for (int j = 0; j < kNb; ++j) {
+                if (j < n) {
+                    const S dg = pg[nb[j]] - cg, dh = ph[nb[j]] - ch;
+                    gx += L[j].x * dg; gy += L[j].y * dg;
+                    hx += L[j].x * dh; hy += L[j].y * dh;
+                    if (D == 3) {
+                        gz += L[j].z * dg;
+                        hz += L[j].z * dh;
+                    }
+                }
+            }
+            S *og = grg + (vv * a.max_ng + l) * D, *oh = grh + (vv * a.max_ng + l) * D;
+            og[0] = gx; og[1] = gy; oh[0] = hx; oh[1] = hy;
+            if (D == 3) {
+                og[2] = gz;
+                oh[2] = hz;
+            }
+        }
+    }
+}
+
+template <class S, int D>
+__device__ inline S recon_l(const S *fb, const S *gr, int l, double ax, double ay, double az)
+{
+    const S *gl = gr + l * D;
+    if (D == 3) return fb[l] + (gl[0] * (S)ax + gl[1] * (S)ay + gl[2] * (S)az);
+    return fb[l] + (gl[0] * (S)ax + gl[1] * (S)ay);
+}
+
+// ------------------------------------------------------------------------- pass A
+template <class S, int D, int VC>
+__global__ void __launch_bounds__(kNTA)
+    k_pass_a(Geo g, FArgs a, const S *__restrict__ gT, const S *__restrict__ hT, const double *dtp)
+{
+    constexpr int NMA = (D == 3 ? 17 : 11) + 1;
+    constexpr int CPT = fused_cells_per_thread_a(D);
+    int blkid, vg;
+    if (!wg_map(a, blkid, vg)) return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const BlockInfo bi = a.blk[blkid];
+    const int ch0 = vg * a.cpg, ch1 = min(a.nchunk, ch0 + a.cpg);
+    S *fbg = (S *)smem;
+    S *fbh = fbg + VC * a.max_nl;
+    S *grg = fbh + VC * a.max_nl;
+    S *grh = grg + VC * a.max_ng * D;
+    const double dt = *dtp;
+    const double hx = 0.5 * dt;
+
+    double acc[kSlotsA][NMA];
+    int fid[kSlotsA], fown[kSlotsA], fnei[kSlotsA], ftyp[kSlotsA];
+    d3 fS[kSlotsA], armO[kSlotsA], armN[kSlotsA];
+#pragma unroll
+    for (int s = 0; s < kSlotsA; ++s) {
+#pragma unroll
+        for (int k = 0; k < NMA; ++k) acc[s][k] = 0.0;
+        const int j = threadIdx.x + s * kNTA;
+        fid[s] = -1;
+        fown[s] = fnei[s] = ftyp[s] = 0;
+        fS[s] = armO[s] = armN[s] = d3{0, 0, 0};
+        if (j < bi.nfa) {
+            const int f = a.fa_face[bi.fa_off + j];
+            fid[s] = f;
+            fown[s] = a.fa_own[bi.fa_off + j];
+            fnei[s] = a.fa_nei[bi.fa_off + j];
+            ftyp[s] = f < g.nIF ? 0 : g.bf_type[f - g.nIF];
+            fS[s] = g.Sf[f];
+            const d3 Cf = g.Cf[f];
+            const d3 Co = g.C[a.lcell[bi.l_off + fown[s]]];
+            armO[s] = d3{Cf.x - Co.x, Cf.y - Co.y, Cf.z - Co.z};
+            if (fnei[s] >= 0) {
+                const d3 Cn = g.C[a.lcell[bi.l_off + fnei[s]]];
+                armN[s] = d3{Cf.x - Cn.x, Cf.y - Cn.y, Cf.z - Cn.z};
+            } else {
+                armN[s] = armO[s];
+            }
+        }
+    }
+
+    double cex[CPT][VC];
+    int stripe = -1;
+    for (int ch = ch0; ch < ch1; ++ch) {
+        const Chunk ck = chunk_of(a, ch, VC, stripe);
+        stage_bar<S, VC, kNTA, CPT>(g, a, bi, gT, hT, dt, ck, cex, fbg, fbh, (S *)nullptr, (S *)nullptr);
+        __syncthreads();
+        stage_grad<S, D, VC, kNTA>(a, bi, ck.nvv, fbg, fbh, grg, grh);
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < kSlotsA; ++s) {
+            if (fid[s] < 0) continue;
+#pragma unroll 1
+            for (int vv = 0; vv < ck.nvv; ++vv) {
+                const int v = ck.vrow + ck.ix0 + vv;
+                const d3 xi = g.xi[v];
+                const double w = g.w[v];
+                const double xs = sigma_dot(xi, fS[s]);
+                const S *pg = fbg + vv * a.max_nl, *ph = fbh + vv * a.max_nl;
+                const S *qg = grg + vv * a.max_ng * D, *qh = grh + vv * a.max_ng * D;
+                const double ox = armO[s].x - xi.x * hx, oy = armO[s].y - xi.y * hx, oz = armO[s].z - xi.z * hx;
+                double gf, hf;
+                if (ftyp[s] == 0) {
+                    const double nx = armN[s].x - xi.x * hx, ny = armN[s].y - xi.y * hx, nz = armN[s].z - xi.z * hx;
+                    if (xs >= kVSmall) {
+                        gf = (double)recon_l<S, D>(pg, qg, fown[s], ox, oy, oz);
+                        hf = (double)recon_l<S, D>(ph, qh, fown[s], ox, oy, oz);
+                    } else if (xs < -kVSmall) {
+                        gf = (double)recon_l<S, D>(pg, qg, fnei[s], nx, ny, nz);
+                        hf = (double)recon_l<S, D>(ph, qh, fnei[s], nx, ny, nz);
+                    } else {
+                        gf = 0.5 * ((double)recon_l<S, D>(pg, qg, fnei[s], nx, ny, nz) +
+                                    (double)recon_l<S, D>(pg, qg, fown[s], ox, oy, oz));
+                        hf = 0.5 * ((double)recon_l<S, D>(ph, qh, fnei[s], nx, ny, nz) +
+                                    (double)recon_l<S, D>(ph, qh, fown[s], ox, oy, oz));
+                    }
+                } else {
+                    if (!(xs > 0)) continue;  // incoming half added from constants
+                    gf = (double)recon_l<S, D>(pg, qg, fown[s], ox, oy, oz);
+                    hf = (double)recon_l<S, D>(ph, qh, fown[s], ox, oy, oz);
+                    if (ftyp[s] == kMaxwellWall) acc[s][NMA - 1] += w * xs * gf;
+                }
+                add_moments<D>(acc[s], w * gf, w * hf, xi);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int s = 0; s < kSlotsA; ++s) {
+        if (fid[s] < 0) continue;
+        double *o = a.fpart + ((size_t)vg * g.nF + fid[s]) * NMA;
+#pragma unroll
+        for (int k = 0; k < NMA; ++k) o[k] = acc[s][k];
+    }
+}
+
+// ------------------------------------------------------------------- face macros
+template <int D>
+__global__ void __launch_bounds__(kBlk)
+    k_face_macros(Geo g, FArgs a, Macros fm, double *rho_b, const double *dtp)
+{
+    constexpr int NM = D == 3 ? 17 : 11;
+    constexpr int NMA = NM + 1;
+    const int f = blockIdx.x * kBlk + threadIdx.x;
+    if (f >= g.nF) return;
+    double m[NMA];
+#pragma unroll
+    for (int k = 0; k < NMA; ++k) m[k] = 0.0;
+    for (int vg = 0; vg < a.nvg; ++vg) {
+        const double *p = a.fpart + ((size_t)vg * g.nF + f) * NMA;
+#pragma unroll
+        for (int k = 0; k < NMA; ++k) m[k] += p[k];
+    }
+    if (f >= g.nIF) {
+        const int b = f - g.nIF;
+        double scale = 1.0;
+        if (g.bf_type[b] == kMaxwellWall) {
+            // calculatedMaxwellFvPatchField::evaluate: rho_w = outGoing / |inComingByRho|
+            scale = m[NMA - 1] / fabs(g.in_by_rho[b]);
+            rho_b[b] = scale;
+        }
+        const double *in = a.bin + (size_t)b * NM;
+#pragma unroll
+        for (int k = 0; k < NM; ++k) m[k] += scale * in[k];
+    }
+    double rho, T, tau;
+    d3 U, q;
+    macros_from_moments<D>(g, m, 0.5 * (*dtp), rho, U, T, tau, q);
+    fm.rho[f] = rho;
+    fm.U[f] = U;
+    fm.T[f] = T;
+    fm.tau[f] = tau;
+    fm.q[f] = q;
+}
+
+// face equilibrium coefficients, with the relaxation weight h/(2 tau + h) (fvDVM.C:1380)
+// and, on patches with q1_compat, R times the internal-face temperature at the
+// patch-local index (fvDVM.C:1367, 1413)
+__global__ void __launch_bounds__(kBlk) k_face_coef(Geo g, Macros fm, EqCoef *fc, const double *dtp)
+{
+    const int f = blockIdx.x * kBlk + threadIdx.x;
+    if (f >= g.nF) return;
+    const double h = 0.5 * (*dtp);
+    const double T = fm.T[f];
+    double Th = T;
+    if (f >= g.nIF && g.q1) Th = fm.T[g.bf_local[f - g.nIF]];
+    fc[f] = make_coef(g, fm.rho[f], fm.U[f], T, fm.q[f], g.R * Th, h / (2.0 * fm.tau[f] + h));
+}
+
+// ------------------------------------------------------------------------- pass B
+template <class S, int D, int VC>
+__global__ void __launch_bounds__(kNTB)
+    k_pass_b(Geo g, FArgs a, const S *__restrict__ gT, const S *__restrict__ hT, S *__restrict__ gN,
+             S *__restrict__ hN, const double *rho_b, const double *dtp)
+{
+    constexpr int NM = D == 3 ? 17 : 11;
+    constexpr int CPT = fused_cells_per_thread_b(D);
+    constexpr int FPT = fused_faces_per_thread_b(D);
+    int blkid, vg;
+    if (!wg_map(a, blkid, vg)) return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const BlockInfo bi = a.blk[blkid];
+    const int ch0 = vg * a.cpg, ch1 = min(a.nchunk, ch0 + a.cpg);
+    double *flg = (double *)smem;                   // [VC][max_nfb]
+    double *flh = flg + VC * a.max_nfb;
+    S *fbg = (S *)(flh + VC * a.max_nfb);
+    S *fbh = fbg + VC * a.max_nl;
+    S *grg = fbh + VC * a.max_nl;
+    S *grh = grg + VC * a.max_ng * D;
+    S *ftg = grh + VC * a.max_ng * D;
+    S *fth = ftg + VC * a.max_nc;
+    const double dt = *dtp;
+    const double hx = 0.5 * dt;
+
+    double mom[NM];
+#pragma unroll
+    for (int k = 0; k < NM; ++k) mom[k] = 0.0;
+    const int my = threadIdx.x;
+    const bool has_cell = my < bi.nc;
+    const int cell = bi.c0 + my;
+    double dtV = 0.0;
+    int nf = 0;
+    int fidx[kNb];
+    signed char fsg[kNb];
+    if (has_cell) {
+        dtV = dt / g.V[cell];
+        const int k0 = a.fbc_beg[bi.fbc_off + my];
+        nf = a.fbc_beg[bi.fbc_off + my + 1] - k0;
+#pragma unroll
+        for (int j = 0; j < kNb; ++j) {
+            if (j < nf) {
+                fidx[j] = a.fbc_idx[k0 + j];
+                fsg[j] = a.fbc_sgn[k0 + j];
+            }
+        }
+    }
+
+    double cex[CPT][VC], fex[FPT][VC];
+    int stripe = -1;
+    for (int ch = ch0; ch < ch1; ++ch) {
+        const Chunk ck = chunk_of(a, ch, VC, stripe);
+        stage_bar<S, VC, kNTB, CPT>(g, a, bi, gT, hT, dt, ck, cex, fbg, fbh, ftg, fth);
+        __syncthreads();
+        stage_grad<S, D, VC, kNTB>(a, bi, ck.nvv, fbg, fbh, grg, grh);
+        __syncthreads();
+        // stage 3: face values -> flux sigma * f_f in LDS
+        const d3 xr = g.xi[ck.vrow];
+#pragma unroll
+        for (int s = 0; s < FPT; ++s) {
+            const int j = threadIdx.x + s * kNTB;
+            if (j >= bi.nfb) break;
+            const int f = a.fb_face[bi.fb_off + j];
+            const int lo = a.fb_own[bi.fb_off + j], ln = a.fb_nei[bi.fb_off + j];
+            const d3 Sf = g.Sf[f];
+            const d3 Cf = g.Cf[f];
+            const d3 Co = g.C[a.lcell[bi.l_off + lo]];
+            const double aox = Cf.x - Co.x, aoy = Cf.y - Co.y, aoz = Cf.z - Co.z;
+            double anx = aox, any = aoy, anz = aoz;
+            if (ln >= 0) {
+                const d3 Cn = g.C[a.lcell[bi.l_off + ln]];
+                anx = Cf.x - Cn.x;
+                any = Cf.y - Cn.y;
+                anz = Cf.z - Cn.z;
+            }
+            const EqCoef k = a.fc[f];
+            if (ck.new_stripe) stripe_factors<VC>(g, k, ck.ix0, a.n1, fex[s]);
+            const double eyz = row_factor(k, xr);
+            const int b = f - g.nIF;
+#pragma unroll
+            for (int vv = 0; vv < VC; ++vv) {
+                if (vv >= ck.nvv) break;
+                const d3 xi = g.xi[ck.vrow + ck.ix0 + vv];
+                const double xs = sigma_dot(xi, Sf);
+                const S *pg = fbg + vv * a.max_nl, *ph = fbh + vv * a.max_nl;
+                const S *qg = grg + vv * a.max_ng * D, *qh = grh + vv * a.max_ng * D;
+                const double ox = aox - xi.x * hx, oy = aoy - xi.y * hx, oz = aoz - xi.z * hx;
+                double gv, hv;
+                bool relax = true;
+                if (b < 0) {
+                    const double nx = anx - xi.x * hx, ny = any - xi.y * hx, nz = anz - xi.z * hx;
+                    if (xs >= kVSmall) {
+                        gv = (double)recon_l<S, D>(pg, qg, lo, ox, oy, oz);
+                        hv = (double)recon_l<S, D>(ph, qh, lo, ox, oy, oz);
+                    } else if (xs < -kVSmall) {
+                        gv = (double)recon_l<S, D>(pg, qg, ln, nx, ny, nz);
+                        hv = (double)recon_l<S, D>(ph, qh, ln, nx, ny, nz);
+                    } else {
+                        gv = 0.5 * ((double)recon_l<S, D>(pg, qg, ln, nx, ny, nz) +
+                                    (double)recon_l<S, D>(pg, qg, lo, ox, oy, oz));
+                        hv = 0.5 * ((double)recon_l<S, D>(ph, qh, ln, nx, ny, nz) +
+                                    (double)recon_l<S, D>(ph, qh, lo, ox, oy, oz));
+                    }
+                } else if (xs > 0) {
+                    gv = (double)recon_l<S, D>(pg, qg, lo, ox, oy, oz);
+                    hv = (double)recon_l<S, D>(ph, qh, lo, ox, oy, oz);
+                } else {
+                    relax = false;
+                    const BndCoef kb = a.bc[b];
+                    const double scale = g.bf_type[b] == kMaxwellWall ? rho_b[b] : 1.0;
+                    const double cx = xi.x - kb.ux, cy = xi.y - kb.uy, cz = xi.z - kb.uz;
+                    gv = scale * (kb.A * exp(-((cx * cx + cy * cy + cz * cz) * kb.invRT) / 2.0));
+                    hv = gv * kb.hfac;
+                }
+                if (relax) {
+                    double gEq, hEq;
+                    shakhov_poly(g, k, xi, fex[s][vv] * eyz, gEq, hEq);
+                    gv = (1.0 - k.tr) * gv + k.tr * gEq;
+                    hv = (1.0 - k.tr) * hv + k.tr * hEq;
+                }
+                flg[vv * a.max_nfb + j] = xs * gv;
+                flh[vv * a.max_nfb + j] = xs * hv;
+            }
+        }
+        __syncthreads();
+        // stage 4: cell update in ascending face order, new values to the other buffer
+        if (has_cell) {
+#pragma unroll 1
+            for (int vv = 0; vv < ck.nvv; ++vv) {
+                const int v = ck.vrow + ck.ix0 + vv;
+                double ag = -1.0 / 3 * (double)ftg[vv * a.max_nc + my] + 4.0 / 3 * (double)fbg[vv * a.max_nl + my];
+                double ah = -1.0 / 3 * (double)fth[vv * a.max_nc + my] + 4.0 / 3 * (double)fbh[vv * a.max_nl + my];
+#pragma unroll
+                for (int j = 0; j < kNb; ++j) {
+                    if (j < nf) {
+                        const double tg = flg[vv * a.max_nfb + fidx[j]] * dtV;
+                        const double th = flh[vv * a.max_nfb + fidx[j]] * dtV;
+                        if (fsg[j] > 0) {
+                            ag += tg;
+                            ah += th;
+                        } else {
+                            ag -= tg;
+                            ah -= th;
+                        }
+                    }
+                }
+                const S sg = (S)ag, sh = (S)ah;
+                const size_t kk = (size_t)v * g.nC + cell;
+                gN[kk] = sg;
+                hN[kk] = sh;
+                const double w = g.w[v];
+                add_moments<D>(mom, w * (double)sg, w * (double)sh, g.xi[v]);
+            }
+        }
+        __syncthreads();
+    }
+    if (has_cell) {
+        double *o = a.cpart + ((size_t)vg * g.nC + cell) * NM;
+#pragma unroll
+        for (int k = 0; k < NM; ++k) o[k] = mom[k];
+    }
+}
+
+// ------------------------------------------------------------------- cell macros
+template <int D>
+__global__ void __launch_bounds__(kBlk) k_cell_macros(Geo g, FArgs a, Macros cm, const double *dtp)
+{
+    constexpr int NM = D == 3 ? 17 : 11;
+    const int c = blockIdx.x * kBlk + threadIdx.x;
+    if (c >= g.nC) return;
+    double m[NM];
+#pragma unroll
+    for (int k = 0; k < NM; ++k) m[k] = 0.0;
+    for (int vg = 0; vg < a.nvg; ++vg) {
+        const double *p = a.cpart + ((size_t)vg * g.nC + c) * NM;
+#pragma unroll
+        for (int k = 0; k < NM; ++k) m[k] += p[k];
+    }
+    double rho, T, tau;
+    d3 U, q;
+    macros_from_moments<D>(g, m, *dtp, rho, U, T, tau, q);
+    cm.rho[c] = rho;
+    cm.U[c] = U;
+    cm.T[c] = T;
+    cm.tau[c] = tau;
+    cm.q[c] = q;
+    a.cc[c] = make_coef(g, rho, U, T, q, g.R * T, tau);
+}
+
+__global__ void __launch_bounds__(kBlk) k_cell_coef(Geo g, Macros cm, EqCoef *cc)
+{
+    const int c = blockIdx.x * kBlk + threadIdx.x;
+    if (c >= g.nC) return;
+    cc[c] = make_coef(g, cm.rho[c], cm.U[c], cm.T[c], cm.q[c], g.R * cm.T[c], cm.tau[c]);
+}
+
+// incoming-half moments of the wall (per unit wall density) and inlet faces, fixed for
+// the run (the wall and inlet values are constant, fvDVM.C:578-624, discreteVelocity.C:244-257),
+// and the coefficients pass B uses to form the incoming values
+template <int D>
+__global__ void __launch_bounds__(kBlk) k_incoming(Geo g, double *bin, BndCoef *bc)
+{
+    constexpr int NM = D == 3 ? 17 : 11;
+    const int b = blockIdx.x * kBlk + threadIdx.x;
+    if (b >= g.nB) return;
+    const d3 Sf = g.Sf[g.nIF + b];
+    const d3 Ub = g.bf_U[b];
+    const double Tb = g.bf_T[b];
+    const bool wall = g.bf_type[b] == kMaxwellWall;
+    BndCoef k{};
+    k.A = (wall ? 1.0 : g.bf_rho[b]) / pow(sqrt(2.0 * kPi * g.R * Tb), (double)g.D);
+    k.ux = Ub.x; k.uy = Ub.y; k.uz = Ub.z;
+    k.invRT = 1.0 / (g.R * Tb);
+    k.hfac = (g.K + 3 - g.D) * g.R * Tb;
+    bc[b] = k;
+    double m[NM];
+#pragma unroll
+    for (int j = 0; j < NM; ++j) m[j] = 0.0;
+    for (int v = 0; v < g.Nv; ++v) {
+        const d3 xi = g.xi[v];
+        if (sigma_dot(xi, Sf) > 0) continue;
+        const double cx = xi.x - k.ux, cy = xi.y - k.uy, cz = xi.z - k.uz;
+        const double gv = k.A * exp(-((cx * cx + cy * cy + cz * cz) * k.invRT) / 2.0);
+        add_moments<D>(m, g.w[v] * gv, g.w[v] * (gv * k.hfac), xi);
+    }
+    for (int j = 0; j < NM; ++j) bin[(size_t)b * NM + j] = m[j];
+}
+
+template <class S>
+size_t lds_pass_a(const FusedPlan &p, int D, int VC)
+{
+    return (size_t)VC * (2 * p.max_nl + 2 * p.max_ng * D) * sizeof(S);
+}
+template <class S>
+size_t lds_pass_b(const FusedPlan &p, int D, int VC)
+{
+    return (size_t)VC * 2 * p.max_nfb * sizeof(double) + lds_pass_a<S>(p, D, VC) +
+           (size_t)VC * 2 * p.max_nc * sizeof(S);
+}
+
+constexpr size_t kLdsBudget = 80 * 1024;
+
+template <class S>
+int choose_vc(const FusedPlan &p, int D)
+{
+    for (int vc : {4, 2, 1})
+        if (lds_pass_b<S>(p, D, vc) <= kLdsBudget) return vc;
+    return 0;
+}
+
+inline int grid_ab(const FusedPlan &p) { return kFusedXcds * p.b8 * p.nvg; }
+
+template <class S, int D, int VC>
+void launch_a(const Geo &g, FusedPlan &p, const S *gT, const S *hT, const double *dt, hipStream_t st)
+{
+    const size_t lds = lds_pass_a<S>(p, D, VC);
+    hipLaunchKernelGGL((k_pass_a<S, D, VC>), dim3(grid_ab(p)), dim3(kNTA), lds, st, g, fargs(p), gT, hT, dt);
+}
+
+template <class S, int D, int VC>
+void launch_b(const Geo &g, FusedPlan &p, const S *gT, const S *hT, S *gN, S *hN, const double *rho_b,
+              const double *dt, hipStream_t st)
+{
+    const size_t lds = lds_pass_b<S>(p, D, VC);
+    hipLaunchKernelGGL((k_pass_b<S, D, VC>), dim3(grid_ab(p)), dim3(kNTB), lds, st, g, fargs(p), gT, hT, gN, hN,
+                       rho_b, dt);
+}
+
+template <class S>
+bool set_lds_attrs(int D, int vc, const FusedPlan &p)
+{
+    auto set = [](const void *fn, size_t bytes) {
+        return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess;
+    };
+    bool ok = true;
+#define DUGKS_SET(DD, VV)                                                                              \
+    if (D == DD && vc == VV) {                                                                         \
+        ok &= set((const void *)k_pass_a<S, DD, VV>, lds_pass_a<S>(p, DD, VV));                        \
+        ok &= set((const void *)k_pass_b<S, DD, VV>, lds_pass_b<S>(p, DD, VV));                        \
+    }
+    DUGKS_SET(2, 1) DUGKS_SET(2, 2) DUGKS_SET(2, 4) DUGKS_SET(3, 1) DUGKS_SET(3, 2) DUGKS_SET(3, 4)
+#undef DUGKS_SET
+    return ok;
+}
+
+inline dim3 grid1(int n) { return dim3((n + kBlk - 1) / kBlk); }
+
+}  // namespace
+
+// ----------------------------------------------------------------- plan upload / init
+
+template <class T>
+static bool up(FusedPlan &p, T **dst, const std::vector<T> &src, std::string &err)
+{
+    void *q = nullptr;
+    const size_t bytes = std::max<size_t>(1, src.size()) * sizeof(T);
+    if (hipMalloc(&q, bytes) != hipSuccess) {
+        err = "hipMalloc failed for fused tables";
+        return false;
+    }
+    p.allocs.push_back(q);
+    if (!src.empty() && hipMemcpy(q, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) {
+        err = "hipMemcpy failed for fused tables";
+        return false;
+    }
+    *dst = (T *)q;
+    return true;
+}
+
+static bool dbuf(FusedPlan &p, void **dst, size_t bytes, std::string &err)
+{
+    void *q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(1, bytes)) != hipSuccess) {
+        err = "hipMalloc of " + std::to_string(bytes) + " bytes failed (fused buffers)";
+        return false;
+    }
+    p.allocs.push_back(q);
+    *dst = q;
+    return true;
+}
+
+void fused_velocity_groups(FusedPlan &p, int D, int n1, int vc)
+{
+    p.vc = vc;
+    p.n1 = n1;
+    p.nrow = 1;
+    for (int k = 1; k < D; ++k) p.nrow *= n1;
+    p.nstripe = (n1 + vc - 1) / vc;
+    p.nchunk = p.nstripe * p.nrow;
+    const int want = std::max(1, std::min(p.nchunk, (p.wg_target + p.nblk - 1) / p.nblk));
+    p.cpg = (p.nchunk + want - 1) / want;
+    p.nvg = (p.nchunk + p.cpg - 1) / p.cpg;
+    p.b8 = (p.nblk + kFusedXcds - 1) / kFusedXcds;
+}
+
+bool fused_plan_upload(FusedPlan &p, const Geo &g, size_t es, std::string &err)
+{
+    const int vc = es == 4 ? choose_vc<float>(p, g.D) : choose_vc<double>(p, g.D);
+    if (vc == 0) {
+        err = "block tables exceed the LDS budget";
+        return false;
+    }
+    if (g.n1d < 1 || g.Nv != (g.D == 3 ? g.n1d * g.n1d * g.n1d : g.n1d * g.n1d)) {
+        err = "fused path needs a tensor-product velocity set";
+        return false;
+    }
+    fused_velocity_groups(p, g.D, g.n1d, vc);
+    if (!(es == 4 ? set_lds_attrs<float>(g.D, vc, p) : set_lds_attrs<double>(g.D, vc, p))) {
+        err = "cannot raise the dynamic LDS limit";
+        return false;
+    }
+    if (!up(p, &p.d_blk, p.blk, err) || !up(p, &p.d_lcell, p.lcell, err) || !up(p, &p.d_gs_beg, p.gs_beg, err) ||
+        !up(p, &p.d_gs_nb, p.gs_nb, err) || !up(p, &p.d_fa_face, p.fa_face, err) ||
+        !up(p, &p.d_fa_own, p.fa_own, err) || !up(p, &p.d_fa_nei, p.fa_nei, err) ||
+        !up(p, &p.d_fb_face, p.fb_face, err) || !up(p, &p.d_fb_own, p.fb_own, err) ||
+        !up(p, &p.d_fb_nei, p.fb_nei, err) || !up(p, &p.d_fbc_beg, p.fbc_beg, err) ||
+        !up(p, &p.d_fbc_idx, p.fbc_idx, err) || !up(p, &p.d_fbc_sgn, p.fbc_sgn, err))
+        return false;
+    if (es == 4) {
+        std::vector<V3<float>> ls(p.gs_ls.size());
+        for (size_t i = 0; i < ls.size(); ++i) ls[i] = V3<float>{(float)p.gs_ls[i].x, (float)p.gs_ls[i].y, (float)p.gs_ls[i].z};
+        V3<float> *d = nullptr;
+        if (!up(p, &d, ls, err)) return false;
+        p.d_gs_ls = d;
+    } else {
+        d3 *d = nullptr;
+        if (!up(p, &d, p.gs_ls, err)) return false;
+        p.d_gs_ls = d;
+    }
+    void *fp = nullptr, *cp = nullptr, *bi = nullptr, *cc = nullptr, *fc = nullptr, *bc = nullptr;
+    if (!dbuf(p, &fp, (size_t)p.nvg * g.nF * p.nma * 8, err) || !dbuf(p, &cp, (size_t)p.nvg * g.nC * p.nm * 8, err) ||
+        !dbuf(p, &bi, (size_t)g.nB * p.nm * 8, err) || !dbuf(p, &cc, (size_t)g.nC * sizeof(EqCoef), err) ||
+        !dbuf(p, &fc, (size_t)g.nF * sizeof(EqCoef), err) || !dbuf(p, &bc, (size_t)g.nB * sizeof(BndCoef), err) ||
+        !dbuf(p, &p.gT2, (size_t)g.Nv * g.nC * es, err) || !dbuf(p, &p.hT2, (size_t)g.Nv * g.nC * es, err))
+        return false;
+    p.fpart = (double *)fp;
+    p.cpart = (double *)cp;
+    p.bin = (double *)bi;
+    p.cc = (EqCoef *)cc;
+    p.fc = (EqCoef *)fc;
+    p.bc = (BndCoef *)bc;
+    p.lds_a = es == 4 ? lds_pass_a<float>(p, g.D, vc) : lds_pass_a<double>(p, g.D, vc);
+    p.lds_b = es == 4 ? lds_pass_b<float>(p, g.D, vc) : lds_pass_b<double>(p, g.D, vc);
+    return true;
+}
+
+bool fused_init_boundary(FusedPlan &p, const Geo &g, const Macros &cm, hipStream_t st, std::string &err)
+{
+    if (g.nB > 0) {
+        if (g.D == 3)
+            hipLaunchKernelGGL(k_incoming<3>, grid1(g.nB), dim3(kBlk), 0, st, g, p.bin, p.bc);
+        else
+            hipLaunchKernelGGL(k_incoming<2>, grid1(g.nB), dim3(kBlk), 0, st, g, p.bin, p.bc);
+    }
+    hipLaunchKernelGGL(k_cell_coef, grid1(g.nC), dim3(kBlk), 0, st, g, cm, p.cc);
+    if (hipGetLastError() != hipSuccess) {
+        err = "fused init kernels failed to launch";
+        return false;
+    }
+    return true;
+}
+
+void fused_plan_free(FusedPlan &p)
+{
+    for (void *q : p.allocs)
+        if (q) (void)hipFree(q);
+    p.allocs.clear();
+    p.ready = false;
+}
+
+// ------------------------------------------------------------------- launchers
+
+template <class S>
+void Fused<S>::face_pass(const Geo &g, FusedPlan &p, const S *gT, const S *hT, const Macros &, const double *dt,
+                         hipStream_t st)
+{
+#define DUGKS_A(DD, VV) \
+    if (g.D == DD && p.vc == VV) return launch_a<S, DD, VV>(g, p, gT, hT, dt, st);
+    DUGKS_A(2, 1) DUGKS_A(2, 2) DUGKS_A(2, 4) DUGKS_A(3, 1) DUGKS_A(3, 2) DUGKS_A(3, 4)
+#undef DUGKS_A
+}
+
+template <class S>
+void Fused<S>::face_macros(const Geo &g, FusedPlan &p, Macros fm, double *rho_b, const double *dt, double *cfl_part,
+                           hipStream_t st)
+{
+    if (g.D == 3)
+        hipLaunchKernelGGL(k_face_macros<3>, grid1(g.nF), dim3(kBlk), 0, st, g, fargs(p), fm, rho_b, dt);
+    else
+        hipLaunchKernelGGL(k_face_macros<2>, grid1(g.nF), dim3(kBlk), 0, st, g, fargs(p), fm, rho_b, dt);
+    hipLaunchKernelGGL(k_face_coef, grid1(g.nF), dim3(kBlk), 0, st, g, fm, p.fc, dt);
+    launch_cfl_partials(g, fm, cfl_part, st);
+}
+
+template <class S>
+void Fused<S>::update_pass(const Geo &g, FusedPlan &p, const S *gT, const S *hT, S *gN, S *hN, const Macros &,
+                           const Macros &, const double *rho_b, const double *dt, hipStream_t st)
+{
+#define DUGKS_B(DD, VV) \
+    if (g.D == DD && p.vc == VV) return launch_b<S, DD, VV>(g, p, gT, hT, gN, hN, rho_b, dt, st);
+    DUGKS_B(2, 1) DUGKS_B(2, 2) DUGKS_B(2, 4) DUGKS_B(3, 1) DUGKS_B(3, 2) DUGKS_B(3, 4)
+#undef DUGKS_B
+}
+
+template <class S>
+void Fused<S>::cell_macros(const Geo &g, FusedPlan &p, Macros cm, const double *dt, hipStream_t st)
+{
+    if (g.D == 3)
+        hipLaunchKernelGGL(k_cell_macros<3>, grid1(g.nC), dim3(kBlk), 0, st, g, fargs(p), cm, dt);
+    else
+        hipLaunchKernelGGL(k_cell_macros<2>, grid1(g.nC), dim3(kBlk), 0, st, g, fargs(p), cm, dt);
+}
+
+template <class S>
+void Fused<S>::cell_coef(const Geo &g, FusedPlan &p, const Macros &cm, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_cell_coef, grid1(g.nC), dim3(kBlk), 0, st, g, cm, p.cc);
+}
+
+template struct Fused<float>;
+template struct Fused<double>;
+
+}  // namespace rc
+}  // namespace dugks

@@ -98,6 +98,7 @@ def load_library(path: str = LIB_PATH):
     L.dugks_destroy.argtypes = [vp]
     L.dugks_destroy.restype = None
     L.dugks_adjoint.argtypes = [vp, ct.c_int, _D, ct.c_int, ct.POINTER(Objective), _D, _D, _D, _D, _D, _D]
+    L.dugks_path_info.argtypes = [vp, _I, _I, _I, _I]
     _lib = L
     return L
 
@@ -320,6 +321,12 @@ class FvDVM:
 
     def checkpoint_load(self, path):
         self._check(self.L.dugks_checkpoint_load(self._h, str(path).encode()))
+
+    def path_info(self):
+        """dict(fused, block_cells, chunk_velocities, velocity_groups) of this handle."""
+        v = [ct.c_int() for _ in range(4)]
+        self._check(self.L.dugks_path_info(self._h, *[ct.byref(x) for x in v]))
+        return dict(zip(("fused", "block_cells", "chunk_velocities", "velocity_groups"), [x.value for x in v]))
 
     def velocity_set(self):
         xi = np.empty((self.Nv, 3))
