@@ -231,8 +231,8 @@ def adjoint_check(a):
         prob = adjoint.SphereDrag(steps=a.adjoint_steps or 300, checkpoint_every=30, precision=hip.MIXED)
         th = prob.nominal()
         th[1:] = 1.0 + 0.05 * np.sin(np.arange(prob.n_patch))
-        fd_idx = [0, 1, 17, 40]
-        names = ["Kn", "T_patch[0]", "T_patch[16]", "T_patch[39]"]
+        fd_idx = [0, 17]
+        names = ["Kn", "T_patch[16]"]
     t0 = time.perf_counter()
     prob.time_steps(th)
     t1 = time.perf_counter()

@@ -708,30 +708,37 @@ int run_adjoint(const Geo &g, DevState &d, hipStream_t st, const AdjointIO &io, 
     ADJ_CHK(B.get(&gB, ncell, err) && B.get(&hB, ncell, err) && B.get(&gS, nface, err) && B.get(&hS, nface, err) &&
             B.get(&gR, nface, err) && B.get(&hR, nface, err));
     launch_init_mixed_faces<S>(g, gS, hS, st);
-    // states: checkpoints every K steps and one segment of K consecutive steps
-    const int nseg = (N + K - 1) / K;
-    std::vector<Slot<S>> ck(nseg), seg(K);
+    // states: checkpoints every K steps and one segment of K consecutive steps (none when
+    // only the objective is asked for)
+    const bool want_grad = io.dmu || io.dTw || io.dUw || io.dg0 || io.dh0;
+    const int nseg = want_grad ? (N + K - 1) / K : 0;
+    std::vector<Slot<S>> ck(nseg), seg(want_grad ? K : 0);
     for (auto *v : {&ck, &seg})
         for (auto &s : *v) ADJ_CHK(B.get(&s.g, ncell, err) && B.get(&s.h, ncell, err) && alloc_macros(B, s.m, nC, err));
     // adjoint buffers
-    double *lg, *lh, *pg, *ph, *rg, *rh, *bg, *bh, *gg, *gh, *thb, *mu_f, *mu_c, *Tw_bar, *jf;
-    d3 *Uw_bar;
-    ADJ_CHK(B.get(&lg, ncell, err) && B.get(&lh, ncell, err) && B.get(&pg, nface, err) && B.get(&ph, nface, err) &&
-            B.get(&rg, nface, err) && B.get(&rh, nface, err) && B.get(&bg, ncell, err) && B.get(&bh, ncell, err) &&
-            B.get(&gg, 3 * ncell, err) && B.get(&gh, 3 * ncell, err) && B.get(&thb, nB, err) &&
-            B.get(&mu_f, nF, err) && B.get(&mu_c, nC, err) && B.get(&Tw_bar, nB, err) && B.get(&Uw_bar, nB, err) &&
-            B.get(&jf, nF, err));
-    auto zero = [&](double *p, size_t n) {
-        if (n) hipLaunchKernelGGL(k_zero, dim3((unsigned)((n + kB - 1) / kB)), dim3(kB), 0, st, p, n);
-    };
-    zero(lg, ncell);
-    zero(lh, ncell);
-    zero(mu_f, nF);
-    zero(mu_c, nC);
-    zero(Tw_bar, nB);
-    zero((double *)Uw_bar, 3 * (size_t)nB);
-    zero(rg, nface);
-    zero(rh, nface);
+    double *lg = nullptr, *lh = nullptr, *pg = nullptr, *ph = nullptr, *rg = nullptr, *rh = nullptr, *bg = nullptr,
+           *bh = nullptr, *gg = nullptr, *gh = nullptr, *thb = nullptr, *mu_f = nullptr, *mu_c = nullptr,
+           *Tw_bar = nullptr, *jf = nullptr;
+    d3 *Uw_bar = nullptr;
+    ADJ_CHK(B.get(&jf, nF, err));
+    if (want_grad) {
+        ADJ_CHK(B.get(&lg, ncell, err) && B.get(&lh, ncell, err) && B.get(&pg, nface, err) &&
+                B.get(&ph, nface, err) && B.get(&rg, nface, err) && B.get(&rh, nface, err) && B.get(&bg, ncell, err) &&
+                B.get(&bh, ncell, err) && B.get(&gg, 3 * ncell, err) && B.get(&gh, 3 * ncell, err) &&
+                B.get(&thb, nB, err) && B.get(&mu_f, nF, err) && B.get(&mu_c, nC, err) && B.get(&Tw_bar, nB, err) &&
+                B.get(&Uw_bar, nB, err));
+        auto zero = [&](double *p, size_t n) {
+            if (n) hipLaunchKernelGGL(k_zero, dim3((unsigned)((n + kB - 1) / kB)), dim3(kB), 0, st, p, n);
+        };
+        zero(lg, ncell);
+        zero(lh, ncell);
+        zero(mu_f, nF);
+        zero(mu_c, nC);
+        zero(Tw_bar, nB);
+        zero((double *)Uw_bar, 3 * (size_t)nB);
+        zero(rg, nface);
+        zero(rh, nface);
+    }
 
     Macros fm = d.face;
     // one forward step n on the state (gt, ht, cm): cm holds the step's start macros on entry
@@ -758,7 +765,8 @@ int run_adjoint(const Geo &g, DevState &d, hipStream_t st, const AdjointIO &io, 
     // forward sweep with checkpoints, on the handle's own state
     S *gT = (S *)d.gT, *hT = (S *)d.hT;
     for (int n = 0; n < N; ++n) {
-        if (n % K == 0) ADJ_CHK(copy_state(ck[n / K].g, ck[n / K].h, ck[n / K].m.view(), gT, hT, d.cell));
+        if (want_grad && n % K == 0)
+            ADJ_CHK(copy_state(ck[n / K].g, ck[n / K].h, ck[n / K].m.view(), gT, hT, d.cell));
         step(n, gT, hT, d.cell);
     }
     hipLaunchKernelGGL(k_objective<S>, grid_x(nF), dim3(kB), 0, st, g, fm, gS, d_qw, d_mw, jf);
@@ -769,6 +777,7 @@ int run_adjoint(const Geo &g, DevState &d, hipStream_t st, const AdjointIO &io, 
     double J = 0.0;
     for (double x : jh) J += x;
     if (io.J) *io.J = J;
+    if (!want_grad) return 0;   // objective only: the handle already holds the state after N steps
     // the reverse sweep uses the handle's buffers as scratch: keep the state after N steps
     // (distributions, cell and face macros, wall density, CFL partials) and restore it at the end
     Slot<S> fin;
@@ -782,9 +791,7 @@ int run_adjoint(const Geo &g, DevState &d, hipStream_t st, const AdjointIO &io, 
     ADJ_CHK(hipMemcpyAsync(rho_b_save, d.rho_b, sizeof(double) * nB, hipMemcpyDeviceToDevice, st) == hipSuccess);
     ADJ_CHK(hipMemcpyAsync(cfl_save, d.cfl_part, sizeof(double) * ncfl, hipMemcpyDeviceToDevice, st) == hipSuccess);
 
-    // reverse sweep (skipped when no derivative is asked for: objective only)
-    const bool want_grad = io.dmu || io.dTw || io.dUw || io.dg0 || io.dh0;
-    if (!want_grad) return 0;
+    // reverse sweep
     FaceAdjArgs fa{thb, d_pstart, d_psize, (int)p_start.size(), nullptr, mu_f, Tw_bar, Uw_bar};
     for (int s = nseg - 1; s >= 0; --s) {
         const int n0 = s * K, n1 = std::min(N, n0 + K);

@@ -36,6 +36,20 @@ constexpr int kNTB = kFusedThreadsB;
 constexpr int kNb = kFusedMaxNb;
 constexpr int kBlk = 256;
 
+// Tuning knobs of the recompute schedule: LDS budget per work-group (caps the chunk size)
+// and the minimum waves per SIMD the passes are compiled for (caps their registers).
+#ifndef DUGKS_RC_LDS_KB
+#define DUGKS_RC_LDS_KB 80
+#endif
+#ifndef DUGKS_RC_WAVES_A
+#define DUGKS_RC_WAVES_A 1
+#endif
+#ifndef DUGKS_RC_WAVES_B
+#define DUGKS_RC_WAVES_B 1
+#endif
+constexpr int kWavesA = DUGKS_RC_WAVES_A;
+constexpr int kWavesB = DUGKS_RC_WAVES_B;
+
 template <class S>
 struct V3 {
     S x, y, z;
@@ -329,7 +343,7 @@ __device__ inline S recon_l(const S *fb, const S *gr, int l, double ax, double a
 
 // ------------------------------------------------------------------------- pass A
 template <class S, int D, int VC>
-__global__ void __launch_bounds__(kNTA)
+__global__ void __launch_bounds__(kNTA, kWavesA)
     k_pass_a(Geo g, FArgs a, const S *__restrict__ gT, const S *__restrict__ hT, const double *dtp)
 {
     constexpr int NMA = (D == 3 ? 17 : 11) + 1;
@@ -486,7 +500,7 @@ __global__ void __launch_bounds__(kBlk) k_face_coef(Geo g, Macros fm, EqCoef *fc
 
 // ------------------------------------------------------------------------- pass B
 template <class S, int D, int VC>
-__global__ void __launch_bounds__(kNTB)
+__global__ void __launch_bounds__(kNTB, kWavesB)
     k_pass_b(Geo g, FArgs a, const S *__restrict__ gT, const S *__restrict__ hT, S *__restrict__ gN,
              S *__restrict__ hN, const double *rho_b, const double *dtp)
 {
@@ -724,7 +738,7 @@ size_t lds_pass_b(const FusedPlan &p, int D, int VC)
            (size_t)VC * 2 * p.max_nc * sizeof(S);
 }
 
-constexpr size_t kLdsBudget = 80 * 1024;
+constexpr size_t kLdsBudget = DUGKS_RC_LDS_KB * 1024;
 
 template <class S>
 int choose_vc(const FusedPlan &p, int D)
