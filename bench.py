@@ -165,27 +165,37 @@ def rel2(a, b):
 
 
 def compare_precisions(case, steps, every, fused=-1, label=None):
-    """Mixed (FP32 storage) and FP64 runs side by side on the GPU, compared every `every`
-    steps up to `steps` (max-norm relative error max|a-b|/max|b| and 2-norm relative error);
-    a run whose fields stop being finite ends the comparison (the step is reported)."""
+    """Mixed (FP32 storage) and FP64 runs of the same case on the GPU, compared every `every`
+    steps up to `steps` (max-norm relative error max|a-b|/max|b| and 2-norm relative error).
+    The two runs go one after the other, each keeping host copies of its cell macros at the
+    comparison steps, so only one solver's distributions are resident at a time; a run whose
+    fields stop being finite ends the comparison (the step is reported)."""
     import numpy as np
     from dugks_amd import hip
-    sm = hip.FvDVM(case, precision=hip.MIXED, fused=fused)
-    sf = hip.FvDVM(case, precision=hip.FP64, fused=fused)
-    rows, done, diverged = [], 0, None
-    tm = tf = 0.0
-    while done < steps:
-        n = min(every, steps - done)
-        t0 = time.perf_counter()
-        sm.run(n)
-        sm.synchronize()
-        t1 = time.perf_counter()
-        sf.run(n)
-        sf.synchronize()
-        t2 = time.perf_counter()
-        tm, tf = tm + t1 - t0, tf + t2 - t1
-        done += n
-        mo, mf = sm.cell_macros(), sf.cell_macros()
+
+    def run(precision):
+        s = hip.FvDVM(case, precision=precision, fused=fused)
+        snaps, done, t = [], 0, 0.0
+        while done < steps:
+            n = min(every, steps - done)
+            t0 = time.perf_counter()
+            s.run(n)
+            s.synchronize()
+            t += time.perf_counter() - t0
+            done += n
+            m = s.cell_macros()
+            snaps.append((done, m))
+            print(json.dumps({"progress": label, "precision": "mixed" if precision == hip.MIXED else "fp64",
+                              "step": done}), flush=True)
+            if not np.all(np.isfinite(m["rho"])):
+                break
+        s.close()
+        return snaps, t
+
+    mixed, tm = run(hip.MIXED)
+    fp64, tf = run(hip.FP64)
+    rows, diverged = [], None
+    for (done, mo), (_, mf) in zip(mixed, fp64):
         finite = {k: bool(np.all(np.isfinite(v))) for k, v in (("mixed", mo["rho"]), ("fp64", mf["rho"]))}
         if not all(finite.values()):
             diverged = {"step": done, "finite": finite}
@@ -196,11 +206,14 @@ def compare_precisions(case, steps, every, fused=-1, label=None):
                "T_range": [float(mf["T"].min()), float(mf["T"].max())]}
         rows.append(row)
         print(json.dumps({"progress": label, **row}), flush=True)
-    sm.close()
-    sf.close()
-    upd = case.n_cells * case.nv * done
-    return {"mixed_rel_err_vs_fp64": rows, "diverged": diverged, "mixed_value": upd / tm if tm else None,
-            "fp64_value": upd / tf if tf else None, "mixed_over_fp64": tf / tm if tm else None}
+    if diverged is None and len(mixed) != len(fp64):
+        short = mixed if len(mixed) < len(fp64) else fp64
+        diverged = {"step": short[-1][0], "finite": {"mixed": len(mixed) >= len(fp64), "fp64": len(fp64) >= len(mixed)}}
+    upd = case.n_cells * case.nv
+    nm, nf = mixed[-1][0], fp64[-1][0]
+    return {"mixed_rel_err_vs_fp64": rows, "diverged": diverged, "mixed_value": upd * nm / tm if tm else None,
+            "fp64_value": upd * nf / tf if tf else None,
+            "mixed_over_fp64": (tf / nf) / (tm / nm) if tm and tf else None}
 
 
 def kn_sweep(a):

@@ -525,7 +525,8 @@ __global__ void __launch_bounds__(kB)
     for (int v = 0; v < g.Nv; ++v) {
         const d3 xi = g.xi[v];
         const size_t k = (size_t)v * g.nC + c;
-        const double gt = (double)gT[k], ht = (double)hT[k];
+        // stored tilde values are deviations from the per-velocity reference
+        const double gt = (double)gT[k] + g.ref_g[v], ht = (double)hT[k] + g.ref_h[v];
         const double ag = bg[k], ah = bh[k];
         double gE, hE;
         shakhov_vjp(g, rho, U, T, q, T, xi, a * ag, a * ah, gE, hE, mc, th_bar);

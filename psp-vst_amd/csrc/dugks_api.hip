@@ -67,6 +67,7 @@ struct dugks_handle {
     FusedPlan plan;                // face-value schedule data
     rc::FusedPlan plan_rc;         // recompute schedule data
     std::vector<int> perm;         // internal cell -> caller's cell label
+    std::vector<double> ref_g, ref_h;  // host copies of the storage reference per velocity
     int hist_cap = 0;
     std::vector<void *> allocs;
     // profiling
@@ -405,6 +406,23 @@ int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patch
         upload(d.dc, mesh->delta_coeffs, nIF, st) || upload(d.weights, mesh->weights, nIF, st) ||
         upload(d.xi, xi.data(), Nv, st) || upload(d.w, w.data(), Nv, st) || upload(d.rho_b, bf_rho.data(), nB, st))
         return bail(1);
+    // reference Maxwellian of the stored deviations (FP32 storage only)
+    {
+        std::vector<double> rg(Nv, 0.0), rh(Nv, 0.0);
+        if (o.precision == DUGKS_MIXED) {
+            const double RT = gas->R * gas->Tref;
+            const double A = 1.0 / std::pow(std::sqrt(2.0 * kPi * RT), (double)D);
+            for (int v = 0; v < Nv; ++v) {
+                rg[v] = A * std::exp(-magsqr(xi[v]) / (2.0 * RT));
+                rh[v] = (gas->KInner + 3.0 - D) * RT * rg[v];
+            }
+        }
+        if (dalloc(h, &d.ref_g, Nv) || dalloc(h, &d.ref_h, Nv) || upload(d.ref_g, rg.data(), Nv, st) ||
+            upload(d.ref_h, rh.data(), Nv, st))
+            return bail(1);
+        h->ref_g = rg;
+        h->ref_h = rh;
+    }
 
     Geo &g = h->g;
     g.nC = nC; g.nIF = nIF; g.nB = nB; g.nF = nF; g.D = D; g.K = gas->KInner; g.Nv = Nv; g.n1d = n1;
@@ -414,6 +432,7 @@ int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patch
     g.cf_off = d.cf_off; g.cf_face = d.cf_face; g.cg_off = d.cg_off; g.cg_nb = d.cg_nb; g.cg_ls = d.cg_ls;
     g.bf_type = d.bf_type; g.bf_local = d.bf_local; g.bf_rho = d.bf_rho; g.bf_T = d.bf_T; g.bf_U = d.bf_U;
     g.in_by_rho = d.in_by_rho; g.delta_coeffs = d.dc; g.weights = d.weights; g.xi = d.xi; g.w = d.w;
+    g.ref_g = d.ref_g; g.ref_h = d.ref_h;
 
     // distributions
     const size_t ncell = (size_t)Nv * nC, nface = (size_t)Nv * nF;
@@ -619,10 +638,12 @@ int dugks_get_df(dugks_handle *h, double *g, double *hh)
         std::vector<char> tmp(n * h->es);
         HIPCHK(hipMemcpyAsync(tmp.data(), src[k], n * h->es, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
+        const std::vector<double> &ref = k ? h->ref_h : h->ref_g;
         for (int v = 0; v < Nv; ++v)
             for (int i = 0; i < nC; ++i) {
                 const size_t s = (size_t)v * nC + i, t = (size_t)v * nC + h->perm[i];
-                outs[k][t] = h->es == 8 ? ((const double *)tmp.data())[s] : (double)((const float *)tmp.data())[s];
+                outs[k][t] = h->es == 8 ? ((const double *)tmp.data())[s]
+                                        : (double)((const float *)tmp.data())[s] + ref[v];
             }
     }
     return 0;
@@ -638,13 +659,14 @@ int dugks_set_df(dugks_handle *h, const double *g, const double *hh)
     void *dst[2] = {h->d.gT, h->d.hT};
     for (int k = 0; k < 2; ++k) {
         std::vector<char> tmp(n * h->es);
+        const std::vector<double> &ref = k ? h->ref_h : h->ref_g;
         for (int v = 0; v < Nv; ++v)
             for (int i = 0; i < nC; ++i) {
                 const size_t s = (size_t)v * nC + h->perm[i], t = (size_t)v * nC + i;
                 if (h->es == 8)
                     ((double *)tmp.data())[t] = ins[k][s];
                 else
-                    ((float *)tmp.data())[t] = (float)ins[k][s];
+                    ((float *)tmp.data())[t] = (float)(ins[k][s] - ref[v]);
             }
         HIPCHK(hipMemcpyAsync(dst[k], tmp.data(), n * h->es, hipMemcpyHostToDevice, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
@@ -678,10 +700,12 @@ int dugks_set_cell_macros(dugks_handle *h, const double *rho, const double *U, c
 
 // ------------------------------------------------------------------- checkpoints
 namespace {
+// Version 2: the stored distributions are deviations from the per-velocity reference; the
+// header carries the sums of that reference so a file is only loaded where it means the same.
 struct CkptHeader {
     char magic[8];
     int version, nC, nF, nB, Nv, es;
-    double reserved[4];
+    double ref_sum_g, ref_sum_h, reserved[2];
 };
 
 int ckpt_io(dugks_handle *h, const char *path, bool save)
@@ -694,7 +718,11 @@ int ckpt_io(dugks_handle *h, const char *path, bool save)
     if (!fp) return fail(std::string("checkpoint: cannot open ") + path);
     CkptHeader hd{};
     std::memcpy(hd.magic, "DUGKSCK", 8);
-    hd.version = 1; hd.nC = g.nC; hd.nF = g.nF; hd.nB = g.nB; hd.Nv = g.Nv; hd.es = (int)h->es;
+    hd.version = 2; hd.nC = g.nC; hd.nF = g.nF; hd.nB = g.nB; hd.Nv = g.Nv; hd.es = (int)h->es;
+    for (int v = 0; v < g.Nv; ++v) {
+        hd.ref_sum_g += h->ref_g[v];
+        hd.ref_sum_h += h->ref_h[v];
+    }
     const size_t ndf = (size_t)g.Nv * g.nC * h->es;
     struct Blk { void *p; size_t bytes; };
     std::vector<Blk> blocks = {
@@ -709,10 +737,12 @@ int ckpt_io(dugks_handle *h, const char *path, bool save)
         if (std::fwrite(&hd, sizeof hd, 1, fp) != 1) rc = fail("checkpoint: write failed");
     } else {
         CkptHeader in{};
-        if (std::fread(&in, sizeof in, 1, fp) != 1 || std::memcmp(in.magic, "DUGKSCK", 8) != 0 || in.version != 1)
+        if (std::fread(&in, sizeof in, 1, fp) != 1 || std::memcmp(in.magic, "DUGKSCK", 8) != 0 || in.version != 2)
             rc = fail("checkpoint: not a checkpoint file");
         else if (in.nC != hd.nC || in.nF != hd.nF || in.nB != hd.nB || in.Nv != hd.Nv || in.es != hd.es)
             rc = fail("checkpoint: sizes or precision differ from this handle");
+        else if (in.ref_sum_g != hd.ref_sum_g || in.ref_sum_h != hd.ref_sum_h)
+            rc = fail("checkpoint: storage reference (Tref, R, K) differs from this handle");
     }
     const size_t chunk = 64ull << 20;
     std::vector<char> buf;

@@ -95,6 +95,8 @@ struct FusedPlan {
     double *fpart = nullptr;           // [nvg][nF][2][nma]: per (face, side) partial moments
     double *cpart = nullptr;           // [nvg][nC][nm]
     double *bin = nullptr;             // [nB][nm] incoming moments (walls: per unit density)
+    double *bref = nullptr;            // [nB][nma] outgoing-half moments of the storage reference
+    double *cref = nullptr;            // [nma] all-velocity moments of the storage reference
     EqCoef *cc = nullptr, *fc = nullptr;  // equilibrium coefficients of cells / faces
     BndCoef *bc = nullptr;                // incoming-half coefficients of boundary faces
     // face values [Nv][slots]: written by the block that reconstructs them into its own slot

@@ -53,7 +53,7 @@ struct FArgs {
     const signed char *fs_flag, *fbc_sgn;
     const void *gs_ls;
     double *fpart, *cpart;
-    const double *bin;
+    const double *bin, *bref, *cref;
     EqCoef *cc, *fc;
     const BndCoef *bc;
     int nblk, b8, nvg, n1, nrow, nchunk, cpg, n_slots;
@@ -68,7 +68,7 @@ FArgs fargs(const FusedPlan &p)
     a.n_slots = p.n_slots;
     a.fbc_beg = p.d_fbc_beg; a.fbc_idx = p.d_fbc_idx; a.fs_flag = p.d_fs_flag; a.fbc_sgn = p.d_fbc_sgn;
     a.gs_ls = p.d_gs_ls;
-    a.fpart = p.fpart; a.cpart = p.cpart; a.bin = p.bin; a.cc = p.cc; a.fc = p.fc; a.bc = p.bc;
+    a.fpart = p.fpart; a.cpart = p.cpart; a.bin = p.bin; a.bref = p.bref; a.cref = p.cref; a.cc = p.cc; a.fc = p.fc; a.bc = p.bc;
     a.nblk = p.nblk; a.b8 = p.b8; a.nvg = p.nvg; a.n1 = p.n1; a.nrow = p.nrow; a.nchunk = p.nchunk; a.cpg = p.cpg;
     a.max_nl = p.max_nl; a.max_nfs = p.max_nfs; a.max_nc = p.max_nc;
     a.nm = p.nm; a.nma = p.nma;
@@ -271,8 +271,15 @@ __device__ inline void cell_bar(const Geo &g, const FArgs &a, int c, const S *__
     const double af = 1.5 * dt / (2.0 * k.tr + dt);
 #pragma unroll
     for (int vv = 0; vv < VC; ++vv) {
+        const int v = ck.vrow + ck.ix0 + min(vv, ck.nvv - 1);
         double gEq, hEq;
-        shakhov_poly(g, k, g.xi[ck.vrow + ck.ix0 + min(vv, ck.nvv - 1)], cex[vv] * eyz, gEq, hEq);
+        shakhov_poly(g, k, g.xi[v], cex[vv] * eyz, gEq, hEq);
+        // with FP32 storage, stored values and gb, hb are deviations from the per-velocity
+        // reference; FP64 storage keeps values (its reference is zero)
+        if (sizeof(S) == 4) {
+            gEq -= g.ref_g[v];
+            hEq -= g.ref_h[v];
+        }
         gb[vv] = (1.0 - af) * (double)gt[vv] + af * gEq;
         hb[vv] = (1.0 - af) * (double)ht[vv] + af * hEq;
     }
@@ -462,13 +469,13 @@ __global__ void __launch_bounds__(kNTA)
                     const double ox = armO[s].x - xi.x * hx, oy = armO[s].y - xi.y * hx, oz = armO[s].z - xi.z * hx;
                     gf = (double)recon_l<S, D>(pg, qg, flo[s], ox, oy, oz);
                     hf = (double)recon_l<S, D>(ph, qh, flo[s], ox, oy, oz);
-                    if (ftyp[s] == kMaxwellWall) acc[s][NMA - 1] += w * xs * gf;
                 }
-                if (store) {
+                if (store) {  // face values are stored as deviations
                     const size_t kf = (size_t)v * a.n_slots + fpos[s];
                     gF[kf] = (S)gf;
                     hF[kf] = (S)hf;
                 }
+                if (ftyp[s] == kMaxwellWall) acc[s][NMA - 1] += w * xs * gf;
                 add_moments<D>(acc[s], w * gf, w * hf, xi);
             }
         }
@@ -502,6 +509,13 @@ __global__ void __launch_bounds__(kBlk)
         for (int k = 0; k < NMA; ++k) m[k] += p[k];
 #pragma unroll
         for (int k = 0; k < NMA; ++k) m[k] += p[NMA + k];
+    }
+    {
+        // the storage reference's share: every velocity on internal faces, the outgoing half
+        // on boundary faces (the passes accumulate deviations from it)
+        const double *r = f < g.nIF ? a.cref : a.bref + (size_t)(f - g.nIF) * NMA;
+#pragma unroll
+        for (int k = 0; k < NMA; ++k) m[k] += r[k];
     }
     if (f >= g.nIF) {
         const int b = f - g.nIF;
@@ -540,6 +554,14 @@ __global__ void __launch_bounds__(kBlk) k_face_coef(Geo g, Macros fm, EqCoef *fc
 }
 
 // ------------------------------------------------------------------------- pass B
+// a stored face value as a value: FP32 slots hold deviations from the reference
+template <class S>
+__device__ inline double face_value(S stored, const double *ref, int v)
+{
+    if (sizeof(S) == 4) return (double)stored + ref[v];
+    return (double)stored;
+}
+
 template <class S, int D, int VC>
 __global__ void __launch_bounds__(kNTB)
     k_pass_b(Geo g, FArgs a, S *gT, S *hT, const S *__restrict__ gF, const S *__restrict__ hF, const double *rho_b,
@@ -616,15 +638,15 @@ __global__ void __launch_bounds__(kNTB)
                         const size_t kf = (size_t)v * a.n_slots + (mine ? own_slot : twin);
                         double gEq, hEq;
                         shakhov_poly(g, k, xi, fex[s][vv] * eyz, gEq, hEq);
-                        fg = xs * ((1.0 - k.tr) * (double)gF[kf] + k.tr * gEq);
-                        fh = xs * ((1.0 - k.tr) * (double)hF[kf] + k.tr * hEq);
+                        fg = xs * ((1.0 - k.tr) * face_value<S>(gF[kf], g.ref_g, v) + k.tr * gEq);
+                        fh = xs * ((1.0 - k.tr) * face_value<S>(hF[kf], g.ref_h, v) + k.tr * hEq);
                     }
                 } else if (xs > 0) {
                     const size_t kf = (size_t)v * a.n_slots + own_slot;
                     double gEq, hEq;
                     shakhov_poly(g, k, xi, fex[s][vv] * eyz, gEq, hEq);
-                    fg = xs * ((1.0 - k.tr) * (double)gF[kf] + k.tr * gEq);
-                    fh = xs * ((1.0 - k.tr) * (double)hF[kf] + k.tr * hEq);
+                    fg = xs * ((1.0 - k.tr) * face_value<S>(gF[kf], g.ref_g, v) + k.tr * gEq);
+                    fh = xs * ((1.0 - k.tr) * face_value<S>(hF[kf], g.ref_h, v) + k.tr * hEq);
                 } else {
                     const BndCoef kb = a.bc[b];
                     const double scale = g.bf_type[b] == kMaxwellWall ? rho_b[b] : 1.0;
@@ -665,7 +687,7 @@ __global__ void __launch_bounds__(kNTB)
                 gT[kk] = sg;
                 hT[kk] = sh;
                 const double w = g.w[v];
-                add_moments<D>(mom, w * (double)sg, w * (double)sh, g.xi[v]);
+                add_moments<D>(mom, w * (double)sg, w * (double)sh, g.xi[v]);  // deviations
             }
         }
         __syncthreads();
@@ -692,6 +714,8 @@ __global__ void __launch_bounds__(kBlk) k_cell_macros(Geo g, FArgs a, Macros cm,
 #pragma unroll
         for (int k = 0; k < NM; ++k) m[k] += p[k];
     }
+#pragma unroll
+    for (int k = 0; k < NM; ++k) m[k] += a.cref[k];  // the storage reference's share
     double rho, T, tau;
     d3 U, q;
     macros_from_moments<D>(g, m, *dtp, rho, U, T, tau, q);
@@ -714,9 +738,10 @@ __global__ void __launch_bounds__(kBlk) k_cell_coef(Geo g, Macros cm, EqCoef *cc
 // the run (the wall and inlet values are constant, fvDVM.C:578-624, discreteVelocity.C:244-257),
 // and the coefficients pass B uses to form the incoming values
 template <int D>
-__global__ void __launch_bounds__(kBlk) k_incoming(Geo g, double *bin, BndCoef *bc)
+__global__ void __launch_bounds__(kBlk) k_incoming(Geo g, double *bin, BndCoef *bc, double *bref)
 {
     constexpr int NM = D == 3 ? 17 : 11;
+    constexpr int NMA = NM + 1;
     const int b = blockIdx.x * kBlk + threadIdx.x;
     if (b >= g.nB) return;
     const d3 Sf = g.Sf[g.nIF + b];
@@ -740,6 +765,33 @@ __global__ void __launch_bounds__(kBlk) k_incoming(Geo g, double *bin, BndCoef *
         add_moments<D>(m, g.w[v] * gv, g.w[v] * (gv * k.hfac), xi);
     }
     for (int j = 0; j < NM; ++j) bin[(size_t)b * NM + j] = m[j];
+    // outgoing half of the storage reference (the passes accumulate deviations from it),
+    // with its share of the wall's outgoing flux
+    double r[NMA];
+#pragma unroll
+    for (int j = 0; j < NMA; ++j) r[j] = 0.0;
+    for (int v = 0; v < g.Nv; ++v) {
+        const d3 xi = g.xi[v];
+        const double xs = sigma_dot(xi, Sf);
+        if (!(xs > 0)) continue;
+        const double wg = g.w[v] * g.ref_g[v];
+        add_moments<D>(r, wg, g.w[v] * g.ref_h[v], xi);
+        if (wall) r[NMA - 1] += wg * xs;
+    }
+    for (int j = 0; j < NMA; ++j) bref[(size_t)b * NMA + j] = r[j];
+}
+
+// moments of the storage reference over all velocities (one thread; run once per init)
+template <int D>
+__global__ void k_ref_moments(Geo g, double *cref)
+{
+    constexpr int NMA = (D == 3 ? 17 : 11) + 1;
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double r[NMA];
+#pragma unroll
+    for (int j = 0; j < NMA; ++j) r[j] = 0.0;
+    for (int v = 0; v < g.Nv; ++v) add_moments<D>(r, g.w[v] * g.ref_g[v], g.w[v] * g.ref_h[v], g.xi[v]);
+    for (int j = 0; j < NMA; ++j) cref[j] = r[j];
 }
 
 template <class S>
@@ -889,15 +941,19 @@ bool fused_plan_upload(FusedPlan &p, const Geo &g, size_t es, std::string &err)
         p.d_gs_ls = d;
     }
     void *fp = nullptr, *cp = nullptr, *bi = nullptr, *cc = nullptr, *fc = nullptr, *bc = nullptr;
+    void *br = nullptr, *cr = nullptr;
     const size_t nfpart = (size_t)p.nvg * g.nF * 2 * p.nma;
     if (!dbuf(p, &fp, nfpart * 8, err) || !dbuf(p, &cp, (size_t)p.nvg_b * g.nC * p.nm * 8, err) ||
-        !dbuf(p, &bi, (size_t)g.nB * p.nm * 8, err) || !dbuf(p, &cc, (size_t)g.nC * sizeof(EqCoef), err) ||
+        !dbuf(p, &bi, (size_t)g.nB * p.nm * 8, err) || !dbuf(p, &br, (size_t)g.nB * p.nma * 8, err) ||
+        !dbuf(p, &cr, (size_t)p.nma * 8, err) || !dbuf(p, &cc, (size_t)g.nC * sizeof(EqCoef), err) ||
         !dbuf(p, &fc, (size_t)g.nF * sizeof(EqCoef), err) || !dbuf(p, &bc, (size_t)g.nB * sizeof(BndCoef), err) ||
         !dbuf(p, &p.gF, (size_t)g.Nv * p.n_slots * es, err) || !dbuf(p, &p.hF, (size_t)g.Nv * p.n_slots * es, err))
         return false;
     p.fpart = (double *)fp;
     p.cpart = (double *)cp;
     p.bin = (double *)bi;
+    p.bref = (double *)br;
+    p.cref = (double *)cr;
     p.cc = (EqCoef *)cc;
     p.fc = (EqCoef *)fc;
     p.bc = (BndCoef *)bc;
@@ -916,10 +972,14 @@ bool fused_init_boundary(FusedPlan &p, const Geo &g, const Macros &cm, hipStream
 {
     if (g.nB > 0) {
         if (g.D == 3)
-            hipLaunchKernelGGL(k_incoming<3>, grid1(g.nB), dim3(kBlk), 0, st, g, p.bin, p.bc);
+            hipLaunchKernelGGL(k_incoming<3>, grid1(g.nB), dim3(kBlk), 0, st, g, p.bin, p.bc, p.bref);
         else
-            hipLaunchKernelGGL(k_incoming<2>, grid1(g.nB), dim3(kBlk), 0, st, g, p.bin, p.bc);
+            hipLaunchKernelGGL(k_incoming<2>, grid1(g.nB), dim3(kBlk), 0, st, g, p.bin, p.bc, p.bref);
     }
+    if (g.D == 3)
+        hipLaunchKernelGGL(k_ref_moments<3>, dim3(1), dim3(64), 0, st, g, p.cref);
+    else
+        hipLaunchKernelGGL(k_ref_moments<2>, dim3(1), dim3(64), 0, st, g, p.cref);
     hipLaunchKernelGGL(k_cell_coef, grid1(g.nC), dim3(kBlk), 0, st, g, cm, p.cc);
     if (hipGetLastError() != hipSuccess) {
         err = "fused init kernels failed to launch";

@@ -90,6 +90,8 @@ struct FusedPlan {
     double *fpart = nullptr;           // [nvg][nF][nma]
     double *cpart = nullptr;           // [nvg][nC][nm]
     double *bin = nullptr;             // [nB][nm] incoming moments (walls: per unit density)
+    double *bref = nullptr;            // [nB][nma] outgoing-half moments of the storage reference
+    double *cref = nullptr;            // [nma] all-velocity moments of the storage reference
     EqCoef *cc = nullptr, *fc = nullptr;  // equilibrium coefficients of cells / faces
     BndCoef *bc = nullptr;                // incoming-half coefficients of boundary faces
     void *gT2 = nullptr, *hT2 = nullptr;  // second distribution buffer

@@ -61,7 +61,7 @@ struct FArgs {
     const signed char *fbc_sgn;
     const void *gs_ls;
     double *fpart, *cpart;
-    const double *bin;
+    const double *bin, *bref, *cref;
     EqCoef *cc, *fc;
     const BndCoef *bc;
     int nblk, b8, nvg, n1, nrow, nchunk, cpg;
@@ -75,7 +75,7 @@ FArgs fargs(const FusedPlan &p)
     a.fa_face = p.d_fa_face; a.fa_own = p.d_fa_own; a.fa_nei = p.d_fa_nei;
     a.fb_face = p.d_fb_face; a.fb_own = p.d_fb_own; a.fb_nei = p.d_fb_nei;
     a.fbc_beg = p.d_fbc_beg; a.fbc_idx = p.d_fbc_idx; a.fbc_sgn = p.d_fbc_sgn; a.gs_ls = p.d_gs_ls;
-    a.fpart = p.fpart; a.cpart = p.cpart; a.bin = p.bin; a.cc = p.cc; a.fc = p.fc; a.bc = p.bc;
+    a.fpart = p.fpart; a.cpart = p.cpart; a.bin = p.bin; a.bref = p.bref; a.cref = p.cref; a.cc = p.cc; a.fc = p.fc; a.bc = p.bc;
     a.nblk = p.nblk; a.b8 = p.b8; a.nvg = p.nvg; a.n1 = p.n1; a.nrow = p.nrow; a.nchunk = p.nchunk; a.cpg = p.cpg;
     a.max_nl = p.max_nl; a.max_ng = p.max_ng; a.max_nfb = p.max_nfb; a.max_nc = p.max_nc;
     a.nm = p.nm; a.nma = p.nma;
@@ -276,8 +276,15 @@ __device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &
 #pragma unroll
         for (int vv = 0; vv < VC; ++vv) {
             if (vv >= ck.nvv) break;
+            const int v = ck.vrow + ck.ix0 + vv;
             double gEq, hEq;
-            shakhov_poly(g, k, g.xi[ck.vrow + ck.ix0 + vv], cex[s][vv] * eyz, gEq, hEq);
+            shakhov_poly(g, k, g.xi[v], cex[s][vv] * eyz, gEq, hEq);
+            // LDS holds f-bar-plus as deviations from the storage reference, like g~ itself
+            // (FP32 storage only; FP64 storage keeps values, its reference being zero)
+            if (sizeof(S) == 4) {
+                gEq -= g.ref_g[v];
+                hEq -= g.ref_h[v];
+            }
             fbg[vv * a.max_nl + l] = (S)((1.0 - af) * (double)gt[vv] + af * gEq);
             fbh[vv * a.max_nl + l] = (S)((1.0 - af) * (double)ht[vv] + af * hEq);
             if (ftg && l < bi.nc) {
@@ -344,7 +351,8 @@ __device__ inline S recon_l(const S *fb, const S *gr, int l, double ax, double a
 // ------------------------------------------------------------------------- pass A
 template <class S, int D, int VC>
 __global__ void __launch_bounds__(kNTA, kWavesA)
-    k_pass_a(Geo g, FArgs a, const S *__restrict__ gT, const S *__restrict__ hT, const double *dtp)
+    k_pass_a(Geo g, FArgs a, const S *__restrict__ gT,
+             const S *__restrict__ hT, const double *dtp)
 {
     constexpr int NMA = (D == 3 ? 17 : 11) + 1;
     constexpr int CPT = fused_cells_per_thread_a(D);
@@ -429,8 +437,8 @@ __global__ void __launch_bounds__(kNTA, kWavesA)
                     if (!(xs > 0)) continue;  // incoming half added from constants
                     gf = (double)recon_l<S, D>(pg, qg, fown[s], ox, oy, oz);
                     hf = (double)recon_l<S, D>(ph, qh, fown[s], ox, oy, oz);
-                    if (ftyp[s] == kMaxwellWall) acc[s][NMA - 1] += w * xs * gf;
                 }
+                if (ftyp[s] == kMaxwellWall) acc[s][NMA - 1] += w * xs * gf;
                 add_moments<D>(acc[s], w * gf, w * hf, xi);
             }
         }
@@ -461,6 +469,13 @@ __global__ void __launch_bounds__(kBlk)
         const double *p = a.fpart + ((size_t)vg * g.nF + f) * NMA;
 #pragma unroll
         for (int k = 0; k < NMA; ++k) m[k] += p[k];
+    }
+    {
+        // the storage reference's share: every velocity on internal faces, the outgoing half
+        // on boundary faces (the passes accumulate deviations from it)
+        const double *r = f < g.nIF ? a.cref : a.bref + (size_t)(f - g.nIF) * NMA;
+#pragma unroll
+        for (int k = 0; k < NMA; ++k) m[k] += r[k];
     }
     if (f >= g.nIF) {
         const int b = f - g.nIF;
@@ -613,6 +628,11 @@ __global__ void __launch_bounds__(kNTB, kWavesB)
                     hv = gv * kb.hfac;
                 }
                 if (relax) {
+                    if (sizeof(S) == 4) {  // reconstructed deviation -> value
+                        const int v = ck.vrow + ck.ix0 + vv;
+                        gv += g.ref_g[v];
+                        hv += g.ref_h[v];
+                    }
                     double gEq, hEq;
                     shakhov_poly(g, k, xi, fex[s][vv] * eyz, gEq, hEq);
                     gv = (1.0 - k.tr) * gv + k.tr * gEq;
@@ -649,7 +669,7 @@ __global__ void __launch_bounds__(kNTB, kWavesB)
                 gN[kk] = sg;
                 hN[kk] = sh;
                 const double w = g.w[v];
-                add_moments<D>(mom, w * (double)sg, w * (double)sh, g.xi[v]);
+                add_moments<D>(mom, w * (double)sg, w * (double)sh, g.xi[v]);  // deviations
             }
         }
         __syncthreads();
@@ -676,6 +696,8 @@ __global__ void __launch_bounds__(kBlk) k_cell_macros(Geo g, FArgs a, Macros cm,
 #pragma unroll
         for (int k = 0; k < NM; ++k) m[k] += p[k];
     }
+#pragma unroll
+    for (int k = 0; k < NM; ++k) m[k] += a.cref[k];  // the storage reference's share
     double rho, T, tau;
     d3 U, q;
     macros_from_moments<D>(g, m, *dtp, rho, U, T, tau, q);
@@ -698,9 +720,10 @@ __global__ void __launch_bounds__(kBlk) k_cell_coef(Geo g, Macros cm, EqCoef *cc
 // the run (the wall and inlet values are constant, fvDVM.C:578-624, discreteVelocity.C:244-257),
 // and the coefficients pass B uses to form the incoming values
 template <int D>
-__global__ void __launch_bounds__(kBlk) k_incoming(Geo g, double *bin, BndCoef *bc)
+__global__ void __launch_bounds__(kBlk) k_incoming(Geo g, double *bin, BndCoef *bc, double *bref)
 {
     constexpr int NM = D == 3 ? 17 : 11;
+    constexpr int NMA = NM + 1;
     const int b = blockIdx.x * kBlk + threadIdx.x;
     if (b >= g.nB) return;
     const d3 Sf = g.Sf[g.nIF + b];
@@ -724,6 +747,33 @@ __global__ void __launch_bounds__(kBlk) k_incoming(Geo g, double *bin, BndCoef *
         add_moments<D>(m, g.w[v] * gv, g.w[v] * (gv * k.hfac), xi);
     }
     for (int j = 0; j < NM; ++j) bin[(size_t)b * NM + j] = m[j];
+    // outgoing half of the storage reference (the passes accumulate deviations from it),
+    // with its share of the wall's outgoing flux
+    double r[NMA];
+#pragma unroll
+    for (int j = 0; j < NMA; ++j) r[j] = 0.0;
+    for (int v = 0; v < g.Nv; ++v) {
+        const d3 xi = g.xi[v];
+        const double xs = sigma_dot(xi, Sf);
+        if (!(xs > 0)) continue;
+        const double wg = g.w[v] * g.ref_g[v];
+        add_moments<D>(r, wg, g.w[v] * g.ref_h[v], xi);
+        if (wall) r[NMA - 1] += wg * xs;
+    }
+    for (int j = 0; j < NMA; ++j) bref[(size_t)b * NMA + j] = r[j];
+}
+
+// moments of the storage reference over all velocities (one thread; run once per init)
+template <int D>
+__global__ void k_ref_moments(Geo g, double *cref)
+{
+    constexpr int NMA = (D == 3 ? 17 : 11) + 1;
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double r[NMA];
+#pragma unroll
+    for (int j = 0; j < NMA; ++j) r[j] = 0.0;
+    for (int v = 0; v < g.Nv; ++v) add_moments<D>(r, g.w[v] * g.ref_g[v], g.w[v] * g.ref_h[v], g.xi[v]);
+    for (int j = 0; j < NMA; ++j) cref[j] = r[j];
 }
 
 template <class S>
@@ -868,14 +918,18 @@ bool fused_plan_upload(FusedPlan &p, const Geo &g, size_t es, std::string &err)
         p.d_gs_ls = d;
     }
     void *fp = nullptr, *cp = nullptr, *bi = nullptr, *cc = nullptr, *fc = nullptr, *bc = nullptr;
+    void *br = nullptr, *cr = nullptr;
     if (!dbuf(p, &fp, (size_t)p.nvg * g.nF * p.nma * 8, err) || !dbuf(p, &cp, (size_t)p.nvg * g.nC * p.nm * 8, err) ||
-        !dbuf(p, &bi, (size_t)g.nB * p.nm * 8, err) || !dbuf(p, &cc, (size_t)g.nC * sizeof(EqCoef), err) ||
+        !dbuf(p, &bi, (size_t)g.nB * p.nm * 8, err) || !dbuf(p, &br, (size_t)g.nB * p.nma * 8, err) ||
+        !dbuf(p, &cr, (size_t)p.nma * 8, err) || !dbuf(p, &cc, (size_t)g.nC * sizeof(EqCoef), err) ||
         !dbuf(p, &fc, (size_t)g.nF * sizeof(EqCoef), err) || !dbuf(p, &bc, (size_t)g.nB * sizeof(BndCoef), err) ||
         !dbuf(p, &p.gT2, (size_t)g.Nv * g.nC * es, err) || !dbuf(p, &p.hT2, (size_t)g.Nv * g.nC * es, err))
         return false;
     p.fpart = (double *)fp;
     p.cpart = (double *)cp;
     p.bin = (double *)bi;
+    p.bref = (double *)br;
+    p.cref = (double *)cr;
     p.cc = (EqCoef *)cc;
     p.fc = (EqCoef *)fc;
     p.bc = (BndCoef *)bc;
@@ -888,10 +942,14 @@ bool fused_init_boundary(FusedPlan &p, const Geo &g, const Macros &cm, hipStream
 {
     if (g.nB > 0) {
         if (g.D == 3)
-            hipLaunchKernelGGL(k_incoming<3>, grid1(g.nB), dim3(kBlk), 0, st, g, p.bin, p.bc);
+            hipLaunchKernelGGL(k_incoming<3>, grid1(g.nB), dim3(kBlk), 0, st, g, p.bin, p.bc, p.bref);
         else
-            hipLaunchKernelGGL(k_incoming<2>, grid1(g.nB), dim3(kBlk), 0, st, g, p.bin, p.bc);
+            hipLaunchKernelGGL(k_incoming<2>, grid1(g.nB), dim3(kBlk), 0, st, g, p.bin, p.bc, p.bref);
     }
+    if (g.D == 3)
+        hipLaunchKernelGGL(k_ref_moments<3>, dim3(1), dim3(64), 0, st, g, p.cref);
+    else
+        hipLaunchKernelGGL(k_ref_moments<2>, dim3(1), dim3(64), 0, st, g, p.cref);
     hipLaunchKernelGGL(k_cell_coef, grid1(g.nC), dim3(kBlk), 0, st, g, cm, p.cc);
     if (hipGetLastError() != hipSuccess) {
         err = "fused init kernels failed to launch";

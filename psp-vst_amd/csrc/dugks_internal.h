@@ -61,6 +61,10 @@ struct Geo {
     // velocity set
     const d3 *xi;            // [Nv]
     const double *w;         // [Nv]
+    // Stored g~, h~ are deviations from a reference Maxwellian per velocity
+    // (rho = 1, U = 0, T = Tref; h = (K+3-D) R Tref g): value = stored + ref.  FP32 storage
+    // then rounds the deviation, not the full value.  Zero for FP64 storage.
+    const double *ref_g, *ref_h;  // [Nv]
 };
 
 struct Macros {
@@ -78,7 +82,7 @@ struct DevState {
     int *cg_off = nullptr, *cg_nb = nullptr, *bf_type = nullptr, *bf_local = nullptr;
     d3 *Sf = nullptr, *Cf = nullptr, *C = nullptr, *cg_ls = nullptr, *bf_U = nullptr, *xi = nullptr;
     double *V = nullptr, *bf_rho = nullptr, *bf_T = nullptr, *in_by_rho = nullptr, *dc = nullptr, *w = nullptr,
-           *weights = nullptr;
+           *weights = nullptr, *ref_g = nullptr, *ref_h = nullptr;
     // distributions (storage type chosen at run time; void* + element size)
     void *gT = nullptr, *hT = nullptr, *gB = nullptr, *hB = nullptr, *gS = nullptr, *hS = nullptr;
     // macros

@@ -78,8 +78,8 @@ __global__ void __launch_bounds__(kBlock) k_init_df(Geo g, Macros cm, S *gT, S *
     const double cSqrByRT = magsqr(d) / (g.R * T);
     const double gEq = cm.rho[c] / pow(sqrt(2.0 * kPi * g.R * T), (double)g.D) * exp(-cSqrByRT / 2.0);
     const size_t k = (size_t)v * g.nC + c;
-    gT[k] = (S)gEq;
-    hT[k] = (S)((g.K + 3.0 - g.D) * gEq * g.R * T);
+    gT[k] = (S)(gEq - g.ref_g[v]);
+    hT[k] = (S)((g.K + 3.0 - g.D) * gEq * g.R * T - g.ref_h[v]);
 }
 
 template <class S>
@@ -157,8 +157,8 @@ __global__ void __launch_bounds__(kBlock)
     shakhov(g, cm.rho[c], cm.U[c], T, cm.q[c], g.xi[v], T, gEq, hEq);
     const double a = 1.5 * dt / (2.0 * cm.tau[c] + dt);
     const size_t k = (size_t)v * g.nC + c;
-    gB[k] = (S)((1.0 - a) * (double)gT[k] + a * gEq);
-    hB[k] = (S)((1.0 - a) * (double)hT[k] + a * hEq);
+    gB[k] = (S)((1.0 - a) * ((double)gT[k] + g.ref_g[v]) + a * gEq);
+    hB[k] = (S)((1.0 - a) * ((double)hT[k] + g.ref_h[v]) + a * hEq);
 }
 
 // updateGrad + updateGHbarSurf (fvDVM.C:749-1029), outgoing boundary values included
@@ -315,8 +315,8 @@ __global__ void __launch_bounds__(kBlock)
     const double dt = *dtp;
     const d3 xi = g.xi[v];
     const size_t kc = (size_t)v * g.nC + c;
-    double ag = -1.0 / 3 * (double)gT[kc] + 4.0 / 3 * (double)gB[kc];
-    double ah = -1.0 / 3 * (double)hT[kc] + 4.0 / 3 * (double)hB[kc];
+    double ag = -1.0 / 3 * ((double)gT[kc] + g.ref_g[v]) + 4.0 / 3 * (double)gB[kc];
+    double ah = -1.0 / 3 * ((double)hT[kc] + g.ref_h[v]) + 4.0 / 3 * (double)hB[kc];
     const double V = g.V[c];
     const S *gv = gS + (size_t)v * g.nF;
     const S *hv = hS + (size_t)v * g.nF;
@@ -333,8 +333,8 @@ __global__ void __launch_bounds__(kBlock)
             ah -= fh;
         }
     }
-    gT[kc] = (S)ag;
-    hT[kc] = (S)ah;
+    gT[kc] = (S)(ag - g.ref_g[v]);
+    hT[kc] = (S)(ah - g.ref_h[v]);
 }
 
 // updateMacroVol (fvDVM.C:1521-1554, 1682-1743)
@@ -348,7 +348,7 @@ __global__ void __launch_bounds__(kBlock)
     double rho = 0.0, rx = 0.0, ry = 0.0, rz = 0.0, rhoE = 0.0;
     for (int v = 0; v < g.Nv; ++v) {
         const size_t k = (size_t)v * g.nC + c;
-        const double gv = (double)gT[k], hv = (double)hT[k];
+        const double gv = (double)gT[k] + g.ref_g[v], hv = (double)hT[k] + g.ref_h[v];
         const d3 xi = g.xi[v];
         const double w = g.w[v];
         rho += w * gv;
@@ -364,7 +364,7 @@ __global__ void __launch_bounds__(kBlock)
     double qx = 0.0, qy = 0.0, qz = 0.0;
     for (int v = 0; v < g.Nv; ++v) {
         const size_t k = (size_t)v * g.nC + c;
-        const double gv = (double)gT[k], hv = (double)hT[k];
+        const double gv = (double)gT[k] + g.ref_g[v], hv = (double)hT[k] + g.ref_h[v];
         const d3 xi = g.xi[v];
         const d3 cc{xi.x - U.x, xi.y - U.y, xi.z - U.z};
         const double m = magsqr(cc) * gv + hv;
