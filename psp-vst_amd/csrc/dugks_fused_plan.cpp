@@ -3,6 +3,7 @@
 // cells, the faces of the block cells with their gather order, the twin slot of every face
 // two blocks share) that the kernels follow.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <numeric>
 #include <string>
@@ -183,39 +184,56 @@ bool fused_plan_build(FusedPlan &p, const Geo &g, const HostPrep &prep, const st
             }
         }
         p.gs_beg.push_back((int)p.gs_nb.size());
-        // faces of the block cells: unique slots + per-cell gather order (ascending face label)
+        // faces of the block cells, one slot each: faces inside the block first, then the
+        // faces shared with each adjacent block (grouped by that block, ascending face
+        // label), then boundary faces.  Both blocks of a shared face order their group the
+        // same way, so the twin slots of a run of lanes are a contiguous run in the other
+        // block and pass B's reads of the other block's values stay coalesced.
         bi.fs_off = (int)p.fs_face.size();
         std::unordered_map<int, int> slot_of;
+        std::vector<std::array<int, 3>> keyed;  // (group, adjacent block, face)
+        for (int c = c0; c < c1; ++c)
+            for (int k = cf_off[c]; k < cf_off[c + 1]; ++k) {
+                const int f = cf_face[k];
+                if (!slot_of.emplace(f, 0).second) continue;
+                if (f >= nIF) {
+                    keyed.push_back({2, 0, f});
+                } else {
+                    const int bo = block_of[owner[f]], bn = block_of[neighbour[f]];
+                    if (bo == b && bn == b)
+                        keyed.push_back({0, 0, f});
+                    else
+                        keyed.push_back({1, bo == b ? bn : bo, f});
+                }
+            }
+        std::sort(keyed.begin(), keyed.end());
+        for (const auto &kf : keyed) {
+            const int f = kf[2];
+            slot_of[f] = (int)p.fs_face.size() - bi.fs_off;
+            const int o = owner[f], n = f < nIF ? neighbour[f] : -1;
+            int flag = 0;
+            if (block_of[o] == b) flag |= kSlotOwnIn;
+            if (n >= 0 && block_of[n] == b) flag |= kSlotNeiIn;
+            const int gslot = (int)p.fs_face.size();
+            p.fs_face.push_back(f);
+            p.fs_twin.push_back(-1);
+            if (first_slot[f] < 0) {
+                first_slot[f] = gslot;
+            } else {
+                p.fs_twin[gslot] = first_slot[f];
+                p.fs_twin[first_slot[f]] = gslot;
+            }
+            p.fs_lo.push_back(local[o]);
+            p.fs_ln.push_back(n >= 0 ? local[n] : -1);
+            p.fs_flag.push_back((signed char)flag);
+        }
+        // per-cell gather order: the cell's faces in ascending face label
         bi.fbc_off = (int)p.fbc_beg.size();
         for (int c = c0; c < c1; ++c) {
             p.fbc_beg.push_back((int)p.fbc_idx.size());
             for (int k = cf_off[c]; k < cf_off[c + 1]; ++k) {
                 const int f = cf_face[k];
-                auto it = slot_of.find(f);
-                int idx;
-                if (it == slot_of.end()) {
-                    idx = (int)p.fs_face.size() - bi.fs_off;
-                    slot_of.emplace(f, idx);
-                    const int o = owner[f], n = f < nIF ? neighbour[f] : -1;
-                    int flag = 0;
-                    if (block_of[o] == b) flag |= kSlotOwnIn;
-                    if (n >= 0 && block_of[n] == b) flag |= kSlotNeiIn;
-                    const int gslot = (int)p.fs_face.size();
-                    p.fs_face.push_back(f);
-                    p.fs_twin.push_back(-1);
-                    if (first_slot[f] < 0) {
-                        first_slot[f] = gslot;
-                    } else {
-                        p.fs_twin[gslot] = first_slot[f];
-                        p.fs_twin[first_slot[f]] = gslot;
-                    }
-                    p.fs_lo.push_back(local[o]);
-                    p.fs_ln.push_back(n >= 0 ? local[n] : -1);
-                    p.fs_flag.push_back((signed char)flag);
-                } else {
-                    idx = it->second;
-                }
-                p.fbc_idx.push_back(idx);
+                p.fbc_idx.push_back(slot_of[f]);
                 // the owner subtracts the flux, the neighbour adds it (fvDVM.C:1493-1496)
                 p.fbc_sgn.push_back((signed char)((f < nIF && owner[f] != c) ? 1 : -1));
             }
