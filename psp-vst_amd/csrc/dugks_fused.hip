@@ -47,6 +47,12 @@ struct V3 {
     S x, y, z;
 };
 
+// g and h of one (cell, velocity) side by side: one LDS access moves both
+template <class S>
+struct alignas(2 * sizeof(S)) P2 {
+    S g, h;
+};
+
 struct FArgs {
     const BlockInfo *blk;
     const int *lcell, *gs_beg, *gs_nb, *fs_face, *fs_twin, *fs_lo, *fs_ln, *fbc_beg, *fbc_idx;
@@ -289,7 +295,7 @@ __device__ inline void cell_bar(const Geo &g, const FArgs &a, int c, const S *__
 template <class S, int VC, int NT, int CPT>
 __device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &bi, const S *__restrict__ gT,
                                  const S *__restrict__ hT, double dt, const Chunk &ck, double (&cex)[CPT][VC],
-                                 S *fbg, S *fbh)
+                                 P2<S> *fb)
 {
     const d3 xr = g.xi[ck.vrow];
 #pragma unroll
@@ -302,16 +308,14 @@ __device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &
 #pragma unroll
         for (int vv = 0; vv < VC; ++vv) {
             if (vv >= ck.nvv) break;
-            fbg[vv * a.max_nl + l] = (S)gb[vv];
-            fbh[vv * a.max_nl + l] = (S)hb[vv];
+            fb[vv * a.max_nl + l] = P2<S>{(S)gb[vv], (S)hb[vv]};
         }
     }
 }
 
 // pass A stage 2: gradients of the block cells
 template <class S, int D, int VC, int NT>
-__device__ inline void stage_grad(const FArgs &a, const BlockInfo &bi, int nvv, const S *fbg, const S *fbh, S *grg,
-                                  S *grh)
+__device__ inline void stage_grad(const FArgs &a, const BlockInfo &bi, int nvv, const P2<S> *fb, P2<S> *gr)
 {
     const V3<S> *ls = (const V3<S> *)a.gs_ls;
     for (int l = threadIdx.x; l < bi.nc; l += NT) {
@@ -328,13 +332,14 @@ __device__ inline void stage_grad(const FArgs &a, const BlockInfo &bi, int nvv, 
         }
 #pragma unroll 1
         for (int vv = 0; vv < nvv; ++vv) {
-            const S *pg = fbg + vv * a.max_nl, *ph = fbh + vv * a.max_nl;
-            const S cg = pg[l], ch = ph[l];
+            const P2<S> *p = fb + vv * a.max_nl;
+            const P2<S> c = p[l];
             S gx = 0, gy = 0, gz = 0, hx = 0, hy = 0, hz = 0;
 #pragma unroll
             for (int j = 0; j < kNb; ++j) {
                 if (j < n) {
-                    const S dg = pg[nb[j]] - cg, dh = ph[nb[j]] - ch;
+                    const P2<S> q = p[nb[j]];
+                    const S dg = q.g - c.g, dh = q.h - c.h;
                     gx += L[j].x * dg; gy += L[j].y * dg;
                     hx += L[j].x * dh; hy += L[j].y * dh;
                     if (D == 3) {
@@ -343,22 +348,27 @@ __device__ inline void stage_grad(const FArgs &a, const BlockInfo &bi, int nvv, 
                     }
                 }
             }
-            S *og = grg + (vv * a.max_nc + l) * D, *oh = grh + (vv * a.max_nc + l) * D;
-            og[0] = gx; og[1] = gy; oh[0] = hx; oh[1] = hy;
-            if (D == 3) {
-                og[2] = gz;
-                oh[2] = hz;
-            }
+            P2<S> *o = gr + (vv * a.max_nc + l) * D;
+            o[0] = P2<S>{gx, hx};
+            o[1] = P2<S>{gy, hy};
+            if (D == 3) o[2] = P2<S>{gz, hz};
         }
     }
 }
 
+// g and h at arm (ax, ay, az) from local cell l: f + grad f . arm
 template <class S, int D>
-__device__ inline S recon_l(const S *fb, const S *gr, int l, double ax, double ay, double az)
+__device__ inline P2<S> recon_l(const P2<S> *fb, const P2<S> *gr, int l, double ax, double ay, double az)
 {
-    const S *gl = gr + l * D;
-    if (D == 3) return fb[l] + (gl[0] * (S)ax + gl[1] * (S)ay + gl[2] * (S)az);
-    return fb[l] + (gl[0] * (S)ax + gl[1] * (S)ay);
+    const P2<S> *gl = gr + l * D;
+    const P2<S> f = fb[l];
+    const S sx = (S)ax, sy = (S)ay;
+    if (D == 3) {
+        const S sz = (S)az;
+        return P2<S>{f.g + (gl[0].g * sx + gl[1].g * sy + gl[2].g * sz),
+                     f.h + (gl[0].h * sx + gl[1].h * sy + gl[2].h * sz)};
+    }
+    return P2<S>{f.g + (gl[0].g * sx + gl[1].g * sy), f.h + (gl[0].h * sx + gl[1].h * sy)};
 }
 
 // ------------------------------------------------------------------------- pass A
@@ -374,10 +384,8 @@ __global__ void __launch_bounds__(kNTA)
     extern __shared__ __align__(16) unsigned char smem[];
     const BlockInfo bi = a.blk[blkid];
     const int ch0 = vg * a.cpg, ch1 = min(a.nchunk, ch0 + a.cpg);
-    S *fbg = (S *)smem;
-    S *fbh = fbg + VC * a.max_nl;
-    S *grg = fbh + VC * a.max_nl;
-    S *grh = grg + VC * a.max_nc * D;
+    P2<S> *fb = (P2<S> *)smem;        // [VC][max_nl]
+    P2<S> *gr = fb + VC * a.max_nl;    // [VC][max_nc][D]
     const double dt = *dtp;
     const double hx = 0.5 * dt;
 
@@ -417,9 +425,9 @@ __global__ void __launch_bounds__(kNTA)
     int stripe = -1;
     for (int ch = ch0; ch < ch1; ++ch) {
         const Chunk ck = chunk_of(a, ch, VC, stripe);
-        stage_bar<S, VC, kNTA, CPT>(g, a, bi, gT, hT, dt, ck, cex, fbg, fbh);
+        stage_bar<S, VC, kNTA, CPT>(g, a, bi, gT, hT, dt, ck, cex, fb);
         __syncthreads();
-        stage_grad<S, D, VC, kNTA>(a, bi, ck.nvv, fbg, fbh, grg, grh);
+        stage_grad<S, D, VC, kNTA>(a, bi, ck.nvv, fb, gr);
         __syncthreads();
 #pragma unroll
         for (int s = 0; s < kSlotsA; ++s) {
@@ -431,44 +439,48 @@ __global__ void __launch_bounds__(kNTA)
                 const d3 xi = g.xi[v];
                 const double w = g.w[v];
                 const double xs = sigma_dot(xi, fS[s]);
-                const S *pg = fbg + vv * a.max_nl, *ph = fbh + vv * a.max_nl;
-                const S *qg = grg + vv * a.max_nc * D, *qh = grh + vv * a.max_nc * D;
+                const P2<S> *pf = fb + vv * a.max_nl, *pq = gr + vv * a.max_nc * D;
                 double gf, hf;
                 bool store = true;
                 if (ftyp[s] == 0) {
                     if (xs >= kVSmall) {
                         if (!own_in) continue;
                         const double ox = armO[s].x - xi.x * hx, oy = armO[s].y - xi.y * hx, oz = armO[s].z - xi.z * hx;
-                        gf = (double)recon_l<S, D>(pg, qg, flo[s], ox, oy, oz);
-                        hf = (double)recon_l<S, D>(ph, qh, flo[s], ox, oy, oz);
+                        const P2<S> r = recon_l<S, D>(pf, pq, flo[s], ox, oy, oz);
+                        gf = (double)r.g;
+                        hf = (double)r.h;
                     } else if (xs < -kVSmall) {
                         if (!nei_in) continue;
                         const double nx = armN[s].x - xi.x * hx, ny = armN[s].y - xi.y * hx, nz = armN[s].z - xi.z * hx;
-                        gf = (double)recon_l<S, D>(pg, qg, fln[s], nx, ny, nz);
-                        hf = (double)recon_l<S, D>(ph, qh, fln[s], nx, ny, nz);
+                        const P2<S> r = recon_l<S, D>(pf, pq, fln[s], nx, ny, nz);
+                        gf = (double)r.g;
+                        hf = (double)r.h;
                     } else {
                         // no flux; each side adds half of its reconstruction to the moments
                         store = false;
                         const double ox = armO[s].x - xi.x * hx, oy = armO[s].y - xi.y * hx, oz = armO[s].z - xi.z * hx;
                         const double nx = armN[s].x - xi.x * hx, ny = armN[s].y - xi.y * hx, nz = armN[s].z - xi.z * hx;
                         if (own_in && nei_in) {
-                            gf = 0.5 * ((double)recon_l<S, D>(pg, qg, fln[s], nx, ny, nz) +
-                                        (double)recon_l<S, D>(pg, qg, flo[s], ox, oy, oz));
-                            hf = 0.5 * ((double)recon_l<S, D>(ph, qh, fln[s], nx, ny, nz) +
-                                        (double)recon_l<S, D>(ph, qh, flo[s], ox, oy, oz));
+                            const P2<S> rn = recon_l<S, D>(pf, pq, fln[s], nx, ny, nz);
+                            const P2<S> ro = recon_l<S, D>(pf, pq, flo[s], ox, oy, oz);
+                            gf = 0.5 * ((double)rn.g + (double)ro.g);
+                            hf = 0.5 * ((double)rn.h + (double)ro.h);
                         } else if (own_in) {
-                            gf = 0.5 * (double)recon_l<S, D>(pg, qg, flo[s], ox, oy, oz);
-                            hf = 0.5 * (double)recon_l<S, D>(ph, qh, flo[s], ox, oy, oz);
+                            const P2<S> r = recon_l<S, D>(pf, pq, flo[s], ox, oy, oz);
+                            gf = 0.5 * (double)r.g;
+                            hf = 0.5 * (double)r.h;
                         } else {
-                            gf = 0.5 * (double)recon_l<S, D>(pg, qg, fln[s], nx, ny, nz);
-                            hf = 0.5 * (double)recon_l<S, D>(ph, qh, fln[s], nx, ny, nz);
+                            const P2<S> r = recon_l<S, D>(pf, pq, fln[s], nx, ny, nz);
+                            gf = 0.5 * (double)r.g;
+                            hf = 0.5 * (double)r.h;
                         }
                     }
                 } else {
                     if (!(xs > 0)) continue;  // incoming half added from constants
                     const double ox = armO[s].x - xi.x * hx, oy = armO[s].y - xi.y * hx, oz = armO[s].z - xi.z * hx;
-                    gf = (double)recon_l<S, D>(pg, qg, flo[s], ox, oy, oz);
-                    hf = (double)recon_l<S, D>(ph, qh, flo[s], ox, oy, oz);
+                    const P2<S> r = recon_l<S, D>(pf, pq, flo[s], ox, oy, oz);
+                    gf = (double)r.g;
+                    hf = (double)r.h;
                 }
                 if (store) {  // face values are stored as deviations
                     const size_t kf = (size_t)v * a.n_slots + fpos[s];
