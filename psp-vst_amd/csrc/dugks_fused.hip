@@ -586,8 +586,7 @@ __global__ void __launch_bounds__(kNTB)
     extern __shared__ __align__(16) unsigned char smem[];
     const BlockInfo bi = a.blk[blkid];
     const int ch0 = vg * a.cpg, ch1 = min(a.nchunk, ch0 + a.cpg);
-    double *flg = (double *)smem;                   // [VC][max_nfs]
-    double *flh = flg + VC * a.max_nfs;
+    P2<double> *fl = (P2<double> *)smem;            // [VC][max_nfs] fluxes sigma * f_f
     const double dt = *dtp;
 
     double mom[NM];
@@ -667,8 +666,7 @@ __global__ void __launch_bounds__(kNTB)
                     fg = xs * gv;
                     fh = xs * (gv * kb.hfac);
                 }
-                flg[vv * a.max_nfs + j] = fg;
-                flh[vv * a.max_nfs + j] = fh;
+                fl[vv * a.max_nfs + j] = P2<double>{fg, fh};
             }
         }
         __syncthreads();
@@ -683,8 +681,9 @@ __global__ void __launch_bounds__(kNTB)
 #pragma unroll
                 for (int j = 0; j < kNb; ++j) {
                     if (j < nf) {
-                        const double tg = flg[vv * a.max_nfs + fidx[j]] * dtV;
-                        const double th = flh[vv * a.max_nfs + fidx[j]] * dtV;
+                        const P2<double> q = fl[vv * a.max_nfs + fidx[j]];
+                        const double tg = q.g * dtV;
+                        const double th = q.h * dtV;
                         if (fsg[j] > 0) {
                             ag += tg;
                             ah += th;

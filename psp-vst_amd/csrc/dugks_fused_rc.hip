@@ -55,6 +55,12 @@ struct V3 {
     S x, y, z;
 };
 
+// g and h of one (cell or face, velocity) side by side: one LDS access moves both
+template <class S>
+struct alignas(2 * sizeof(S)) P2 {
+    S g, h;
+};
+
 struct FArgs {
     const BlockInfo *blk;
     const int *lcell, *gs_beg, *gs_nb, *fa_face, *fa_own, *fa_nei, *fb_face, *fb_own, *fb_nei, *fbc_beg, *fbc_idx;
@@ -254,7 +260,7 @@ __device__ inline Chunk chunk_of(const FArgs &a, int ch, int vc, int &stripe)
 template <class S, int VC, int NT, int CPT>
 __device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &bi, const S *__restrict__ gT,
                                  const S *__restrict__ hT, double dt, const Chunk &ck, double (&cex)[CPT][VC],
-                                 S *fbg, S *fbh, S *ftg, S *fth)
+                                 P2<S> *fb, P2<S> *ft)
 {
     const d3 xr = g.xi[ck.vrow];
 #pragma unroll
@@ -285,11 +291,10 @@ __device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &
                 gEq -= g.ref_g[v];
                 hEq -= g.ref_h[v];
             }
-            fbg[vv * a.max_nl + l] = (S)((1.0 - af) * (double)gt[vv] + af * gEq);
-            fbh[vv * a.max_nl + l] = (S)((1.0 - af) * (double)ht[vv] + af * hEq);
-            if (ftg && l < bi.nc) {
-                ftg[vv * a.max_nc + l] = gt[vv];
-                fth[vv * a.max_nc + l] = ht[vv];
+            fb[vv * a.max_nl + l] = P2<S>{(S)((1.0 - af) * (double)gt[vv] + af * gEq),
+                                          (S)((1.0 - af) * (double)ht[vv] + af * hEq)};
+            if (ft && l < bi.nc) {
+                ft[vv * a.max_nc + l] = P2<S>{gt[vv], ht[vv]};
             }
         }
     }
@@ -297,8 +302,7 @@ __device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &
 
 // stage 2: gradients of block + ring-1 cells
 template <class S, int D, int VC, int NT>
-__device__ inline void stage_grad(const FArgs &a, const BlockInfo &bi, int nvv, const S *fbg, const S *fbh, S *grg,
-                                  S *grh)
+__device__ inline void stage_grad(const FArgs &a, const BlockInfo &bi, int nvv, const P2<S> *fb, P2<S> *gr)
 {
     const V3<S> *ls = (const V3<S> *)a.gs_ls;
     for (int l = threadIdx.x; l < bi.ng; l += NT) {
@@ -315,13 +319,14 @@ __device__ inline void stage_grad(const FArgs &a, const BlockInfo &bi, int nvv, 
         }
 #pragma unroll 1
         for (int vv = 0; vv < nvv; ++vv) {
-            const S *pg = fbg + vv * a.max_nl, *ph = fbh + vv * a.max_nl;
-            const S cg = pg[l], ch = ph[l];
+            const P2<S> *p = fb + vv * a.max_nl;
+            const P2<S> c = p[l];
             S gx = 0, gy = 0, gz = 0, hx = 0, hy = 0, hz = 0;
 #pragma unroll
             for (int j = 0; j < kNb; ++j) {
                 if (j < n) {
-                    const S dg = pg[nb[j]] - cg, dh = ph[nb[j]] - ch;
+                    const P2<S> q = p[nb[j]];
+                    const S dg = q.g - c.g, dh = q.h - c.h;
                     gx += L[j].x * dg; gy += L[j].y * dg;
                     hx += L[j].x * dh; hy += L[j].y * dh;
                     if (D == 3) {
@@ -330,22 +335,27 @@ __device__ inline void stage_grad(const FArgs &a, const BlockInfo &bi, int nvv, 
                     }
                 }
             }
-            S *og = grg + (vv * a.max_ng + l) * D, *oh = grh + (vv * a.max_ng + l) * D;
-            og[0] = gx; og[1] = gy; oh[0] = hx; oh[1] = hy;
-            if (D == 3) {
-                og[2] = gz;
-                oh[2] = hz;
-            }
+            P2<S> *o = gr + (vv * a.max_ng + l) * D;
+            o[0] = P2<S>{gx, hx};
+            o[1] = P2<S>{gy, hy};
+            if (D == 3) o[2] = P2<S>{gz, hz};
         }
     }
 }
 
+// g and h at arm (ax, ay, az) from local cell l: f + grad f . arm
 template <class S, int D>
-__device__ inline S recon_l(const S *fb, const S *gr, int l, double ax, double ay, double az)
+__device__ inline P2<S> recon_l(const P2<S> *fb, const P2<S> *gr, int l, double ax, double ay, double az)
 {
-    const S *gl = gr + l * D;
-    if (D == 3) return fb[l] + (gl[0] * (S)ax + gl[1] * (S)ay + gl[2] * (S)az);
-    return fb[l] + (gl[0] * (S)ax + gl[1] * (S)ay);
+    const P2<S> *gl = gr + l * D;
+    const P2<S> f = fb[l];
+    const S sx = (S)ax, sy = (S)ay;
+    if (D == 3) {
+        const S sz = (S)az;
+        return P2<S>{f.g + (gl[0].g * sx + gl[1].g * sy + gl[2].g * sz),
+                     f.h + (gl[0].h * sx + gl[1].h * sy + gl[2].h * sz)};
+    }
+    return P2<S>{f.g + (gl[0].g * sx + gl[1].g * sy), f.h + (gl[0].h * sx + gl[1].h * sy)};
 }
 
 // ------------------------------------------------------------------------- pass A
@@ -361,10 +371,8 @@ __global__ void __launch_bounds__(kNTA, kWavesA)
     extern __shared__ __align__(16) unsigned char smem[];
     const BlockInfo bi = a.blk[blkid];
     const int ch0 = vg * a.cpg, ch1 = min(a.nchunk, ch0 + a.cpg);
-    S *fbg = (S *)smem;
-    S *fbh = fbg + VC * a.max_nl;
-    S *grg = fbh + VC * a.max_nl;
-    S *grh = grg + VC * a.max_ng * D;
+    P2<S> *fb = (P2<S> *)smem;        // [VC][max_nl]
+    P2<S> *gr = fb + VC * a.max_nl;    // [VC][max_ng][D]
     const double dt = *dtp;
     const double hx = 0.5 * dt;
 
@@ -402,9 +410,9 @@ __global__ void __launch_bounds__(kNTA, kWavesA)
     int stripe = -1;
     for (int ch = ch0; ch < ch1; ++ch) {
         const Chunk ck = chunk_of(a, ch, VC, stripe);
-        stage_bar<S, VC, kNTA, CPT>(g, a, bi, gT, hT, dt, ck, cex, fbg, fbh, (S *)nullptr, (S *)nullptr);
+        stage_bar<S, VC, kNTA, CPT>(g, a, bi, gT, hT, dt, ck, cex, fb, (P2<S> *)nullptr);
         __syncthreads();
-        stage_grad<S, D, VC, kNTA>(a, bi, ck.nvv, fbg, fbh, grg, grh);
+        stage_grad<S, D, VC, kNTA>(a, bi, ck.nvv, fb, gr);
         __syncthreads();
 #pragma unroll
         for (int s = 0; s < kSlotsA; ++s) {
@@ -415,28 +423,30 @@ __global__ void __launch_bounds__(kNTA, kWavesA)
                 const d3 xi = g.xi[v];
                 const double w = g.w[v];
                 const double xs = sigma_dot(xi, fS[s]);
-                const S *pg = fbg + vv * a.max_nl, *ph = fbh + vv * a.max_nl;
-                const S *qg = grg + vv * a.max_ng * D, *qh = grh + vv * a.max_ng * D;
+                const P2<S> *pf = fb + vv * a.max_nl, *pq = gr + vv * a.max_ng * D;
                 const double ox = armO[s].x - xi.x * hx, oy = armO[s].y - xi.y * hx, oz = armO[s].z - xi.z * hx;
                 double gf, hf;
                 if (ftyp[s] == 0) {
                     const double nx = armN[s].x - xi.x * hx, ny = armN[s].y - xi.y * hx, nz = armN[s].z - xi.z * hx;
                     if (xs >= kVSmall) {
-                        gf = (double)recon_l<S, D>(pg, qg, fown[s], ox, oy, oz);
-                        hf = (double)recon_l<S, D>(ph, qh, fown[s], ox, oy, oz);
+                        const P2<S> r = recon_l<S, D>(pf, pq, fown[s], ox, oy, oz);
+                        gf = (double)r.g;
+                        hf = (double)r.h;
                     } else if (xs < -kVSmall) {
-                        gf = (double)recon_l<S, D>(pg, qg, fnei[s], nx, ny, nz);
-                        hf = (double)recon_l<S, D>(ph, qh, fnei[s], nx, ny, nz);
+                        const P2<S> r = recon_l<S, D>(pf, pq, fnei[s], nx, ny, nz);
+                        gf = (double)r.g;
+                        hf = (double)r.h;
                     } else {
-                        gf = 0.5 * ((double)recon_l<S, D>(pg, qg, fnei[s], nx, ny, nz) +
-                                    (double)recon_l<S, D>(pg, qg, fown[s], ox, oy, oz));
-                        hf = 0.5 * ((double)recon_l<S, D>(ph, qh, fnei[s], nx, ny, nz) +
-                                    (double)recon_l<S, D>(ph, qh, fown[s], ox, oy, oz));
+                        const P2<S> rn = recon_l<S, D>(pf, pq, fnei[s], nx, ny, nz);
+                        const P2<S> ro = recon_l<S, D>(pf, pq, fown[s], ox, oy, oz);
+                        gf = 0.5 * ((double)rn.g + (double)ro.g);
+                        hf = 0.5 * ((double)rn.h + (double)ro.h);
                     }
                 } else {
                     if (!(xs > 0)) continue;  // incoming half added from constants
-                    gf = (double)recon_l<S, D>(pg, qg, fown[s], ox, oy, oz);
-                    hf = (double)recon_l<S, D>(ph, qh, fown[s], ox, oy, oz);
+                    const P2<S> r = recon_l<S, D>(pf, pq, fown[s], ox, oy, oz);
+                    gf = (double)r.g;
+                    hf = (double)r.h;
                 }
                 if (ftyp[s] == kMaxwellWall) acc[s][NMA - 1] += w * xs * gf;
                 add_moments<D>(acc[s], w * gf, w * hf, xi);
@@ -527,14 +537,10 @@ __global__ void __launch_bounds__(kNTB, kWavesB)
     extern __shared__ __align__(16) unsigned char smem[];
     const BlockInfo bi = a.blk[blkid];
     const int ch0 = vg * a.cpg, ch1 = min(a.nchunk, ch0 + a.cpg);
-    double *flg = (double *)smem;                   // [VC][max_nfb]
-    double *flh = flg + VC * a.max_nfb;
-    S *fbg = (S *)(flh + VC * a.max_nfb);
-    S *fbh = fbg + VC * a.max_nl;
-    S *grg = fbh + VC * a.max_nl;
-    S *grh = grg + VC * a.max_ng * D;
-    S *ftg = grh + VC * a.max_ng * D;
-    S *fth = ftg + VC * a.max_nc;
+    P2<double> *fl = (P2<double> *)smem;            // [VC][max_nfb] fluxes sigma * f_f
+    P2<S> *fb = (P2<S> *)(fl + VC * a.max_nfb);   // [VC][max_nl]
+    P2<S> *gr = fb + VC * a.max_nl;                // [VC][max_ng][D]
+    P2<S> *ft = gr + VC * a.max_ng * D;            // [VC][max_nc] stored g~, h~ of the block
     const double dt = *dtp;
     const double hx = 0.5 * dt;
 
@@ -565,9 +571,9 @@ __global__ void __launch_bounds__(kNTB, kWavesB)
     int stripe = -1;
     for (int ch = ch0; ch < ch1; ++ch) {
         const Chunk ck = chunk_of(a, ch, VC, stripe);
-        stage_bar<S, VC, kNTB, CPT>(g, a, bi, gT, hT, dt, ck, cex, fbg, fbh, ftg, fth);
+        stage_bar<S, VC, kNTB, CPT>(g, a, bi, gT, hT, dt, ck, cex, fb, ft);
         __syncthreads();
-        stage_grad<S, D, VC, kNTB>(a, bi, ck.nvv, fbg, fbh, grg, grh);
+        stage_grad<S, D, VC, kNTB>(a, bi, ck.nvv, fb, gr);
         __syncthreads();
         // stage 3: face values -> flux sigma * f_f in LDS
         const d3 xr = g.xi[ck.vrow];
@@ -597,28 +603,30 @@ __global__ void __launch_bounds__(kNTB, kWavesB)
                 if (vv >= ck.nvv) break;
                 const d3 xi = g.xi[ck.vrow + ck.ix0 + vv];
                 const double xs = sigma_dot(xi, Sf);
-                const S *pg = fbg + vv * a.max_nl, *ph = fbh + vv * a.max_nl;
-                const S *qg = grg + vv * a.max_ng * D, *qh = grh + vv * a.max_ng * D;
+                const P2<S> *pf = fb + vv * a.max_nl, *pq = gr + vv * a.max_ng * D;
                 const double ox = aox - xi.x * hx, oy = aoy - xi.y * hx, oz = aoz - xi.z * hx;
                 double gv, hv;
                 bool relax = true;
                 if (b < 0) {
                     const double nx = anx - xi.x * hx, ny = any - xi.y * hx, nz = anz - xi.z * hx;
                     if (xs >= kVSmall) {
-                        gv = (double)recon_l<S, D>(pg, qg, lo, ox, oy, oz);
-                        hv = (double)recon_l<S, D>(ph, qh, lo, ox, oy, oz);
+                        const P2<S> r = recon_l<S, D>(pf, pq, lo, ox, oy, oz);
+                        gv = (double)r.g;
+                        hv = (double)r.h;
                     } else if (xs < -kVSmall) {
-                        gv = (double)recon_l<S, D>(pg, qg, ln, nx, ny, nz);
-                        hv = (double)recon_l<S, D>(ph, qh, ln, nx, ny, nz);
+                        const P2<S> r = recon_l<S, D>(pf, pq, ln, nx, ny, nz);
+                        gv = (double)r.g;
+                        hv = (double)r.h;
                     } else {
-                        gv = 0.5 * ((double)recon_l<S, D>(pg, qg, ln, nx, ny, nz) +
-                                    (double)recon_l<S, D>(pg, qg, lo, ox, oy, oz));
-                        hv = 0.5 * ((double)recon_l<S, D>(ph, qh, ln, nx, ny, nz) +
-                                    (double)recon_l<S, D>(ph, qh, lo, ox, oy, oz));
+                        const P2<S> rn = recon_l<S, D>(pf, pq, ln, nx, ny, nz);
+                        const P2<S> ro = recon_l<S, D>(pf, pq, lo, ox, oy, oz);
+                        gv = 0.5 * ((double)rn.g + (double)ro.g);
+                        hv = 0.5 * ((double)rn.h + (double)ro.h);
                     }
                 } else if (xs > 0) {
-                    gv = (double)recon_l<S, D>(pg, qg, lo, ox, oy, oz);
-                    hv = (double)recon_l<S, D>(ph, qh, lo, ox, oy, oz);
+                    const P2<S> r = recon_l<S, D>(pf, pq, lo, ox, oy, oz);
+                    gv = (double)r.g;
+                    hv = (double)r.h;
                 } else {
                     relax = false;
                     const BndCoef kb = a.bc[b];
@@ -638,8 +646,7 @@ __global__ void __launch_bounds__(kNTB, kWavesB)
                     gv = (1.0 - k.tr) * gv + k.tr * gEq;
                     hv = (1.0 - k.tr) * hv + k.tr * hEq;
                 }
-                flg[vv * a.max_nfb + j] = xs * gv;
-                flh[vv * a.max_nfb + j] = xs * hv;
+                fl[vv * a.max_nfb + j] = P2<double>{xs * gv, xs * hv};
             }
         }
         __syncthreads();
@@ -648,13 +655,15 @@ __global__ void __launch_bounds__(kNTB, kWavesB)
 #pragma unroll 1
             for (int vv = 0; vv < ck.nvv; ++vv) {
                 const int v = ck.vrow + ck.ix0 + vv;
-                double ag = -1.0 / 3 * (double)ftg[vv * a.max_nc + my] + 4.0 / 3 * (double)fbg[vv * a.max_nl + my];
-                double ah = -1.0 / 3 * (double)fth[vv * a.max_nc + my] + 4.0 / 3 * (double)fbh[vv * a.max_nl + my];
+                const P2<S> t = ft[vv * a.max_nc + my], bb = fb[vv * a.max_nl + my];
+                double ag = -1.0 / 3 * (double)t.g + 4.0 / 3 * (double)bb.g;
+                double ah = -1.0 / 3 * (double)t.h + 4.0 / 3 * (double)bb.h;
 #pragma unroll
                 for (int j = 0; j < kNb; ++j) {
                     if (j < nf) {
-                        const double tg = flg[vv * a.max_nfb + fidx[j]] * dtV;
-                        const double th = flh[vv * a.max_nfb + fidx[j]] * dtV;
+                        const P2<double> q = fl[vv * a.max_nfb + fidx[j]];
+                        const double tg = q.g * dtV;
+                        const double th = q.h * dtV;
                         if (fsg[j] > 0) {
                             ag += tg;
                             ah += th;
