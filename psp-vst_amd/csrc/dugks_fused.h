@@ -46,7 +46,7 @@ constexpr int fused_faces_per_thread_b(int D) { return D == 2 ? 3 : 4; }
 //   temperature that multiplies h, see q1_compat), q5 = q / (5 rho (R T)^2),
 //   tr = tau (cells) or the face relaxation weight h/(2 tau + h) (faces).
 struct alignas(16) EqCoef {
-    double A, ux, uy, uz, invRT, RTh, qx, qy, qz, tr, pad0, pad1;
+    double A, ux, uy, uz, invRT, RTh, qx, qy, qz, tr;
 };
 
 // Incoming half of wall / inlet faces: g = s * A * exp(-|U - xi|^2 invRT / 2), h = g * hfac,
@@ -99,8 +99,9 @@ struct FusedPlan {
     double *cref = nullptr;            // [nma] all-velocity moments of the storage reference
     EqCoef *cc = nullptr, *fc = nullptr;  // equilibrium coefficients of cells / faces
     BndCoef *bc = nullptr;                // incoming-half coefficients of boundary faces
-    // face values [Nv][slots]: written by the block that reconstructs them into its own slot
-    void *gF = nullptr, *hF = nullptr;
+    // face values [Nv][slots], (g, h) pairs: written by the block that reconstructs them into
+    // its own slot
+    void *F = nullptr;
     int n_slots = 0;
     std::vector<void *> allocs;
 };

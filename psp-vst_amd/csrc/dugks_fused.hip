@@ -120,7 +120,6 @@ __device__ inline EqCoef make_coef(const Geo &g, double rho, const d3 &U, double
     const double q5 = 1.0 / (5.0 * rho * RT * RT);
     k.qx = q.x * q5; k.qy = q.y * q5; k.qz = q.z * q5;
     k.tr = tr;
-    k.pad0 = k.pad1 = 0.0;
     return k;
 }
 
@@ -374,8 +373,8 @@ __device__ inline P2<S> recon_l(const P2<S> *fb, const P2<S> *gr, int l, double 
 // ------------------------------------------------------------------------- pass A
 template <class S, int D, int VC>
 __global__ void __launch_bounds__(kNTA)
-    k_pass_a(Geo g, FArgs a, const S *__restrict__ gT, const S *__restrict__ hT, S *__restrict__ gF,
-             S *__restrict__ hF, const double *dtp)
+    k_pass_a(Geo g, FArgs a, const S *__restrict__ gT, const S *__restrict__ hT, P2<S> *__restrict__ F,
+             const double *dtp)
 {
     constexpr int NMA = (D == 3 ? 17 : 11) + 1;
     constexpr int CPT = fused_cells_per_thread_a(D);
@@ -484,8 +483,7 @@ __global__ void __launch_bounds__(kNTA)
                 }
                 if (store) {  // face values are stored as deviations
                     const size_t kf = (size_t)v * a.n_slots + fpos[s];
-                    gF[kf] = (S)gf;
-                    hF[kf] = (S)hf;
+                    F[kf] = P2<S>{(S)gf, (S)hf};
                 }
                 if (ftyp[s] == kMaxwellWall) acc[s][NMA - 1] += w * xs * gf;
                 add_moments<D>(acc[s], w * gf, w * hf, xi);
@@ -576,7 +574,7 @@ __device__ inline double face_value(S stored, const double *ref, int v)
 
 template <class S, int D, int VC>
 __global__ void __launch_bounds__(kNTB)
-    k_pass_b(Geo g, FArgs a, S *gT, S *hT, const S *__restrict__ gF, const S *__restrict__ hF, const double *rho_b,
+    k_pass_b(Geo g, FArgs a, S *gT, S *hT, const P2<S> *__restrict__ F, const double *rho_b,
              const double *dtp)
 {
     constexpr int NM = D == 3 ? 17 : 11;
@@ -649,15 +647,17 @@ __global__ void __launch_bounds__(kNTB)
                         const size_t kf = (size_t)v * a.n_slots + (mine ? own_slot : twin);
                         double gEq, hEq;
                         shakhov_poly(g, k, xi, fex[s][vv] * eyz, gEq, hEq);
-                        fg = xs * ((1.0 - k.tr) * face_value<S>(gF[kf], g.ref_g, v) + k.tr * gEq);
-                        fh = xs * ((1.0 - k.tr) * face_value<S>(hF[kf], g.ref_h, v) + k.tr * hEq);
+                        const P2<S> fv = F[kf];
+                        fg = xs * ((1.0 - k.tr) * face_value<S>(fv.g, g.ref_g, v) + k.tr * gEq);
+                        fh = xs * ((1.0 - k.tr) * face_value<S>(fv.h, g.ref_h, v) + k.tr * hEq);
                     }
                 } else if (xs > 0) {
                     const size_t kf = (size_t)v * a.n_slots + own_slot;
                     double gEq, hEq;
                     shakhov_poly(g, k, xi, fex[s][vv] * eyz, gEq, hEq);
-                    fg = xs * ((1.0 - k.tr) * face_value<S>(gF[kf], g.ref_g, v) + k.tr * gEq);
-                    fh = xs * ((1.0 - k.tr) * face_value<S>(hF[kf], g.ref_h, v) + k.tr * hEq);
+                    const P2<S> fv = F[kf];
+                    fg = xs * ((1.0 - k.tr) * face_value<S>(fv.g, g.ref_g, v) + k.tr * gEq);
+                    fh = xs * ((1.0 - k.tr) * face_value<S>(fv.h, g.ref_h, v) + k.tr * hEq);
                 } else {
                     const BndCoef kb = a.bc[b];
                     const double scale = g.bf_type[b] == kMaxwellWall ? rho_b[b] : 1.0;
@@ -830,8 +830,7 @@ template <class S, int D, int VC>
 void launch_a(const Geo &g, FusedPlan &p, const S *gT, const S *hT, const double *dt, hipStream_t st)
 {
     const size_t lds = lds_pass_a<S>(p, D, VC);
-    hipLaunchKernelGGL((k_pass_a<S, D, VC>), dim3(grid_a(p)), dim3(kNTA), lds, st, g, fargs(p), gT, hT, (S *)p.gF,
-                       (S *)p.hF, dt);
+    hipLaunchKernelGGL((k_pass_a<S, D, VC>), dim3(grid_a(p)), dim3(kNTA), lds, st, g, fargs(p), gT, hT, (P2<S> *)p.F, dt);
 }
 
 template <class S, int D, int VC>
@@ -839,7 +838,7 @@ void launch_b(const Geo &g, FusedPlan &p, S *gT, S *hT, const double *rho_b, con
 {
     const size_t lds = lds_pass_b(p, VC);
     hipLaunchKernelGGL((k_pass_b<S, D, VC>), dim3(grid_b(p)), dim3(kNTB), lds, st, g, fargs_b(p), gT, hT,
-                       (const S *)p.gF, (const S *)p.hF, rho_b, dt);
+                       (const P2<S> *)p.F, rho_b, dt);
 }
 
 template <class S>
@@ -958,7 +957,7 @@ bool fused_plan_upload(FusedPlan &p, const Geo &g, size_t es, std::string &err)
         !dbuf(p, &bi, (size_t)g.nB * p.nm * 8, err) || !dbuf(p, &br, (size_t)g.nB * p.nma * 8, err) ||
         !dbuf(p, &cr, (size_t)p.nma * 8, err) || !dbuf(p, &cc, (size_t)g.nC * sizeof(EqCoef), err) ||
         !dbuf(p, &fc, (size_t)g.nF * sizeof(EqCoef), err) || !dbuf(p, &bc, (size_t)g.nB * sizeof(BndCoef), err) ||
-        !dbuf(p, &p.gF, (size_t)g.Nv * p.n_slots * es, err) || !dbuf(p, &p.hF, (size_t)g.Nv * p.n_slots * es, err))
+        !dbuf(p, &p.F, (size_t)g.Nv * p.n_slots * 2 * es, err))
         return false;
     p.fpart = (double *)fp;
     p.cpart = (double *)cp;

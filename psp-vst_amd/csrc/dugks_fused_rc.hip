@@ -117,7 +117,6 @@ __device__ inline EqCoef make_coef(const Geo &g, double rho, const d3 &U, double
     const double q5 = 1.0 / (5.0 * rho * RT * RT);
     k.qx = q.x * q5; k.qy = q.y * q5; k.qz = q.z * q5;
     k.tr = tr;
-    k.pad0 = k.pad1 = 0.0;
     return k;
 }
 
