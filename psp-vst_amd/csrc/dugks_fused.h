@@ -33,7 +33,7 @@ constexpr int kFusedThreadsB = 256;   // pass B threads = max cells per block
 constexpr int kFusedMaxNb = 8;        // max internal faces per gradient cell / faces per block cell
 constexpr int kFusedXcds = 8;         // accelerator dies; work-group ids are dealt round-robin over them
 #ifndef DUGKS_PASS_B_VC
-#define DUGKS_PASS_B_VC 2
+#define DUGKS_PASS_B_VC 4
 #endif
 constexpr int kFusedVcB = DUGKS_PASS_B_VC;  // max velocities per pass-B chunk (register budget)
 // local cells (block + halo ring) per pass-A thread in stage 1, face slots per pass-B thread

@@ -389,7 +389,9 @@ __global__ void __launch_bounds__(kNTA)
     const double hx = 0.5 * dt;
 
     double acc[kSlotsA][NMA];
-    int fid[kSlotsA], fpos[kSlotsA], flo[kSlotsA], fln[kSlotsA], fflag[kSlotsA], ftyp[kSlotsA];
+    // fmeta: (twin slot + 1) << 2 | slot flags; the twin is the adjacent block's slot of a
+    // shared face (-1: none)
+    int fid[kSlotsA], fmeta[kSlotsA], flo[kSlotsA], fln[kSlotsA], ftyp[kSlotsA];
     d3 fS[kSlotsA], armO[kSlotsA], armN[kSlotsA];
 #pragma unroll
     for (int s = 0; s < kSlotsA; ++s) {
@@ -397,15 +399,14 @@ __global__ void __launch_bounds__(kNTA)
         for (int k = 0; k < NMA; ++k) acc[s][k] = 0.0;
         const int j = threadIdx.x + s * kNTA;
         fid[s] = -1;
-        fpos[s] = flo[s] = fln[s] = fflag[s] = ftyp[s] = 0;
+        flo[s] = fln[s] = fmeta[s] = ftyp[s] = 0;
         fS[s] = armO[s] = armN[s] = d3{0, 0, 0};
         if (j < bi.nfs) {
             const int f = a.fs_face[bi.fs_off + j];
             fid[s] = f;
-            fpos[s] = bi.fs_off + j;   // the block's own slot of the face
             flo[s] = a.fs_lo[bi.fs_off + j];
             fln[s] = a.fs_ln[bi.fs_off + j];
-            fflag[s] = a.fs_flag[bi.fs_off + j];
+            fmeta[s] = ((a.fs_twin[bi.fs_off + j] + 1) << 2) | a.fs_flag[bi.fs_off + j];
             ftyp[s] = f < g.nIF ? 0 : g.bf_type[f - g.nIF];
             fS[s] = g.Sf[f];
             const d3 Cf = g.Cf[f];
@@ -431,7 +432,7 @@ __global__ void __launch_bounds__(kNTA)
 #pragma unroll
         for (int s = 0; s < kSlotsA; ++s) {
             if (fid[s] < 0) continue;
-            const bool own_in = fflag[s] & kSlotOwnIn, nei_in = fflag[s] & kSlotNeiIn;
+            const bool own_in = fmeta[s] & kSlotOwnIn, nei_in = fmeta[s] & kSlotNeiIn;
 #pragma unroll 1
             for (int vv = 0; vv < ck.nvv; ++vv) {
                 const int v = ck.vrow + ck.ix0 + vv;
@@ -482,8 +483,12 @@ __global__ void __launch_bounds__(kNTA)
                     hf = (double)r.h;
                 }
                 if (store) {  // face values are stored as deviations
-                    const size_t kf = (size_t)v * a.n_slots + fpos[s];
-                    F[kf] = P2<S>{(S)gf, (S)hf};
+                    // into this block's slot and, for a face shared with an adjacent block,
+                    // into that block's slot too: pass B then reads its own slots only
+                    const P2<S> val{(S)gf, (S)hf};
+                    F[(size_t)v * a.n_slots + (bi.fs_off + threadIdx.x + s * kNTA)] = val;  // own slot
+                    const int twin = (fmeta[s] >> 2) - 1;
+                    if (twin >= 0) F[(size_t)v * a.n_slots + twin] = val;
                 }
                 if (ftyp[s] == kMaxwellWall) acc[s][NMA - 1] += w * xs * gf;
                 add_moments<D>(acc[s], w * gf, w * hf, xi);
@@ -494,7 +499,7 @@ __global__ void __launch_bounds__(kNTA)
 #pragma unroll
     for (int s = 0; s < kSlotsA; ++s) {
         if (fid[s] < 0) continue;
-        const int side = (fflag[s] & kSlotOwnIn) ? 0 : 1;
+        const int side = (fmeta[s] & kSlotOwnIn) ? 0 : 1;
         double *o = a.fpart + (((size_t)vg * g.nF + fid[s]) * 2 + side) * NMA;
 #pragma unroll
         for (int k = 0; k < NMA; ++k) o[k] = acc[s][k];
@@ -625,10 +630,8 @@ __global__ void __launch_bounds__(kNTB)
             const int j = threadIdx.x + s * kNTB;
             if (j >= bi.nfs) break;
             const int f = a.fs_face[bi.fs_off + j];
-            // the value sits in the slot of the block holding the upwind cell: this block's
-            // slot, or the twin slot of the adjacent block
-            const int own_slot = bi.fs_off + j, twin = a.fs_twin[own_slot];
-            const int flag = a.fs_flag[own_slot];
+            // pass A wrote the upwind value into the slots of both blocks of a shared face
+            const int own_slot = bi.fs_off + j;
             const d3 Sf = g.Sf[f];
             const EqCoef k = a.fc[f];
             if (ck.new_stripe) stripe_factors<VC>(g, k, ck.ix0, a.n1, fex[s]);
@@ -641,24 +644,14 @@ __global__ void __launch_bounds__(kNTB)
                 const d3 xi = g.xi[v];
                 const double xs = sigma_dot(xi, Sf);
                 double fg = 0.0, fh = 0.0;
-                if (b < 0) {
-                    if (xs >= kVSmall || xs < -kVSmall) {
-                        const bool mine = xs >= kVSmall ? (flag & kSlotOwnIn) : (flag & kSlotNeiIn);
-                        const size_t kf = (size_t)v * a.n_slots + (mine ? own_slot : twin);
-                        double gEq, hEq;
-                        shakhov_poly(g, k, xi, fex[s][vv] * eyz, gEq, hEq);
-                        const P2<S> fv = F[kf];
-                        fg = xs * ((1.0 - k.tr) * face_value<S>(fv.g, g.ref_g, v) + k.tr * gEq);
-                        fh = xs * ((1.0 - k.tr) * face_value<S>(fv.h, g.ref_h, v) + k.tr * hEq);
-                    }
-                } else if (xs > 0) {
+                if (b < 0 ? (xs >= kVSmall || xs < -kVSmall) : xs > 0) {
                     const size_t kf = (size_t)v * a.n_slots + own_slot;
                     double gEq, hEq;
                     shakhov_poly(g, k, xi, fex[s][vv] * eyz, gEq, hEq);
                     const P2<S> fv = F[kf];
                     fg = xs * ((1.0 - k.tr) * face_value<S>(fv.g, g.ref_g, v) + k.tr * gEq);
                     fh = xs * ((1.0 - k.tr) * face_value<S>(fv.h, g.ref_h, v) + k.tr * hEq);
-                } else {
+                } else if (b >= 0) {  // incoming half of a wall or inlet face
                     const BndCoef kb = a.bc[b];
                     const double scale = g.bf_type[b] == kMaxwellWall ? rho_b[b] : 1.0;
                     const double cx = xi.x - kb.ux, cy = xi.y - kb.uy, cz = xi.z - kb.uz;
