@@ -91,7 +91,10 @@ typedef struct dugks_options {
                                   internal-face Tsurf at the patch-local index, as
                                   fvDVM.C:1367,1413 does; 0: the patch temperature */
     int device;                /* HIP device ordinal */
-    int fused;                 /* 1: fused two-pass path (error if unavailable);
+    int fused;                 /* 1: fused two-pass path (error if unavailable), its
+                                     schedule chosen by dimension and precision;
+                                  2: fused, stored-face-value schedule required;
+                                  3: fused, recompute schedule required;
                                   0: staged path with materialised face distributions;
                                   -1: fused when available, else staged */
 } dugks_options;
@@ -176,7 +179,8 @@ int dugks_num_velocities(const dugks_handle *h);
 int dugks_velocity_set(const dugks_handle *h, double *xi, double *w);  /* [Nv][3], [Nv] */
 
 /* Diagnostics (no reference counterpart): the path a handle runs -- 0 staged, 1 fused with
- * stored face values (3-D meshes), 2 fused recomputing two halo rings (2-D meshes) -- and,
+ * stored face values (3-D meshes, FP64 storage), 2 fused recomputing two halo rings (2-D
+ * meshes with FP32 storage) -- and,
  * for the fused paths, the largest cell block, velocities per LDS chunk and velocity
  * groups per block.  Any pointer may be NULL. */
 int dugks_path_info(const dugks_handle *h, int *fused, int *block_cells, int *chunk_velocities,

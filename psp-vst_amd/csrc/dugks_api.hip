@@ -38,9 +38,11 @@ __global__ void k_set_scalar(double *p, double v) { *p = v; }
 // Fused schedules.  Face values (dugks_fused.h): one halo ring, face values stored once
 // per velocity by the upwind block, in-place update.  Recompute (dugks_fused_rc.h): two
 // halo rings rebuilt by both passes, no face values.  Recomputing the rings costs
-// (ring cells / block cells) equilibria per update: about 1.5x in 2-D, where it beats
-// storing face values (measured 15.9 vs 18.3 ms per step on the 256^2 cavity), and about
-// 5x in 3-D, where storing face values wins (356 vs 506 ms on the 211k-cell sphere).
+// (ring cells / block cells) equilibria per update: about 1.5x in 2-D and 5x in 3-D.
+// Measured per step: 256^2 cavity with 64^2 velocities, FP32 storage 15.3 ms recompute vs
+// 16.1 ms face values, FP64 storage 21.6 vs 17.4 ms (the recompute schedule's FP64 LDS
+// tiles hold half the velocities); 211k-cell sphere 356 vs 506 ms (face values) before
+// later tuning.  Face values are used in 3-D and for FP64 storage, recompute otherwise.
 enum Schedule { kSchedFaceValues = 1, kSchedRecompute = 2 };
 
 enum Phase {
@@ -251,6 +253,7 @@ int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patch
     o.fused = -1;
     if (opt) o = *opt;
     if (o.precision != DUGKS_FP64 && o.precision != DUGKS_MIXED) return fail("dugks_create: unknown precision");
+    if (o.fused < -1 || o.fused > 3) return fail("dugks_create: options.fused must be -1, 0, 1, 2 or 3");
 
     const int nF = nIF + nB;
     long long nv_ll = 1;
@@ -297,7 +300,10 @@ int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patch
     // internal numbering) of the largest size whose halo fits the chosen fused schedule's
     // work-group shapes (2-D meshes: 256 cells).  Faces keep the caller's numbering; every
     // cell-indexed array crossing the C-ABI is permuted on the way in and out.
-    const int sched = D == 2 ? kSchedRecompute : kSchedFaceValues;
+    const int sched = o.fused == 2   ? kSchedFaceValues
+                      : o.fused == 3 ? kSchedRecompute
+                      : (D == 2 && o.precision == DUGKS_MIXED) ? kSchedRecompute
+                                                                : kSchedFaceValues;
     std::vector<int> perm, block_start, inv(nC);
     for (int max_block : {256, 192, 128, 96, 64}) {
         rcb_partition(nC, mesh->C, max_block, perm, block_start);
@@ -446,7 +452,7 @@ int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patch
                             : fused_plan_build(h->plan, g, prep, block_start, h->es, 0, err) &&
                                   fused_plan_upload(h->plan, g, h->es, err);
         if (!ok) {
-            if (o.fused == 1) return bail(fail("dugks_create: fused path unavailable: " + err));
+            if (o.fused >= 1) return bail(fail("dugks_create: fused path unavailable: " + err));
             fused_plan_free(h->plan);
             rc::fused_plan_free(h->plan_rc);
             h->fused = false;

@@ -116,7 +116,11 @@ class DugksError(RuntimeError):
 
 
 class FvDVM:
-    """One DUGKS solver on one GPU, built from a :class:`dugks_amd.cases.Case`."""
+    """One DUGKS solver on one GPU, built from a :class:`dugks_amd.cases.Case`.
+
+    ``fused`` is ``dugks_options.fused``: -1 fused when available, else staged; 0 staged;
+    1 fused with the default schedule; 2 fused with stored face values; 3 fused recomputing
+    the halo rings."""
 
     def __init__(self, case, precision: int = FP64, fused: int = -1, device: int = 0, q1_compat=None,
                  ls_vectors=None):

@@ -6,6 +6,10 @@ Tolerances (relative to the field's max magnitude, see conftest.field_rel_err):
     requirement names).
   * mixed path (FP32 g/h storage) vs oracle: 1e-5 in rho, U, T (the stated mixed budget);
     q is reported and bounded at 1e-3 of its scale.
+
+`fused` values follow dugks_options.fused: 0 staged, 1 fused with the default schedule
+(2-D: recompute for FP32 storage, stored face values for FP64 storage; 3-D: stored face
+values), 2 stored-face-value schedule, 3 recompute schedule.
 """
 import numpy as np
 import pytest
@@ -38,7 +42,7 @@ def _errs(o, g):
     return {k: field_rel_err(mg[k], mo[k]) for k in ("rho", "U", "T", "q")}
 
 
-@pytest.mark.parametrize("fused", [0, 1])
+@pytest.mark.parametrize("fused", [0, 1, 2, 3])
 def test_tiny_cavity_fp64_matches_oracle(fused):
     c = _cases().cavity_case(6, "NC38", 10, 4.0, 0.1)
     o, g, ho, hg = _run_pair(c, 30, 0, fused)
@@ -57,7 +61,7 @@ def test_tiny_cavity_fp64_matches_oracle(fused):
     assert field_rel_err(hg_, ho_) < 1e-12
 
 
-@pytest.mark.parametrize("fused", [0, 1])
+@pytest.mark.parametrize("fused", [0, 2, 3])
 def test_tiny_cavity_odd_grid_zero_normal_velocity(fused):
     """Odd Newton-Cotes grid: xi_d = 0 exists, so xi.Sf = 0 on axis-aligned faces and the
     averaging branch of the internal reconstruction (fvDVM.C:847-863) is exercised."""
@@ -69,7 +73,7 @@ def test_tiny_cavity_odd_grid_zero_normal_velocity(fused):
         assert v < 1e-12, (k, v)
 
 
-@pytest.mark.parametrize("fused", [0, 1])
+@pytest.mark.parametrize("fused", [0, 2, 3])
 def test_config0_fp64_1000_steps(fused):
     c = _cases().config0()
     o, g, ho, hg = _run_pair(c, 1000, 0, fused)
@@ -80,7 +84,7 @@ def test_config0_fp64_1000_steps(fused):
         assert v < 1e-10, (k, v)
 
 
-@pytest.mark.parametrize("fused", [0, 1])
+@pytest.mark.parametrize("fused", [0, 2, 3])
 def test_config0_mixed_200_steps(fused):
     c = _cases().config0()
     o, g, ho, hg = _run_pair(c, 200, 1, fused)
@@ -91,7 +95,7 @@ def test_config0_mixed_200_steps(fused):
     assert e["q"] < 1e-3
 
 
-@pytest.mark.parametrize("fused", [0, 1])
+@pytest.mark.parametrize("fused", [0, 2, 3])
 def test_determinism_and_checkpoint_resume(tmp_path, fused):
     from dugks_amd import hip
     c = _cases().cavity_case(20, "NC38", 10, 4.0, 0.1)
@@ -114,13 +118,14 @@ def test_determinism_and_checkpoint_resume(tmp_path, fused):
         assert np.array_equal(ma[k], md[k]), k
 
 
-def test_fused_and_staged_agree_on_multiblock_mesh():
+@pytest.mark.parametrize("fused", [2, 3])
+def test_fused_and_staged_agree_on_multiblock_mesh(fused):
     """A 48x48 cavity gives nine RCB blocks, so block halos, ring-2 cells and faces shared
-    between blocks are all exercised; FP64 fused vs staged after 50 steps."""
+    between blocks (twin face slots) are all exercised; FP64 fused vs staged after 50 steps."""
     from dugks_amd import hip
     c = _cases().cavity_case(48, "NC38", 13, 4.0, 0.05)
     a = hip.FvDVM(c, fused=0)
-    b = hip.FvDVM(c, fused=1)
+    b = hip.FvDVM(c, fused=fused)
     a.run(50)
     b.run(50)
     ma, mb = a.cell_macros(), b.cell_macros()
@@ -131,15 +136,30 @@ def test_fused_and_staged_agree_on_multiblock_mesh():
         assert field_rel_err(fb[k], fa[k]) < 1e-12, k
 
 
-def test_fused_fp64_is_bitwise_deterministic():
+@pytest.mark.parametrize("fused", [2, 3])
+def test_fused_fp64_is_bitwise_deterministic(fused):
     from dugks_amd import hip
     c = _cases().cavity_case(40, "NC38", 16, 4.0, 0.1)
     runs = []
     for _ in range(2):
-        s = hip.FvDVM(c, fused=1)
+        s = hip.FvDVM(c, fused=fused)
         s.run(20)
         runs.append(s.df())
     assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+
+
+def test_default_schedule_choice():
+    """Default fused schedule: 2-D with FP32 storage recomputes the halo rings (path 2),
+    2-D with FP64 storage and 3-D store face values (path 1); options 2 and 3 force one."""
+    from dugks_amd import hip
+    c2 = _cases().cavity_case(20, "NC38", 10, 4.0, 0.1)
+    c3 = _cases().sphere_case(SMALL_SPHERE, "GH", 6, 1.0, 0.1)
+    expect = [(c2, hip.MIXED, 1, 2), (c2, hip.FP64, 1, 1), (c3, hip.MIXED, 1, 1), (c3, hip.FP64, 1, 1),
+              (c2, hip.MIXED, 2, 1), (c2, hip.FP64, 3, 2)]
+    for case, prec, fused, path in expect:
+        s = hip.FvDVM(case, precision=prec, fused=fused)
+        assert s.path_info()["fused"] == path, (prec, fused)
+        s.close()
 
 
 SMALL_SPHERE = dict(nc=4, nr=2, n_up=2, n_down=3, n_side=2)
