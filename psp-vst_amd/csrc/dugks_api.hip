@@ -297,21 +297,29 @@ int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patch
         if (mesh->neighbour[f] < 0 || mesh->neighbour[f] >= nC) return fail("dugks_create: neighbour label out of range");
 
     // Internal cell numbering: recursive coordinate bisection into blocks (contiguous in the
-    // internal numbering) of the largest size whose halo fits the chosen fused schedule's
-    // work-group shapes (2-D meshes: 256 cells).  Faces keep the caller's numbering; every
-    // cell-indexed array crossing the C-ABI is permuted on the way in and out.
+    // internal numbering) whose halo fits the chosen fused schedule's work-group shapes.
+    // Faces keep the caller's numbering; every cell-indexed array crossing the C-ABI is
+    // permuted on the way in and out.
     const int sched = o.fused == 2   ? kSchedFaceValues
                       : o.fused == 3 ? kSchedRecompute
                       : (D == 2 && o.precision == DUGKS_MIXED) ? kSchedRecompute
                                                                 : kSchedFaceValues;
+    // Blocks of 256 cells (192 for 3-D meshes with FP64 storage, whose LDS tiles then hold
+    // twice the velocities per chunk: 311 vs 351 ms per step on the 211k-cell sphere, while
+    // FP32 storage gains from 256: 259 vs 279 ms); a block whose halo or face count exceeds
+    // the work-group shapes is bisected again (only that block), until every block fits or
+    // the rounds run out (the plan build then reports the misfit and the staged path takes
+    // over).
     std::vector<int> perm, block_start, inv(nC);
-    for (int max_block : {256, 192, 128, 96, 64}) {
-        rcb_partition(nC, mesh->C, max_block, perm, block_start);
-        if (o.fused == 0) break;
-        const bool fit = sched == kSchedRecompute
-                             ? rc::block_tables_fit(nC, nIF, nF, mesh->owner, mesh->neighbour, D, perm, block_start)
-                             : block_tables_fit(nC, nIF, nF, mesh->owner, mesh->neighbour, D, perm, block_start);
+    rcb_partition(nC, mesh->C, (D == 3 && o.precision == DUGKS_FP64) ? 192 : 256, perm, block_start);
+    for (int round = 0; o.fused != 0 && round < 8; ++round) {
+        std::vector<int> bad;
+        const bool fit =
+            sched == kSchedRecompute
+                ? rc::block_tables_fit(nC, nIF, nF, mesh->owner, mesh->neighbour, D, perm, block_start, &bad)
+                : block_tables_fit(nC, nIF, nF, mesh->owner, mesh->neighbour, D, perm, block_start, &bad);
         if (fit) break;
+        rcb_refine(mesh->C, bad, perm, block_start);
     }
     for (int i = 0; i < nC; ++i) inv[perm[i]] = i;
     std::vector<int> own_p(nF), nei_p(nIF);

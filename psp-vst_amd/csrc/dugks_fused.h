@@ -109,10 +109,16 @@ struct FusedPlan {
 // Recursive coordinate bisection of the cells into blocks of at most max_block cells.
 // Returns the permutation (new -> old) and the block starts in the new numbering.
 void rcb_partition(int nC, const double *C, int max_block, std::vector<int> &perm, std::vector<int> &block_start);
+// Bisects each listed block (indices into block_start) once more, in place: the cells of a
+// block stay a contiguous range of perm and the block is replaced by its two halves.
+void rcb_refine(const double *C, const std::vector<int> &blocks, std::vector<int> &perm,
+                std::vector<int> &block_start);
 // True when every block with its halo ring fits the work-group shapes of the passes
-// (owner / neighbour in the caller's numbering, perm new -> old).
+// (owner / neighbour in the caller's numbering, perm new -> old).  With `bad`, every block
+// that does not fit is listed there.
 bool block_tables_fit(int nC, int nIF, int nF, const int *owner, const int *neighbour, int D,
-                      const std::vector<int> &perm, const std::vector<int> &block_start);
+                      const std::vector<int> &perm, const std::vector<int> &block_start,
+                      std::vector<int> *bad = nullptr);
 
 bool fused_plan_build(FusedPlan &p, const Geo &g, const HostPrep &prep, const std::vector<int> &block_start,
                       size_t es, int nvg_hint, std::string &err);

@@ -69,9 +69,27 @@ void rcb_partition(int nC, const double *C, int max_block, std::vector<int> &per
     block_start.push_back(nC);
 }
 
-bool block_tables_fit(int nC, int nIF, int nF, const int *owner, const int *neighbour, int D,
-                      const std::vector<int> &perm, const std::vector<int> &block_start)
+void rcb_refine(const double *C, const std::vector<int> &blocks, std::vector<int> &perm,
+                std::vector<int> &block_start)
 {
+    std::vector<char> split(block_start.size(), 0);
+    for (int b : blocks) split[b] = 1;
+    std::vector<int> starts;
+    for (size_t b = 0; b + 1 < block_start.size(); ++b) {
+        const int lo = block_start[b], hi = block_start[b + 1];
+        if (split[b] && hi - lo > 1)
+            rcb(perm, lo, hi, C, (hi - lo + 1) / 2, starts);
+        else
+            starts.push_back(lo);
+    }
+    starts.push_back(block_start.back());
+    block_start.swap(starts);
+}
+
+bool block_tables_fit(int nC, int nIF, int nF, const int *owner, const int *neighbour, int D,
+                      const std::vector<int> &perm, const std::vector<int> &block_start, std::vector<int> *bad)
+{
+    if (bad) bad->clear();
     // cell -> internal-face neighbours and cell -> faces, in the caller's numbering
     std::vector<int> deg(nC + 1, 0), fdeg(nC + 1, 0);
     for (int f = 0; f < nIF; ++f) {
@@ -99,7 +117,11 @@ bool block_tables_fit(int nC, int nIF, int nF, const int *owner, const int *neig
     const int nblk = (int)block_start.size() - 1;
     for (int b = 0; b < nblk; ++b) {
         const int c0 = block_start[b], c1 = block_start[b + 1];
-        if (c1 - c0 > kFusedThreadsB) return false;
+        if (c1 - c0 > kFusedThreadsB) {
+            if (!bad) return false;
+            bad->push_back(b);
+            continue;
+        }
         int nl = c1 - c0, nfs = 0;
         for (int i = c0; i < c1; ++i) mark[perm[i]] = b;
         for (int i = c0; i < c1; ++i) {
@@ -115,9 +137,12 @@ bool block_tables_fit(int nC, int nIF, int nF, const int *owner, const int *neig
                     ++nl;
                 }
         }
-        if (nl > max_local_cells(D) || nfs > max_face_slots(D)) return false;
+        if (nl > max_local_cells(D) || nfs > max_face_slots(D)) {
+            if (!bad) return false;
+            bad->push_back(b);
+        }
     }
-    return true;
+    return !bad || bad->empty();
 }
 
 bool fused_plan_build(FusedPlan &p, const Geo &g, const HostPrep &prep, const std::vector<int> &block_start,

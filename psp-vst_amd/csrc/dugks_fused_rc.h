@@ -99,9 +99,11 @@ struct FusedPlan {
 };
 
 // True when every block with its two halo rings fits the work-group shapes of the passes
-// (owner / neighbour in the caller's numbering, perm new -> old).
+// (owner / neighbour in the caller's numbering, perm new -> old).  With `bad`, every block
+// that does not fit is listed there.
 bool block_tables_fit(int nC, int nIF, int nF, const int *owner, const int *neighbour, int D,
-                      const std::vector<int> &perm, const std::vector<int> &block_start);
+                      const std::vector<int> &perm, const std::vector<int> &block_start,
+                      std::vector<int> *bad = nullptr);
 
 bool fused_plan_build(FusedPlan &p, const Geo &g, const HostPrep &prep, const std::vector<int> &block_start,
                       size_t es, int nvg_hint, std::string &err);

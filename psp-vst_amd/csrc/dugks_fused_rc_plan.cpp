@@ -14,8 +14,9 @@ namespace dugks {
 namespace rc {
 
 bool block_tables_fit(int nC, int nIF, int nF, const int *owner, const int *neighbour, int D,
-                      const std::vector<int> &perm, const std::vector<int> &block_start)
+                      const std::vector<int> &perm, const std::vector<int> &block_start, std::vector<int> *bad)
 {
+    if (bad) bad->clear();
     // cell -> internal-face neighbours and cell -> faces, in the caller's numbering
     std::vector<int> deg(nC + 1, 0), fdeg(nC + 1, 0);
     for (int f = 0; f < nIF; ++f) {
@@ -45,7 +46,11 @@ bool block_tables_fit(int nC, int nIF, int nF, const int *owner, const int *neig
     const int nblk = (int)block_start.size() - 1;
     for (int b = 0; b < nblk; ++b) {
         std::vector<int> ring(perm.begin() + block_start[b], perm.begin() + block_start[b + 1]);
-        if ((int)ring.size() > kFusedThreadsB) return false;
+        if ((int)ring.size() > kFusedThreadsB) {
+            if (!bad) return false;
+            bad->push_back(b);
+            continue;
+        }
         int nl = (int)ring.size(), nfb = 0;
         for (int c : ring) {
             mark[c] = b;
@@ -66,9 +71,12 @@ bool block_tables_fit(int nC, int nIF, int nF, const int *owner, const int *neig
             nl += (int)next.size();
             ring.swap(next);
         }
-        if (nl > max_nl || nfb > max_nfb) return false;
+        if (nl > max_nl || nfb > max_nfb) {
+            if (!bad) return false;
+            bad->push_back(b);
+        }
     }
-    return true;
+    return !bad || bad->empty();
 }
 
 bool fused_plan_build(FusedPlan &p, const Geo &g, const HostPrep &prep, const std::vector<int> &block_start,
