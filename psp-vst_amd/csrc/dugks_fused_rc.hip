@@ -194,6 +194,64 @@ __device__ inline void add_moments(double *m, double wg, double wh, const d3 &xi
     }
 }
 
+// Power sums over the velocities of one chunk (one x-stripe of a velocity row: xi.y and
+// xi.z are shared): g0..g3 = sum w g xi_x^k, h0, h1 = sum w h xi_x^k.  fold_row adds the raw
+// moments they stand for (the order of add_moments) once per chunk, so a face-velocity pair
+// costs 10 operations instead of the 20-30 of add_moments.
+struct RowSums {
+    double g0, g1, g2, g3, h0, h1;
+};
+
+__device__ inline void row_add(RowSums &r, double wg, double wh, double x)
+{
+    r.g0 += wg;
+    const double t1 = wg * x;
+    r.g1 += t1;
+    const double t2 = t1 * x;
+    r.g2 += t2;
+    r.g3 += t2 * x;
+    r.h0 += wh;
+    r.h1 += wh * x;
+}
+
+template <int D>
+__device__ inline void fold_row(double *m, const RowSums &r, double y, double z)
+{
+    if (D == 2) {
+        const double yy = y * y;
+        m[0] += r.g0;
+        m[1] += r.g1;
+        m[2] += y * r.g0;
+        m[3] += r.g2;
+        m[4] += y * r.g1;
+        m[5] += yy * r.g0;
+        m[6] += r.g3 + yy * r.g1;          // sum w g (x^2 + y^2) x
+        m[7] += y * r.g2 + yy * y * r.g0;  // sum w g (x^2 + y^2) y
+        m[8] += r.h0;
+        m[9] += r.h1;
+        m[10] += y * r.h0;
+    } else {
+        const double yy = y * y, zz = z * z, r2 = yy + zz;
+        m[0] += r.g0;
+        m[1] += r.g1;
+        m[2] += y * r.g0;
+        m[3] += z * r.g0;
+        m[4] += r.g2;
+        m[5] += y * r.g1;
+        m[6] += z * r.g1;
+        m[7] += yy * r.g0;
+        m[8] += (y * z) * r.g0;
+        m[9] += zz * r.g0;
+        m[10] += r.g3 + r2 * r.g1;         // sum w g |xi|^2 x
+        m[11] += y * r.g2 + (y * r2) * r.g0;
+        m[12] += z * r.g2 + (z * r2) * r.g0;
+        m[13] += r.h0;
+        m[14] += r.h1;
+        m[15] += y * r.h0;
+        m[16] += z * r.h0;
+    }
+}
+
 // rho, U, T, tau, q from raw moments (the conversions of fvDVM.C:1249-1253, 1683-1696;
 // q = 1/2 sum w c (|c|^2 g + h) with c = xi - U expanded in raw moments), then the
 // 2 tau / (2 tau + h Pr) correction with h = corr_dt.
@@ -413,9 +471,11 @@ __global__ void __launch_bounds__(kNTA, kWavesA)
         __syncthreads();
         stage_grad<S, D, VC, kNTA>(a, bi, ck.nvv, fb, gr);
         __syncthreads();
+        const d3 xr = g.xi[ck.vrow];
 #pragma unroll
         for (int s = 0; s < kSlotsA; ++s) {
             if (fid[s] < 0) continue;
+            RowSums rs{};
 #pragma unroll 1
             for (int vv = 0; vv < ck.nvv; ++vv) {
                 const int v = ck.vrow + ck.ix0 + vv;
@@ -448,8 +508,9 @@ __global__ void __launch_bounds__(kNTA, kWavesA)
                     hf = (double)r.h;
                 }
                 if (ftyp[s] == kMaxwellWall) acc[s][NMA - 1] += w * xs * gf;
-                add_moments<D>(acc[s], w * gf, w * hf, xi);
+                row_add(rs, w * gf, w * hf, xi.x);
             }
+            fold_row<D>(acc[s], rs, xr.y, xr.z);
         }
         __syncthreads();
     }
