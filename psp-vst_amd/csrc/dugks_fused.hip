@@ -726,6 +726,7 @@ __global__ void __launch_bounds__(kNTB)
         __syncthreads();
         // stage 3: cell update in ascending face order, in place
         if (has_cell) {
+            RowSums rs{};
 #pragma unroll
             for (int vv = 0; vv < VC; ++vv) {
                 if (vv >= ck.nvv) break;
@@ -752,8 +753,9 @@ __global__ void __launch_bounds__(kNTB)
                 gT[kk] = sg;
                 hT[kk] = sh;
                 const double w = g.w[v];
-                add_moments<D>(mom, w * (double)sg, w * (double)sh, g.xi[v]);  // deviations
+                row_add(rs, w * (double)sg, w * (double)sh, g.xi[v].x);  // deviations
             }
+            fold_row<D>(mom, rs, xr.y, xr.z);
         }
         __syncthreads();
     }
