@@ -340,7 +340,8 @@ __global__ void __launch_bounds__(kB)
     for (int v = 0; v < g.Nv; ++v) {
         const d3 xi = g.xi[v];
         const size_t k = (size_t)v * g.nF + f;
-        const double gr = (double)gR[k], hr = (double)hR[k];
+        // recorded face values are deviations from the storage reference
+        const double gr = (double)gR[k] + g.ref_g[v], hr = (double)hR[k] + g.ref_h[v];
         const double w = g.w[v];
         add_mom<D>(m, w * gr, w * hr, xi);
         if (bnd && !(sigma_dot(xi, Sf) > 0)) continue;
@@ -569,7 +570,7 @@ __global__ void __launch_bounds__(kB)
         double s = 0.0;
         for (int v = 0; v < g.Nv; ++v) {
             const d3 xi = g.xi[v];
-            s += g.w[v] * dot(xi, e) * sigma_dot(xi, Sf) * (double)gS[(size_t)v * g.nF + f];
+            s += g.w[v] * dot(xi, e) * sigma_dot(xi, Sf) * ((double)gS[(size_t)v * g.nF + f] + g.ref_g[v]);
         }
         j += s;
     }

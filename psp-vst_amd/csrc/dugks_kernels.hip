@@ -94,8 +94,8 @@ __global__ void __launch_bounds__(kBlock) k_init_mixed_faces(Geo g, S *gS, S *hS
     const d3 d{Ub.x - xi.x, Ub.y - xi.y, Ub.z - xi.z};
     const double geq = g.bf_rho[b] / pow(sqrt(2.0 * kPi * g.R * Tb), (double)g.D) * exp(-magsqr(d) / (2.0 * g.R * Tb));
     const size_t k = (size_t)v * g.nF + g.nIF + b;
-    gS[k] = (S)geq;
-    hS[k] = (S)((g.K + 3 - g.D) * g.R * Tb * geq);
+    gS[k] = (S)(geq - g.ref_g[v]);
+    hS[k] = (S)((g.K + 3 - g.D) * g.R * Tb * geq - g.ref_h[v]);
 }
 
 __global__ void __launch_bounds__(kBlock) k_init_incoming(Geo g, double *in_by_rho)
@@ -157,11 +157,15 @@ __global__ void __launch_bounds__(kBlock)
     shakhov(g, cm.rho[c], cm.U[c], T, cm.q[c], g.xi[v], T, gEq, hEq);
     const double a = 1.5 * dt / (2.0 * cm.tau[c] + dt);
     const size_t k = (size_t)v * g.nC + c;
-    gB[k] = (S)((1.0 - a) * ((double)gT[k] + g.ref_g[v]) + a * gEq);
-    hB[k] = (S)((1.0 - a) * ((double)hT[k] + g.ref_h[v]) + a * hEq);
+    // f-bar-plus kept as a deviation from the storage reference, like g~
+    gB[k] = (S)((1.0 - a) * (double)gT[k] + a * (gEq - g.ref_g[v]));
+    hB[k] = (S)((1.0 - a) * (double)hT[k] + a * (hEq - g.ref_h[v]));
 }
 
-// updateGrad + updateGHbarSurf (fvDVM.C:749-1029), outgoing boundary values included
+// updateGrad + updateGHbarSurf (fvDVM.C:749-1029), outgoing boundary values included.
+// f-bar-plus and the face values are deviations from the storage reference, which is the
+// same in every cell, so the reconstruction of the deviation is the deviation of the
+// reconstruction.
 template <class S>
 __global__ void __launch_bounds__(kBlock)
     k_face_recon(Geo g, const S *__restrict__ gB, const S *__restrict__ hB, S *__restrict__ gS, S *__restrict__ hS,
@@ -215,7 +219,7 @@ __global__ void __launch_bounds__(kBlock) k_wall_rho(Geo g, const S *__restrict_
     double out = 0.0;
     for (int v = 0; v < g.Nv; ++v) {
         const double xs = sigma_dot(g.xi[v], Sf);
-        if (xs > 0) out += g.w[v] * xs * (double)gS[(size_t)v * g.nF + f];
+        if (xs > 0) out += g.w[v] * xs * ((double)gS[(size_t)v * g.nF + f] + g.ref_g[v]);
     }
     rho_b[b] = out / fabs(g.in_by_rho[b]);
 }
@@ -233,8 +237,8 @@ __global__ void __launch_bounds__(kBlock) k_wall_in(Geo g, const double *rho_b, 
         const double Tb = g.bf_T[b];
         const double gw = rho_b[b] * maxwell_by_rho(g, xi, g.bf_U[b], Tb);
         const size_t k = (size_t)v * g.nF + f;
-        gS[k] = (S)gw;
-        hS[k] = (S)(gw * (g.R * Tb) * (g.K + 3 - g.D));
+        gS[k] = (S)(gw - g.ref_g[v]);
+        hS[k] = (S)(gw * (g.R * Tb) * (g.K + 3 - g.D) - g.ref_h[v]);
     }
 }
 
@@ -249,7 +253,7 @@ __global__ void __launch_bounds__(kBlock)
     double rho = 0.0, rx = 0.0, ry = 0.0, rz = 0.0, rhoE = 0.0;
     for (int v = 0; v < g.Nv; ++v) {
         const size_t k = (size_t)v * g.nF + f;
-        const double gv = (double)gS[k], hv = (double)hS[k];
+        const double gv = (double)gS[k] + g.ref_g[v], hv = (double)hS[k] + g.ref_h[v];
         const d3 xi = g.xi[v];
         const double w = g.w[v];
         rho += w * gv;
@@ -265,7 +269,7 @@ __global__ void __launch_bounds__(kBlock)
     double qx = 0.0, qy = 0.0, qz = 0.0;
     for (int v = 0; v < g.Nv; ++v) {
         const size_t k = (size_t)v * g.nF + f;
-        const double gv = (double)gS[k], hv = (double)hS[k];
+        const double gv = (double)gS[k] + g.ref_g[v], hv = (double)hS[k] + g.ref_h[v];
         const d3 xi = g.xi[v];
         const d3 c{xi.x - U.x, xi.y - U.y, xi.z - U.z};
         const double m = magsqr(c) * gv + hv;
@@ -299,8 +303,8 @@ __global__ void __launch_bounds__(kBlock) k_face_relax(Geo g, Macros fm, S *gS, 
     shakhov(g, fm.rho[f], fm.U[f], T, fm.q[f], xi, Th, gEq, hEq);
     const double b = 0.5 * dt / (2.0 * fm.tau[f] + 0.5 * dt);
     const size_t k = (size_t)v * g.nF + f;
-    gS[k] = (S)((1.0 - b) * (double)gS[k] + b * gEq);
-    hS[k] = (S)((1.0 - b) * (double)hS[k] + b * hEq);
+    gS[k] = (S)((1.0 - b) * (double)gS[k] + b * (gEq - g.ref_g[v]));
+    hS[k] = (S)((1.0 - b) * (double)hS[k] + b * (hEq - g.ref_h[v]));
 }
 
 // updateGHtildeVol (fvDVM.C:1462-1519) as a cell-wise gather in ascending face order
@@ -315,16 +319,16 @@ __global__ void __launch_bounds__(kBlock)
     const double dt = *dtp;
     const d3 xi = g.xi[v];
     const size_t kc = (size_t)v * g.nC + c;
-    double ag = -1.0 / 3 * ((double)gT[kc] + g.ref_g[v]) + 4.0 / 3 * (double)gB[kc];
-    double ah = -1.0 / 3 * ((double)hT[kc] + g.ref_h[v]) + 4.0 / 3 * (double)hB[kc];
+    double ag = -1.0 / 3 * (double)gT[kc] + 4.0 / 3 * (double)gB[kc];  // deviations
+    double ah = -1.0 / 3 * (double)hT[kc] + 4.0 / 3 * (double)hB[kc];
     const double V = g.V[c];
     const S *gv = gS + (size_t)v * g.nF;
     const S *hv = hS + (size_t)v * g.nF;
     for (int k = g.cf_off[c]; k < g.cf_off[c + 1]; ++k) {
         const int f = g.cf_face[k];
         const double xs = sigma_dot(xi, g.Sf[f]);
-        const double fg = xs * (double)gv[f] * dt / V;
-        const double fh = xs * (double)hv[f] * dt / V;
+        const double fg = xs * ((double)gv[f] + g.ref_g[v]) * dt / V;  // fluxes of values
+        const double fh = xs * ((double)hv[f] + g.ref_h[v]) * dt / V;
         if (f < g.nIF && g.owner[f] != c) {
             ag += fg;
             ah += fh;
@@ -333,8 +337,8 @@ __global__ void __launch_bounds__(kBlock)
             ah -= fh;
         }
     }
-    gT[kc] = (S)(ag - g.ref_g[v]);
-    hT[kc] = (S)(ah - g.ref_h[v]);
+    gT[kc] = (S)ag;
+    hT[kc] = (S)ah;
 }
 
 // updateMacroVol (fvDVM.C:1521-1554, 1682-1743)
