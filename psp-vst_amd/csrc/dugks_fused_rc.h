@@ -54,6 +54,12 @@ struct alignas(16) BndCoef {
     double A, ux, uy, uz, invRT, hfac, pad0, pad1;
 };
 
+// Geometry of one pass-B face slot: area vector and the arms from the owner and neighbour
+// centres to the face centre (neighbour arm = owner arm on boundary faces).
+struct FaceGeo {
+    d3 Sf, ao, an;
+};
+
 struct BlockInfo {
     int c0, nc;        // block cells: permuted ids [c0, c0 + nc)
     int l_off, nl;     // local cell table: lcell[l_off + l], l < nl (block, ring 1, ring 2)
@@ -97,6 +103,7 @@ struct FusedPlan {
     EqCoef *cc = nullptr, *fc = nullptr;  // equilibrium coefficients of cells / faces
     BndCoef *bc = nullptr;                // incoming-half coefficients of boundary faces
     void *gT2 = nullptr, *hT2 = nullptr;  // second distribution buffer
+    FaceGeo *fb_geo = nullptr;            // [pass-B face slots] geometry, formed at upload
     std::vector<void *> allocs;
 };
 

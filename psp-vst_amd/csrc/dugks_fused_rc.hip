@@ -64,6 +64,7 @@ struct alignas(2 * sizeof(S)) P2 {
 struct FArgs {
     const BlockInfo *blk;
     const int *lcell, *gs_beg, *gs_nb, *fa_face, *fa_own, *fa_nei, *fb_face, *fb_own, *fb_nei, *fbc_beg, *fbc_idx;
+    const FaceGeo *fb_geo;
     const signed char *fbc_sgn;
     const void *gs_ls;
     double *fpart, *cpart;
@@ -79,7 +80,7 @@ FArgs fargs(const FusedPlan &p)
     FArgs a;
     a.blk = p.d_blk; a.lcell = p.d_lcell; a.gs_beg = p.d_gs_beg; a.gs_nb = p.d_gs_nb;
     a.fa_face = p.d_fa_face; a.fa_own = p.d_fa_own; a.fa_nei = p.d_fa_nei;
-    a.fb_face = p.d_fb_face; a.fb_own = p.d_fb_own; a.fb_nei = p.d_fb_nei;
+    a.fb_face = p.d_fb_face; a.fb_own = p.d_fb_own; a.fb_nei = p.d_fb_nei; a.fb_geo = p.fb_geo;
     a.fbc_beg = p.d_fbc_beg; a.fbc_idx = p.d_fbc_idx; a.fbc_sgn = p.d_fbc_sgn; a.gs_ls = p.d_gs_ls;
     a.fpart = p.fpart; a.cpart = p.cpart; a.bin = p.bin; a.bref = p.bref; a.cref = p.cref; a.cc = p.cc; a.fc = p.fc; a.bc = p.bc;
     a.nblk = p.nblk; a.b8 = p.b8; a.nvg = p.nvg; a.n1 = p.n1; a.nrow = p.nrow; a.nchunk = p.nchunk; a.cpg = p.cpg;
@@ -572,6 +573,27 @@ __global__ void __launch_bounds__(kBlk)
 // face equilibrium coefficients, with the relaxation weight h/(2 tau + h) (fvDVM.C:1380)
 // and, on patches with q1_compat, R times the internal-face temperature at the
 // patch-local index (fvDVM.C:1367, 1413)
+// pass-B face slot geometry (one work-group per block)
+__global__ void __launch_bounds__(kBlk) k_fb_geo(Geo g, FArgs a)
+{
+    const BlockInfo bi = a.blk[blockIdx.x];
+    for (int j = threadIdx.x; j < bi.nfb; j += kBlk) {
+        const int f = a.fb_face[bi.fb_off + j];
+        const int lo = a.fb_own[bi.fb_off + j], ln = a.fb_nei[bi.fb_off + j];
+        const d3 Cf = g.Cf[f];
+        const d3 Co = g.C[a.lcell[bi.l_off + lo]];
+        FaceGeo r;
+        r.Sf = g.Sf[f];
+        r.ao = d3{Cf.x - Co.x, Cf.y - Co.y, Cf.z - Co.z};
+        r.an = r.ao;
+        if (ln >= 0) {
+            const d3 Cn = g.C[a.lcell[bi.l_off + ln]];
+            r.an = d3{Cf.x - Cn.x, Cf.y - Cn.y, Cf.z - Cn.z};
+        }
+        ((FaceGeo *)a.fb_geo)[bi.fb_off + j] = r;
+    }
+}
+
 __global__ void __launch_bounds__(kBlk) k_face_coef(Geo g, Macros fm, EqCoef *fc, const double *dtp)
 {
     const int f = blockIdx.x * kBlk + threadIdx.x;
@@ -643,17 +665,10 @@ __global__ void __launch_bounds__(kNTB, kWavesB)
             if (j >= bi.nfb) break;
             const int f = a.fb_face[bi.fb_off + j];
             const int lo = a.fb_own[bi.fb_off + j], ln = a.fb_nei[bi.fb_off + j];
-            const d3 Sf = g.Sf[f];
-            const d3 Cf = g.Cf[f];
-            const d3 Co = g.C[a.lcell[bi.l_off + lo]];
-            const double aox = Cf.x - Co.x, aoy = Cf.y - Co.y, aoz = Cf.z - Co.z;
-            double anx = aox, any = aoy, anz = aoz;
-            if (ln >= 0) {
-                const d3 Cn = g.C[a.lcell[bi.l_off + ln]];
-                anx = Cf.x - Cn.x;
-                any = Cf.y - Cn.y;
-                anz = Cf.z - Cn.z;
-            }
+            const FaceGeo fgeo = a.fb_geo[bi.fb_off + j];
+            const d3 Sf = fgeo.Sf;
+            const double aox = fgeo.ao.x, aoy = fgeo.ao.y, aoz = fgeo.ao.z;
+            const double anx = fgeo.an.x, any = fgeo.an.y, anz = fgeo.an.z;
             const EqCoef k = a.fc[f];
             if (ck.new_stripe) stripe_factors<VC>(g, k, ck.ix0, a.n1, fex[s]);
             const double eyz = row_factor(k, xr);
@@ -994,6 +1009,14 @@ bool fused_plan_upload(FusedPlan &p, const Geo &g, size_t es, std::string &err)
         !dbuf(p, &fc, (size_t)g.nF * sizeof(EqCoef), err) || !dbuf(p, &bc, (size_t)g.nB * sizeof(BndCoef), err) ||
         !dbuf(p, &p.gT2, (size_t)g.Nv * g.nC * es, err) || !dbuf(p, &p.hT2, (size_t)g.Nv * g.nC * es, err))
         return false;
+    void *geo = nullptr;
+    if (!dbuf(p, &geo, p.fb_face.size() * sizeof(FaceGeo), err)) return false;
+    p.fb_geo = (FaceGeo *)geo;
+    hipLaunchKernelGGL(k_fb_geo, dim3(p.nblk), dim3(kBlk), 0, 0, g, fargs(p));
+    if (hipDeviceSynchronize() != hipSuccess) {
+        err = "forming the face geometry failed";
+        return false;
+    }
     p.fpart = (double *)fp;
     p.cpart = (double *)cp;
     p.bin = (double *)bi;
