@@ -103,6 +103,8 @@ struct FusedPlan {
     // its own slot
     void *F = nullptr;
     int n_slots = 0;
+    d3 *fs_sf = nullptr;                  // [slots] face area vectors, formed at upload
+    EqCoef *fs_coef = nullptr;            // [slots] face records, copied each step
     std::vector<void *> allocs;
 };
 

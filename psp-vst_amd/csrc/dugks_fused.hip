@@ -56,6 +56,8 @@ struct alignas(2 * sizeof(S)) P2 {
 struct FArgs {
     const BlockInfo *blk;
     const int *lcell, *gs_beg, *gs_nb, *fs_face, *fs_twin, *fs_lo, *fs_ln, *fbc_beg, *fbc_idx;
+    d3 *fs_sf;
+    EqCoef *fs_coef;
     const signed char *fs_flag, *fbc_sgn;
     const void *gs_ls;
     double *fpart, *cpart;
@@ -71,6 +73,7 @@ FArgs fargs(const FusedPlan &p)
     FArgs a;
     a.blk = p.d_blk; a.lcell = p.d_lcell; a.gs_beg = p.d_gs_beg; a.gs_nb = p.d_gs_nb;
     a.fs_face = p.d_fs_face; a.fs_twin = p.d_fs_twin; a.fs_lo = p.d_fs_lo; a.fs_ln = p.d_fs_ln;
+    a.fs_sf = p.fs_sf; a.fs_coef = p.fs_coef;
     a.n_slots = p.n_slots;
     a.fbc_beg = p.d_fbc_beg; a.fbc_idx = p.d_fbc_idx; a.fs_flag = p.d_fs_flag; a.fbc_sgn = p.d_fbc_sgn;
     a.gs_ls = p.d_gs_ls;
@@ -349,10 +352,10 @@ __device__ inline void cell_bar(const Geo &g, const FArgs &a, int c, const S *__
 }
 
 // pass A stage 1: f-bar-plus of the local cells (block + ring) into LDS
-template <class S, int VC, int NT, int CPT>
+template <class S, int VC, int NT, int CPT, bool D3>
 __device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &bi, const S *__restrict__ gT,
-                                 const S *__restrict__ hT, double dt, const Chunk &ck, double (&cex)[CPT][VC],
-                                 P2<S> *fb)
+                                 const S *__restrict__ hT, double dt, const Chunk &ck, const int (&lc)[CPT],
+                                 double (&cex)[CPT][VC], P2<S> *fb)
 {
     const d3 xr = g.xi[ck.vrow];
 #pragma unroll
@@ -361,7 +364,9 @@ __device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &
         if (l >= bi.nl) break;
         S gt[VC], ht[VC];
         double gb[VC], hb[VC];
-        cell_bar<S, VC>(g, a, a.lcell[bi.l_off + l], gT, hT, dt, ck, xr, cex[s], gt, ht, gb, hb);
+        // the hoisted cell ids cost FP64 3-D pass A registers it does not have (spills)
+        const int c = (sizeof(S) == 8 && D3) ? a.lcell[bi.l_off + l] : lc[s];
+        cell_bar<S, VC>(g, a, c, gT, hT, dt, ck, xr, cex[s], gt, ht, gb, hb);
 #pragma unroll
         for (int vv = 0; vv < VC; ++vv) {
             if (vv >= ck.nvv) break;
@@ -480,10 +485,16 @@ __global__ void __launch_bounds__(kNTA)
     }
 
     double cex[CPT][VC];
+    int lc[CPT];  // the thread's local cells (fixed for the kernel)
+#pragma unroll
+    for (int s = 0; s < CPT; ++s) {
+        const int l = threadIdx.x + s * kNTA;
+        lc[s] = l < bi.nl ? a.lcell[bi.l_off + l] : 0;
+    }
     int stripe = -1;
     for (int ch = ch0; ch < ch1; ++ch) {
         const Chunk ck = chunk_of(a, ch, VC, stripe);
-        stage_bar<S, VC, kNTA, CPT>(g, a, bi, gT, hT, dt, ck, cex, fb);
+        stage_bar<S, VC, kNTA, CPT, D == 3>(g, a, bi, gT, hT, dt, ck, lc, cex, fb);
         __syncthreads();
         stage_grad<S, D, VC, kNTA>(a, bi, ck.nvv, fb, gr);
         __syncthreads();
@@ -629,6 +640,19 @@ __global__ void __launch_bounds__(kBlk) k_face_coef(Geo g, Macros fm, EqCoef *fc
     fc[f] = make_coef(g, fm.rho[f], fm.U[f], T, fm.q[f], g.R * Th, h / (2.0 * fm.tau[f] + h));
 }
 
+// face records and area vectors in face-slot order, so pass B loads them by slot
+__global__ void __launch_bounds__(kBlk) k_slot_coef(FArgs a, int n_slots)
+{
+    const int j = blockIdx.x * kBlk + threadIdx.x;
+    if (j < n_slots) a.fs_coef[j] = a.fc[a.fs_face[j]];
+}
+
+__global__ void __launch_bounds__(kBlk) k_slot_sf(Geo g, FArgs a, int n_slots)
+{
+    const int j = blockIdx.x * kBlk + threadIdx.x;
+    if (j < n_slots) a.fs_sf[j] = g.Sf[a.fs_face[j]];
+}
+
 // ------------------------------------------------------------------------- pass B
 // a stored face value as a value: FP32 slots hold deviations from the reference
 template <class S>
@@ -693,8 +717,10 @@ __global__ void __launch_bounds__(kNTB)
             const int f = a.fs_face[bi.fs_off + j];
             // pass A wrote the upwind value into the slots of both blocks of a shared face
             const int own_slot = bi.fs_off + j;
-            const d3 Sf = g.Sf[f];
-            const EqCoef k = a.fc[f];
+            // 2-D: per-slot copies, loads that do not wait on the face label (in 3-D the copies
+            // cost registers that push pass B to one wave per SIMD: 131 -> 196 ms on the sphere)
+            const d3 Sf = D == 2 ? a.fs_sf[own_slot] : g.Sf[f];
+            const EqCoef k = D == 2 ? a.fs_coef[own_slot] : a.fc[f];
             if (ck.new_stripe) stripe_factors<VC>(g, k, ck.ix0, a.n1, fex[s]);
             const double eyz = row_factor(k, xr);
             const int b = f - g.nIF;
@@ -1022,9 +1048,16 @@ bool fused_plan_upload(FusedPlan &p, const Geo &g, size_t es, std::string &err)
     p.cref = (double *)cr;
     p.cc = (EqCoef *)cc;
     p.fc = (EqCoef *)fc;
+    void *sf = nullptr, *sc = nullptr;
+    if (!dbuf(p, &sf, (size_t)p.n_slots * sizeof(d3), err) || !dbuf(p, &sc, (size_t)p.n_slots * sizeof(EqCoef), err))
+        return false;
+    p.fs_sf = (d3 *)sf;
+    p.fs_coef = (EqCoef *)sc;
     p.bc = (BndCoef *)bc;
     // (face, side) slots no block writes (the second side of a face inside one block) stay 0
     hipLaunchKernelGGL(k_zero_d, dim3((unsigned)((nfpart + kBlk - 1) / kBlk)), dim3(kBlk), 0, 0, p.fpart, nfpart);
+    hipLaunchKernelGGL(k_slot_sf, dim3((unsigned)((p.n_slots + kBlk - 1) / kBlk)), dim3(kBlk), 0, 0, g, fargs(p),
+                       p.n_slots);
     if (hipDeviceSynchronize() != hipSuccess) {
         err = "clearing the face partials failed";
         return false;
@@ -1082,6 +1115,7 @@ void Fused<S>::face_macros(const Geo &g, FusedPlan &p, Macros fm, double *rho_b,
     else
         hipLaunchKernelGGL(k_face_macros<2>, grid1(g.nF), dim3(kBlk), 0, st, g, fargs(p), fm, rho_b, dt);
     hipLaunchKernelGGL(k_face_coef, grid1(g.nF), dim3(kBlk), 0, st, g, fm, p.fc, dt);
+    if (g.D == 2) hipLaunchKernelGGL(k_slot_coef, grid1(p.n_slots), dim3(kBlk), 0, st, fargs(p), p.n_slots);
     launch_cfl_partials(g, fm, cfl_part, st);
 }
 

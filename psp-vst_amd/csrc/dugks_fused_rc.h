@@ -104,6 +104,7 @@ struct FusedPlan {
     BndCoef *bc = nullptr;                // incoming-half coefficients of boundary faces
     void *gT2 = nullptr, *hT2 = nullptr;  // second distribution buffer
     FaceGeo *fb_geo = nullptr;            // [pass-B face slots] geometry, formed at upload
+    EqCoef *fb_coef = nullptr;            // [pass-B face slots] face records, copied each step
     std::vector<void *> allocs;
 };
 

@@ -65,6 +65,7 @@ struct FArgs {
     const BlockInfo *blk;
     const int *lcell, *gs_beg, *gs_nb, *fa_face, *fa_own, *fa_nei, *fb_face, *fb_own, *fb_nei, *fbc_beg, *fbc_idx;
     const FaceGeo *fb_geo;
+    EqCoef *fb_coef;
     const signed char *fbc_sgn;
     const void *gs_ls;
     double *fpart, *cpart;
@@ -80,7 +81,7 @@ FArgs fargs(const FusedPlan &p)
     FArgs a;
     a.blk = p.d_blk; a.lcell = p.d_lcell; a.gs_beg = p.d_gs_beg; a.gs_nb = p.d_gs_nb;
     a.fa_face = p.d_fa_face; a.fa_own = p.d_fa_own; a.fa_nei = p.d_fa_nei;
-    a.fb_face = p.d_fb_face; a.fb_own = p.d_fb_own; a.fb_nei = p.d_fb_nei; a.fb_geo = p.fb_geo;
+    a.fb_face = p.d_fb_face; a.fb_own = p.d_fb_own; a.fb_nei = p.d_fb_nei; a.fb_geo = p.fb_geo; a.fb_coef = p.fb_coef;
     a.fbc_beg = p.d_fbc_beg; a.fbc_idx = p.d_fbc_idx; a.fbc_sgn = p.d_fbc_sgn; a.gs_ls = p.d_gs_ls;
     a.fpart = p.fpart; a.cpart = p.cpart; a.bin = p.bin; a.bref = p.bref; a.cref = p.cref; a.cc = p.cc; a.fc = p.fc; a.bc = p.bc;
     a.nblk = p.nblk; a.b8 = p.b8; a.nvg = p.nvg; a.n1 = p.n1; a.nrow = p.nrow; a.nchunk = p.nchunk; a.cpg = p.cpg;
@@ -313,19 +314,19 @@ __device__ inline Chunk chunk_of(const FArgs &a, int ch, int vc, int &stripe)
     return c;
 }
 
-// stage 1: f-bar-plus of local cells for one chunk; cex[s] holds the x-factors of the
-// thread's slot-s cell for the current stripe
+// stage 1: f-bar-plus of local cells for one chunk; lc[s] is the cell of the thread's
+// slot-s local cell (fixed for the kernel), cex[s] holds its x-factors for the current stripe
 template <class S, int VC, int NT, int CPT>
 __device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &bi, const S *__restrict__ gT,
-                                 const S *__restrict__ hT, double dt, const Chunk &ck, double (&cex)[CPT][VC],
-                                 P2<S> *fb, P2<S> *ft)
+                                 const S *__restrict__ hT, double dt, const Chunk &ck, const int (&lc)[CPT],
+                                 double (&cex)[CPT][VC], P2<S> *fb, P2<S> *ft)
 {
     const d3 xr = g.xi[ck.vrow];
 #pragma unroll
     for (int s = 0; s < CPT; ++s) {
         const int l = threadIdx.x + s * NT;
         if (l >= bi.nl) break;
-        const int c = a.lcell[bi.l_off + l];
+        const int c = lc[s];
         const EqCoef k = a.cc[c];
         S gt[VC], ht[VC];
 #pragma unroll
@@ -465,10 +466,16 @@ __global__ void __launch_bounds__(kNTA, kWavesA)
     }
 
     double cex[CPT][VC];
+    int lc[CPT];
+#pragma unroll
+    for (int s = 0; s < CPT; ++s) {
+        const int l = threadIdx.x + s * kNTA;
+        lc[s] = l < bi.nl ? a.lcell[bi.l_off + l] : 0;
+    }
     int stripe = -1;
     for (int ch = ch0; ch < ch1; ++ch) {
         const Chunk ck = chunk_of(a, ch, VC, stripe);
-        stage_bar<S, VC, kNTA, CPT>(g, a, bi, gT, hT, dt, ck, cex, fb, (P2<S> *)nullptr);
+        stage_bar<S, VC, kNTA, CPT>(g, a, bi, gT, hT, dt, ck, lc, cex, fb, (P2<S> *)nullptr);
         __syncthreads();
         stage_grad<S, D, VC, kNTA>(a, bi, ck.nvv, fb, gr);
         __syncthreads();
@@ -605,6 +612,13 @@ __global__ void __launch_bounds__(kBlk) k_face_coef(Geo g, Macros fm, EqCoef *fc
     fc[f] = make_coef(g, fm.rho[f], fm.U[f], T, fm.q[f], g.R * Th, h / (2.0 * fm.tau[f] + h));
 }
 
+// face records gathered into pass B's face-slot order, so pass B loads them by slot
+__global__ void __launch_bounds__(kBlk) k_slot_coef(FArgs a, int n_slots)
+{
+    const int j = blockIdx.x * kBlk + threadIdx.x;
+    if (j < n_slots) a.fb_coef[j] = a.fc[a.fb_face[j]];
+}
+
 // ------------------------------------------------------------------------- pass B
 template <class S, int D, int VC>
 __global__ void __launch_bounds__(kNTB, kWavesB)
@@ -650,10 +664,16 @@ __global__ void __launch_bounds__(kNTB, kWavesB)
     }
 
     double cex[CPT][VC], fex[FPT][VC];
+    int lc[CPT];
+#pragma unroll
+    for (int s = 0; s < CPT; ++s) {
+        const int l = threadIdx.x + s * kNTB;
+        lc[s] = l < bi.nl ? a.lcell[bi.l_off + l] : 0;
+    }
     int stripe = -1;
     for (int ch = ch0; ch < ch1; ++ch) {
         const Chunk ck = chunk_of(a, ch, VC, stripe);
-        stage_bar<S, VC, kNTB, CPT>(g, a, bi, gT, hT, dt, ck, cex, fb, ft);
+        stage_bar<S, VC, kNTB, CPT>(g, a, bi, gT, hT, dt, ck, lc, cex, fb, ft);
         __syncthreads();
         stage_grad<S, D, VC, kNTB>(a, bi, ck.nvv, fb, gr);
         __syncthreads();
@@ -669,7 +689,7 @@ __global__ void __launch_bounds__(kNTB, kWavesB)
             const d3 Sf = fgeo.Sf;
             const double aox = fgeo.ao.x, aoy = fgeo.ao.y, aoz = fgeo.ao.z;
             const double anx = fgeo.an.x, any = fgeo.an.y, anz = fgeo.an.z;
-            const EqCoef k = a.fc[f];
+            const EqCoef k = a.fb_coef[bi.fb_off + j];
             if (ck.new_stripe) stripe_factors<VC>(g, k, ck.ix0, a.n1, fex[s]);
             const double eyz = row_factor(k, xr);
             const int b = f - g.nIF;
@@ -1009,9 +1029,12 @@ bool fused_plan_upload(FusedPlan &p, const Geo &g, size_t es, std::string &err)
         !dbuf(p, &fc, (size_t)g.nF * sizeof(EqCoef), err) || !dbuf(p, &bc, (size_t)g.nB * sizeof(BndCoef), err) ||
         !dbuf(p, &p.gT2, (size_t)g.Nv * g.nC * es, err) || !dbuf(p, &p.hT2, (size_t)g.Nv * g.nC * es, err))
         return false;
-    void *geo = nullptr;
-    if (!dbuf(p, &geo, p.fb_face.size() * sizeof(FaceGeo), err)) return false;
+    void *geo = nullptr, *fbc = nullptr;
+    if (!dbuf(p, &geo, p.fb_face.size() * sizeof(FaceGeo), err) ||
+        !dbuf(p, &fbc, p.fb_face.size() * sizeof(EqCoef), err))
+        return false;
     p.fb_geo = (FaceGeo *)geo;
+    p.fb_coef = (EqCoef *)fbc;
     hipLaunchKernelGGL(k_fb_geo, dim3(p.nblk), dim3(kBlk), 0, 0, g, fargs(p));
     if (hipDeviceSynchronize() != hipSuccess) {
         err = "forming the face geometry failed";
@@ -1079,6 +1102,8 @@ void Fused<S>::face_macros(const Geo &g, FusedPlan &p, Macros fm, double *rho_b,
     else
         hipLaunchKernelGGL(k_face_macros<2>, grid1(g.nF), dim3(kBlk), 0, st, g, fargs(p), fm, rho_b, dt);
     hipLaunchKernelGGL(k_face_coef, grid1(g.nF), dim3(kBlk), 0, st, g, fm, p.fc, dt);
+    const int n_slots = (int)p.fb_face.size();
+    hipLaunchKernelGGL(k_slot_coef, grid1(n_slots), dim3(kBlk), 0, st, fargs(p), n_slots);
     launch_cfl_partials(g, fm, cfl_part, st);
 }
 
