@@ -38,6 +38,7 @@ namespace {
 
 constexpr int kNTA = kFusedThreadsA;
 constexpr int kSlotsA = kFusedSlotsA;
+static_assert(kFusedThreadsB <= kFusedThreadsA, "pass A takes one gradient (block) cell per thread");
 constexpr int kNTB = kFusedThreadsB;
 constexpr int kNb = kFusedMaxNb;
 constexpr int kBlk = 256;
@@ -376,13 +377,32 @@ __device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &
 }
 
 // pass A stage 2: gradients of the block cells
-template <class S, int D, int VC, int NT>
-__device__ inline void stage_grad(const FArgs &a, const BlockInfo &bi, int nvv, const P2<S> *fb, P2<S> *gr)
+// stencil range (first entry, count) of the thread's gradient cells, fixed for the kernel
+template <int NT, int GPT>
+__device__ inline void grad_ranges(const FArgs &a, const BlockInfo &bi, int ng, int2 (&gk)[GPT])
+{
+#pragma unroll
+    for (int s = 0; s < GPT; ++s) {
+        const int l = threadIdx.x + s * NT;
+        gk[s] = int2{0, 0};
+        if (l < ng) {
+            const int k0 = a.gs_beg[bi.gsb_off + l];
+            gk[s] = int2{k0, a.gs_beg[bi.gsb_off + l + 1] - k0};
+        }
+    }
+}
+
+template <class S, int D, int VC, int NT, int GPT>
+__device__ inline void stage_grad(const FArgs &a, const BlockInfo &bi, int nvv, const int2 (&gk)[GPT],
+                                  const P2<S> *fb, P2<S> *gr)
 {
     const V3<S> *ls = (const V3<S> *)a.gs_ls;
-    for (int l = threadIdx.x; l < bi.nc; l += NT) {
-        const int k0 = a.gs_beg[bi.gsb_off + l];
-        const int n = a.gs_beg[bi.gsb_off + l + 1] - k0;
+#pragma unroll
+    for (int s = 0; s < GPT; ++s) {
+        const int l = threadIdx.x + s * NT;
+        if (l >= bi.nc) break;
+        const int k0 = gk[s].x;
+        const int n = gk[s].y;
         int nb[kNb];
         V3<S> L[kNb];
 #pragma unroll
@@ -491,12 +511,14 @@ __global__ void __launch_bounds__(kNTA)
         const int l = threadIdx.x + s * kNTA;
         lc[s] = l < bi.nl ? a.lcell[bi.l_off + l] : 0;
     }
+    int2 gk[1];  // block cells <= 256 < kNTA: one gradient cell per thread
+    grad_ranges<kNTA, 1>(a, bi, bi.nc, gk);
     int stripe = -1;
     for (int ch = ch0; ch < ch1; ++ch) {
         const Chunk ck = chunk_of(a, ch, VC, stripe);
         stage_bar<S, VC, kNTA, CPT, D == 3>(g, a, bi, gT, hT, dt, ck, lc, cex, fb);
         __syncthreads();
-        stage_grad<S, D, VC, kNTA>(a, bi, ck.nvv, fb, gr);
+        stage_grad<S, D, VC, kNTA, 1>(a, bi, ck.nvv, gk, fb, gr);
         __syncthreads();
         const d3 xr = g.xi[ck.vrow];
 #pragma unroll
