@@ -64,7 +64,8 @@ struct BlockInfo {
     int c0, nc;        // block cells: permuted ids [c0, c0 + nc)
     int l_off, nl;     // local cell table: lcell[l_off + l], l < nl (block, ring 1, ring 2)
     int ng;            // local cells l < ng (block + ring 1) get a gradient
-    int gsb_off;       // gs_beg[gsb_off + l], l <= ng: stencil ranges (absolute)
+    int gsb_off;       // gradient cell gsb_off + l, l < ng: gs_cnt[.] neighbours at gs_nb / gs_ls
+                       // [(gsb_off + l) * kFusedMaxNb + j]
     int fa_off, nfa;   // faces owned by the block (owner in block), pass A
     int fb_off, nfb;   // faces of the block cells, pass B
     int fbc_off;       // fbc_beg[fbc_off + i], i <= nc: per block cell ranges into fbc_*
@@ -85,12 +86,12 @@ struct FusedPlan {
     size_t lds_a = 0, lds_b = 0;       // dynamic LDS bytes of the two passes
     // host tables
     std::vector<BlockInfo> blk;
-    std::vector<int> lcell, gs_beg, gs_nb, fa_face, fa_own, fa_nei, fb_face, fb_own, fb_nei, fbc_beg, fbc_idx;
+    std::vector<int> lcell, gs_cnt, gs_nb, fa_face, fa_own, fa_nei, fb_face, fb_own, fb_nei, fbc_beg, fbc_idx;
     std::vector<signed char> fbc_sgn;
     std::vector<d3> gs_ls;
     // device tables
     BlockInfo *d_blk = nullptr;
-    int *d_lcell = nullptr, *d_gs_beg = nullptr, *d_gs_nb = nullptr, *d_fa_face = nullptr, *d_fa_own = nullptr,
+    int *d_lcell = nullptr, *d_gs_cnt = nullptr, *d_gs_nb = nullptr, *d_fa_face = nullptr, *d_fa_own = nullptr,
         *d_fa_nei = nullptr, *d_fb_face = nullptr, *d_fb_own = nullptr, *d_fb_nei = nullptr, *d_fbc_beg = nullptr,
         *d_fbc_idx = nullptr;
     signed char *d_fbc_sgn = nullptr;

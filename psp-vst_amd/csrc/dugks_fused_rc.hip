@@ -63,7 +63,7 @@ struct alignas(2 * sizeof(S)) P2 {
 
 struct FArgs {
     const BlockInfo *blk;
-    const int *lcell, *gs_beg, *gs_nb, *fa_face, *fa_own, *fa_nei, *fb_face, *fb_own, *fb_nei, *fbc_beg, *fbc_idx;
+    const int *lcell, *gs_cnt, *gs_nb, *fa_face, *fa_own, *fa_nei, *fb_face, *fb_own, *fb_nei, *fbc_beg, *fbc_idx;
     const FaceGeo *fb_geo;
     EqCoef *fb_coef;
     const signed char *fbc_sgn;
@@ -79,7 +79,7 @@ struct FArgs {
 FArgs fargs(const FusedPlan &p)
 {
     FArgs a;
-    a.blk = p.d_blk; a.lcell = p.d_lcell; a.gs_beg = p.d_gs_beg; a.gs_nb = p.d_gs_nb;
+    a.blk = p.d_blk; a.lcell = p.d_lcell; a.gs_cnt = p.d_gs_cnt; a.gs_nb = p.d_gs_nb;
     a.fa_face = p.d_fa_face; a.fa_own = p.d_fa_own; a.fa_nei = p.d_fa_nei;
     a.fb_face = p.d_fb_face; a.fb_own = p.d_fb_own; a.fb_nei = p.d_fb_nei; a.fb_geo = p.fb_geo; a.fb_coef = p.fb_coef;
     a.fbc_beg = p.d_fbc_beg; a.fbc_idx = p.d_fbc_idx; a.fbc_sgn = p.d_fbc_sgn; a.gs_ls = p.d_gs_ls;
@@ -365,13 +365,16 @@ __device__ inline void stage_grad(const FArgs &a, const BlockInfo &bi, int nvv, 
 {
     const V3<S> *ls = (const V3<S> *)a.gs_ls;
     for (int l = threadIdx.x; l < bi.ng; l += NT) {
-        const int k0 = a.gs_beg[bi.gsb_off + l];
-        const int n = a.gs_beg[bi.gsb_off + l + 1] - k0;
+        const int k0 = (bi.gsb_off + l) * kNb;
+        const int n = a.gs_cnt[bi.gsb_off + l];
+        // the first NU entries (faces of a quad / hex) are read without waiting for the count;
+        // padded entries are local cell 0 with a zero vector and are not used
+        constexpr int NU = D == 2 ? 4 : 6;
         int nb[kNb];
         V3<S> L[kNb];
 #pragma unroll
         for (int j = 0; j < kNb; ++j) {
-            if (j < n) {
+            if (j < NU || j < n) {
                 nb[j] = a.gs_nb[k0 + j];
                 L[j] = ls[k0 + j];
             }
@@ -1003,7 +1006,7 @@ bool fused_plan_upload(FusedPlan &p, const Geo &g, size_t es, std::string &err)
         err = "cannot raise the dynamic LDS limit";
         return false;
     }
-    if (!up(p, &p.d_blk, p.blk, err) || !up(p, &p.d_lcell, p.lcell, err) || !up(p, &p.d_gs_beg, p.gs_beg, err) ||
+    if (!up(p, &p.d_blk, p.blk, err) || !up(p, &p.d_lcell, p.lcell, err) || !up(p, &p.d_gs_cnt, p.gs_cnt, err) ||
         !up(p, &p.d_gs_nb, p.gs_nb, err) || !up(p, &p.d_fa_face, p.fa_face, err) ||
         !up(p, &p.d_fa_own, p.fa_own, err) || !up(p, &p.d_fa_nei, p.fa_nei, err) ||
         !up(p, &p.d_fb_face, p.fb_face, err) || !up(p, &p.d_fb_own, p.fb_own, err) ||

@@ -135,10 +135,17 @@ bool fused_plan_build(FusedPlan &p, const Geo &g, const HostPrep &prep, const st
         for (int l = 0; l < bi.nl; ++l) local[L[l]] = l;
         p.lcell.insert(p.lcell.end(), L.begin(), L.end());
         // gradient stencils of block + ring 1, ascending face label (the order of cg)
-        bi.gsb_off = (int)p.gs_beg.size();
+        // fixed stride of kFusedMaxNb entries per gradient cell (padded with local cell 0 and
+        // zero vectors), so a neighbour's address does not wait on a loaded range
+        bi.gsb_off = (int)p.gs_cnt.size();
         for (int l = 0; l < bi.ng; ++l) {
             const int c = L[l];
-            p.gs_beg.push_back((int)p.gs_nb.size());
+            const int n_st = cg_off[c + 1] - cg_off[c];
+            if (n_st > kFusedMaxNb) {
+                err = "cells with more than " + std::to_string(kFusedMaxNb) + " faces";
+                return false;
+            }
+            p.gs_cnt.push_back(n_st);
             for (int k = cg_off[c]; k < cg_off[c + 1]; ++k) {
                 const int n = cg_nb[k];
                 if (mark[n] != b || local[n] < 0 || L[local[n]] != n) {
@@ -148,8 +155,11 @@ bool fused_plan_build(FusedPlan &p, const Geo &g, const HostPrep &prep, const st
                 p.gs_nb.push_back(local[n]);
                 p.gs_ls.push_back(prep.cg_ls[k]);
             }
+            for (int j = n_st; j < kFusedMaxNb; ++j) {
+                p.gs_nb.push_back(0);
+                p.gs_ls.push_back(d3{0, 0, 0});
+            }
         }
-        p.gs_beg.push_back((int)p.gs_nb.size());
         // owned faces (owner in block), ascending face label
         bi.fa_off = (int)p.fa_face.size();
         for (int c = c0; c < c1; ++c)
@@ -196,7 +206,7 @@ bool fused_plan_build(FusedPlan &p, const Geo &g, const HostPrep &prep, const st
         p.max_nc = std::max(p.max_nc, bi.nc);
     }
     int max_st = 0, max_cf = 0;
-    for (size_t i = 0; i + 1 < p.gs_beg.size(); ++i) max_st = std::max(max_st, p.gs_beg[i + 1] - p.gs_beg[i]);
+    for (int n_st : p.gs_cnt) max_st = std::max(max_st, n_st);
     for (size_t i = 0; i + 1 < p.fbc_beg.size(); ++i) max_cf = std::max(max_cf, p.fbc_beg[i + 1] - p.fbc_beg[i]);
     if (max_st > kFusedMaxNb || max_cf > kFusedMaxNb) {
         err = "cells with more than " + std::to_string(kFusedMaxNb) + " faces";
