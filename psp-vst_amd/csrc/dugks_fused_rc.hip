@@ -314,12 +314,34 @@ __device__ inline Chunk chunk_of(const FArgs &a, int ch, int vc, int &stripe)
     return c;
 }
 
-// stage 1: f-bar-plus of local cells for one chunk; lc[s] is the cell of the thread's
-// slot-s local cell (fixed for the kernel), cex[s] holds its x-factors for the current stripe
+// g~, h~ of the thread's local cells for one chunk (lc[s]: cell of the slot-s local cell);
+// velocities past the end of a short stripe repeat its last one
 template <class S, int VC, int NT, int CPT>
+__device__ inline void load_tilde(const Geo &g, const BlockInfo &bi, const S *__restrict__ gT,
+                                  const S *__restrict__ hT, const Chunk &ck, const int (&lc)[CPT],
+                                  S (&gt)[CPT][VC], S (&ht)[CPT][VC])
+{
+#pragma unroll
+    for (int s = 0; s < CPT; ++s) {
+        const int l = threadIdx.x + s * NT;
+        if (l >= bi.nl) break;
+#pragma unroll
+        for (int vv = 0; vv < VC; ++vv) {
+            const size_t kk = (size_t)(ck.vrow + ck.ix0 + min(vv, ck.nvv - 1)) * g.nC + lc[s];
+            gt[s][vv] = gT[kk];
+            ht[s][vv] = hT[kk];
+        }
+    }
+}
+
+// stage 1: f-bar-plus of local cells for one chunk; cex[s] holds the slot-s cell's
+// x-factors for the current stripe.  PRE: g~, h~ were loaded into pg, ph beforehand;
+// otherwise each slot loads its own just before use (fewer live registers)
+template <class S, int VC, int NT, int CPT, bool PRE>
 __device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &bi, const S *__restrict__ gT,
                                  const S *__restrict__ hT, double dt, const Chunk &ck, const int (&lc)[CPT],
-                                 double (&cex)[CPT][VC], P2<S> *fb, P2<S> *ft)
+                                 const S (&pg)[CPT][VC], const S (&ph)[CPT][VC], double (&cex)[CPT][VC],
+                                 P2<S> *fb, P2<S> *ft)
 {
     const d3 xr = g.xi[ck.vrow];
 #pragma unroll
@@ -331,9 +353,14 @@ __device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &
         S gt[VC], ht[VC];
 #pragma unroll
         for (int vv = 0; vv < VC; ++vv) {
-            const size_t kk = (size_t)(ck.vrow + ck.ix0 + min(vv, ck.nvv - 1)) * g.nC + c;
-            gt[vv] = gT[kk];
-            ht[vv] = hT[kk];
+            if (PRE) {
+                gt[vv] = pg[s][vv];
+                ht[vv] = ph[s][vv];
+            } else {
+                const size_t kk = (size_t)(ck.vrow + ck.ix0 + min(vv, ck.nvv - 1)) * g.nC + c;
+                gt[vv] = gT[kk];
+                ht[vv] = hT[kk];
+            }
         }
         if (ck.new_stripe) stripe_factors<VC>(g, k, ck.ix0, a.n1, cex[s]);
         const double eyz = row_factor(k, xr);
@@ -469,6 +496,7 @@ __global__ void __launch_bounds__(kNTA, kWavesA)
     }
 
     double cex[CPT][VC];
+    const S unused[CPT][VC] = {};
     int lc[CPT];
 #pragma unroll
     for (int s = 0; s < CPT; ++s) {
@@ -478,7 +506,8 @@ __global__ void __launch_bounds__(kNTA, kWavesA)
     int stripe = -1;
     for (int ch = ch0; ch < ch1; ++ch) {
         const Chunk ck = chunk_of(a, ch, VC, stripe);
-        stage_bar<S, VC, kNTA, CPT>(g, a, bi, gT, hT, dt, ck, lc, cex, fb, (P2<S> *)nullptr);
+        stage_bar<S, VC, kNTA, CPT, false>(g, a, bi, gT, hT, dt, ck, lc, unused, unused, cex, fb,
+                                           (P2<S> *)nullptr);
         __syncthreads();
         stage_grad<S, D, VC, kNTA>(a, bi, ck.nvv, fb, gr);
         __syncthreads();
@@ -673,10 +702,19 @@ __global__ void __launch_bounds__(kNTB, kWavesB)
         const int l = threadIdx.x + s * kNTB;
         lc[s] = l < bi.nl ? a.lcell[bi.l_off + l] : 0;
     }
+    // 2-D FP32 storage: the next chunk's g~, h~ are loaded before the chunk's last barrier,
+    // so their latency overlaps the barrier wait (16 registers; elsewhere the registers
+    // are not there to spare)
+    constexpr bool kAhead = D == 2 && sizeof(S) == 4;
+    S gt[CPT][VC] = {}, ht[CPT][VC] = {};
     int stripe = -1;
+    if (kAhead && ch0 < ch1) {
+        int st = stripe;
+        load_tilde<S, VC, kNTB, CPT>(g, bi, gT, hT, chunk_of(a, ch0, VC, st), lc, gt, ht);
+    }
     for (int ch = ch0; ch < ch1; ++ch) {
         const Chunk ck = chunk_of(a, ch, VC, stripe);
-        stage_bar<S, VC, kNTB, CPT>(g, a, bi, gT, hT, dt, ck, lc, cex, fb, ft);
+        stage_bar<S, VC, kNTB, CPT, kAhead>(g, a, bi, gT, hT, dt, ck, lc, gt, ht, cex, fb, ft);
         __syncthreads();
         stage_grad<S, D, VC, kNTB>(a, bi, ck.nvv, fb, gr);
         __syncthreads();
@@ -778,6 +816,10 @@ __global__ void __launch_bounds__(kNTB, kWavesB)
                 const double w = g.w[v];
                 add_moments<D>(mom, w * (double)sg, w * (double)sh, g.xi[v]);  // deviations
             }
+        }
+        if (kAhead && ch + 1 < ch1) {
+            int st = stripe;
+            load_tilde<S, VC, kNTB, CPT>(g, bi, gT, hT, chunk_of(a, ch + 1, VC, st), lc, gt, ht);
         }
         __syncthreads();
     }

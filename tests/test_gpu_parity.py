@@ -186,6 +186,40 @@ def test_small_sphere_mixed_matches_oracle():
         assert e[k] < 1e-5, (k, e[k])
 
 
+def test_full_size_cavity_matches_oracle():
+    """The 256^2 cavity with the 64^2 trapezoid set on [-5, 5] (Kn 0.1) at full size, 3 steps
+    from the same state: both default schedules (FP64: stored face values, 1e-10; FP32
+    storage: recomputed rings, 1e-5 in rho, U, T) against one oracle run, time steps
+    included."""
+    from dugks_amd import hip
+    from oracle import Oracle
+    c = _cases().config1(0.1)
+    o = Oracle(c)
+    ho = []
+    for _ in range(3):
+        o.step()
+        ho.append(o.dt)
+    ho = np.array(ho)
+    mo, fo = o.cell_macros(), o.face_macros()
+    wo = o.wall_rho()
+    del o
+    for prec, tol in ((hip.FP64, 1e-10), (hip.MIXED, 1e-5)):
+        g = hip.FvDVM(c, precision=prec)
+        hg = g.run(3, record_dt=True)
+        mg, fg = g.cell_macros(), g.face_macros()
+        e = {k: field_rel_err(mg[k], mo[k]) for k in ("rho", "U", "T", "q")}
+        ef = {k: field_rel_err(fg[k], fo[k]) for k in ("rho", "U", "T", "q")}
+        print("full-size cavity", prec, g.path_info()["fused"], e, ef, flush=True)
+        assert np.max(np.abs(hg - ho) / ho) < tol
+        for k in ("rho", "U", "T"):
+            assert e[k] < tol, (prec, k, e[k])
+            assert ef[k] < tol, (prec, k, ef[k])
+        if prec == hip.FP64:
+            assert e["q"] < tol and ef["q"] < tol
+            assert field_rel_err(g.boundary_rho(), wo) < tol
+        g.close()
+
+
 def test_courant_limit_for_long_runs():
     """dugksFoam's Courant rule with maxCo 0.8 lets diagonal velocities exceed the unsplit
     reconstruction's stability bound (sum_d |xi_d| dt/dx <= 1): a 128^2 cavity with the
