@@ -319,20 +319,29 @@ __device__ inline Chunk chunk_of(const FArgs &a, int ch, int vc, int &stripe)
     return c;
 }
 
-// f-bar-plus of cell c for the chunk's velocities (FP64), x-factors cex of the stripe kept
-// by the caller
+// g~, h~ of cell c for the chunk's velocities (velocities past the end of a short stripe
+// repeat its last one)
 template <class S, int VC>
-__device__ inline void cell_bar(const Geo &g, const FArgs &a, int c, const S *__restrict__ gT,
-                                const S *__restrict__ hT, double dt, const Chunk &ck, const d3 &xr, double *cex,
-                                S *gt, S *ht, double *gb, double *hb)
+__device__ inline void load_cell(const Geo &g, int c, const S *__restrict__ gT, const S *__restrict__ hT,
+                                 const Chunk &ck, S *gt, S *ht)
 {
-    const EqCoef k = a.cc[c];
 #pragma unroll
     for (int vv = 0; vv < VC; ++vv) {
         const size_t kk = (size_t)(ck.vrow + ck.ix0 + min(vv, ck.nvv - 1)) * g.nC + c;
         gt[vv] = gT[kk];
         ht[vv] = hT[kk];
     }
+}
+
+// f-bar-plus of cell c for the chunk's velocities (FP64), x-factors cex of the stripe kept
+// by the caller; LOAD: g~, h~ are loaded here, else gt, ht hold them already
+template <class S, int VC, bool LOAD = true>
+__device__ inline void cell_bar(const Geo &g, const FArgs &a, int c, const S *__restrict__ gT,
+                                const S *__restrict__ hT, double dt, const Chunk &ck, const d3 &xr, double *cex,
+                                S *gt, S *ht, double *gb, double *hb)
+{
+    const EqCoef k = a.cc[c];
+    if (LOAD) load_cell<S, VC>(g, c, gT, hT, ck, gt, ht);
     if (ck.new_stripe) stripe_factors<VC>(g, k, ck.ix0, a.n1, cex);
     const double eyz = row_factor(k, xr);
     const double af = 1.5 * dt / (2.0 * k.tr + dt);
@@ -352,11 +361,13 @@ __device__ inline void cell_bar(const Geo &g, const FArgs &a, int c, const S *__
     }
 }
 
-// pass A stage 1: f-bar-plus of the local cells (block + ring) into LDS
-template <class S, int VC, int NT, int CPT, bool D3>
+// pass A stage 1: f-bar-plus of the local cells (block + ring) into LDS.  PRE: g~, h~ of
+// the thread's cells were loaded into pg, ph beforehand, else each cell loads its own
+template <class S, int VC, int NT, int CPT, bool D3, bool PRE>
 __device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &bi, const S *__restrict__ gT,
                                  const S *__restrict__ hT, double dt, const Chunk &ck, const int (&lc)[CPT],
-                                 double (&cex)[CPT][VC], P2<S> *fb)
+                                 const S (&pg)[CPT][VC], const S (&ph)[CPT][VC], double (&cex)[CPT][VC],
+                                 P2<S> *fb)
 {
     const d3 xr = g.xi[ck.vrow];
 #pragma unroll
@@ -367,7 +378,14 @@ __device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &
         double gb[VC], hb[VC];
         // the hoisted cell ids cost FP64 3-D pass A registers it does not have (spills)
         const int c = (sizeof(S) == 8 && D3) ? a.lcell[bi.l_off + l] : lc[s];
-        cell_bar<S, VC>(g, a, c, gT, hT, dt, ck, xr, cex[s], gt, ht, gb, hb);
+        if (PRE) {
+#pragma unroll
+            for (int vv = 0; vv < VC; ++vv) {
+                gt[vv] = pg[s][vv];
+                ht[vv] = ph[s][vv];
+            }
+        }
+        cell_bar<S, VC, !PRE>(g, a, c, gT, hT, dt, ck, xr, cex[s], gt, ht, gb, hb);
 #pragma unroll
         for (int vv = 0; vv < VC; ++vv) {
             if (vv >= ck.nvv) break;
@@ -513,10 +531,21 @@ __global__ void __launch_bounds__(kNTA)
     }
     int2 gk[1];  // block cells <= 256 < kNTA: one gradient cell per thread
     grad_ranges<kNTA, 1>(a, bi, bi.nc, gk);
+    // 2-D: the next chunk's g~, h~ are loaded before the chunk's last barrier, so their
+    // latency overlaps the barrier wait (3-D has no registers to spare)
+    constexpr bool kAhead = D == 2;
+    S gt[CPT][VC] = {}, ht[CPT][VC] = {};
     int stripe = -1;
+    if (kAhead && ch0 < ch1) {
+        int st = stripe;
+        const Chunk c0 = chunk_of(a, ch0, VC, st);
+#pragma unroll
+        for (int s = 0; s < CPT; ++s)
+            if (threadIdx.x + s * kNTA < bi.nl) load_cell<S, VC>(g, lc[s], gT, hT, c0, gt[s], ht[s]);
+    }
     for (int ch = ch0; ch < ch1; ++ch) {
         const Chunk ck = chunk_of(a, ch, VC, stripe);
-        stage_bar<S, VC, kNTA, CPT, D == 3>(g, a, bi, gT, hT, dt, ck, lc, cex, fb);
+        stage_bar<S, VC, kNTA, CPT, D == 3, kAhead>(g, a, bi, gT, hT, dt, ck, lc, gt, ht, cex, fb);
         __syncthreads();
         stage_grad<S, D, VC, kNTA, 1>(a, bi, ck.nvv, gk, fb, gr);
         __syncthreads();
@@ -587,6 +616,13 @@ __global__ void __launch_bounds__(kNTA)
                 row_add(rs, w * gf, w * hf, xi.x);
             }
             fold_row<D>(acc[s], rs, xr.y, xr.z);
+        }
+        if (kAhead && ch + 1 < ch1) {
+            int st = stripe;
+            const Chunk nx = chunk_of(a, ch + 1, VC, st);
+#pragma unroll
+            for (int s = 0; s < CPT; ++s)
+                if (threadIdx.x + s * kNTA < bi.nl) load_cell<S, VC>(g, lc[s], gT, hT, nx, gt[s], ht[s]);
         }
         __syncthreads();
     }
