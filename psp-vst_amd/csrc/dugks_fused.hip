@@ -333,14 +333,13 @@ __device__ inline void load_cell(const Geo &g, int c, const S *__restrict__ gT, 
     }
 }
 
-// f-bar-plus of cell c for the chunk's velocities (FP64), x-factors cex of the stripe kept
-// by the caller; LOAD: g~, h~ are loaded here, else gt, ht hold them already
+// f-bar-plus of cell c (equilibrium record k) for the chunk's velocities (FP64), x-factors
+// cex of the stripe kept by the caller; LOAD: g~, h~ are loaded here, else gt, ht hold them
 template <class S, int VC, bool LOAD = true>
-__device__ inline void cell_bar(const Geo &g, const FArgs &a, int c, const S *__restrict__ gT,
+__device__ inline void cell_bar(const Geo &g, const FArgs &a, const EqCoef &k, int c, const S *__restrict__ gT,
                                 const S *__restrict__ hT, double dt, const Chunk &ck, const d3 &xr, double *cex,
                                 S *gt, S *ht, double *gb, double *hb)
 {
-    const EqCoef k = a.cc[c];
     if (LOAD) load_cell<S, VC>(g, c, gT, hT, ck, gt, ht);
     if (ck.new_stripe) stripe_factors<VC>(g, k, ck.ix0, a.n1, cex);
     const double eyz = row_factor(k, xr);
@@ -362,12 +361,13 @@ __device__ inline void cell_bar(const Geo &g, const FArgs &a, int c, const S *__
 }
 
 // pass A stage 1: f-bar-plus of the local cells (block + ring) into LDS.  PRE: g~, h~ of
-// the thread's cells were loaded into pg, ph beforehand, else each cell loads its own
-template <class S, int VC, int NT, int CPT, bool D3, bool PRE>
+// the thread's cells were loaded into pg, ph beforehand, else each cell loads its own.
+// HK: the cells' equilibrium records are held in kc for the kernel, else loaded per chunk
+template <class S, int VC, int NT, int CPT, bool D3, bool PRE, bool HK>
 __device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &bi, const S *__restrict__ gT,
                                  const S *__restrict__ hT, double dt, const Chunk &ck, const int (&lc)[CPT],
-                                 const S (&pg)[CPT][VC], const S (&ph)[CPT][VC], double (&cex)[CPT][VC],
-                                 P2<S> *fb)
+                                 const S (&pg)[CPT][VC], const S (&ph)[CPT][VC], const EqCoef (&kc)[CPT],
+                                 double (&cex)[CPT][VC], P2<S> *fb)
 {
     const d3 xr = g.xi[ck.vrow];
 #pragma unroll
@@ -385,7 +385,12 @@ __device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &
                 ht[vv] = ph[s][vv];
             }
         }
-        cell_bar<S, VC, !PRE>(g, a, c, gT, hT, dt, ck, xr, cex[s], gt, ht, gb, hb);
+        if (HK) {
+            cell_bar<S, VC, !PRE>(g, a, kc[s], c, gT, hT, dt, ck, xr, cex[s], gt, ht, gb, hb);
+        } else {
+            const EqCoef k = a.cc[c];
+            cell_bar<S, VC, !PRE>(g, a, k, c, gT, hT, dt, ck, xr, cex[s], gt, ht, gb, hb);
+        }
 #pragma unroll
         for (int vv = 0; vv < VC; ++vv) {
             if (vv >= ck.nvv) break;
@@ -535,6 +540,14 @@ __global__ void __launch_bounds__(kNTA)
     // latency overlaps the barrier wait (3-D has no registers to spare)
     constexpr bool kAhead = D == 2;
     S gt[CPT][VC] = {}, ht[CPT][VC] = {};
+    // 2-D: the local cells' equilibrium records are read once per kernel, not per chunk
+    constexpr bool kHoldK = D == 2;
+    EqCoef kc[CPT] = {};
+    if (kHoldK) {
+#pragma unroll
+        for (int s = 0; s < CPT; ++s)
+            if (threadIdx.x + s * kNTA < bi.nl) kc[s] = a.cc[lc[s]];
+    }
     int stripe = -1;
     if (kAhead && ch0 < ch1) {
         int st = stripe;
@@ -545,7 +558,7 @@ __global__ void __launch_bounds__(kNTA)
     }
     for (int ch = ch0; ch < ch1; ++ch) {
         const Chunk ck = chunk_of(a, ch, VC, stripe);
-        stage_bar<S, VC, kNTA, CPT, D == 3, kAhead>(g, a, bi, gT, hT, dt, ck, lc, gt, ht, cex, fb);
+        stage_bar<S, VC, kNTA, CPT, D == 3, kAhead, kHoldK>(g, a, bi, gT, hT, dt, ck, lc, gt, ht, kc, cex, fb);
         __syncthreads();
         stage_grad<S, D, VC, kNTA, 1>(a, bi, ck.nvv, gk, fb, gr);
         __syncthreads();
@@ -766,7 +779,10 @@ __global__ void __launch_bounds__(kNTB)
         // stage 1: f-bar-plus of the thread's cell
         S gt[VC], ht[VC];
         double gb[VC], hb[VC];
-        if (has_cell) cell_bar<S, VC>(g, a, cell, gT, hT, dt, ck, xr, cex, gt, ht, gb, hb);
+        if (has_cell) {
+            const EqCoef k = a.cc[cell];
+            cell_bar<S, VC>(g, a, k, cell, gT, hT, dt, ck, xr, cex, gt, ht, gb, hb);
+        }
         // stage 2: face values -> flux sigma * f_f in LDS
 #pragma unroll
         for (int s = 0; s < FPT; ++s) {
