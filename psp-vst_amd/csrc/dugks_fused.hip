@@ -791,10 +791,11 @@ __global__ void __launch_bounds__(kNTB)
             const int f = a.fs_face[bi.fs_off + j];
             // pass A wrote the upwind value into the slots of both blocks of a shared face
             const int own_slot = bi.fs_off + j;
-            // 2-D: per-slot copies, loads that do not wait on the face label (in 3-D the copies
-            // cost registers that push pass B to one wave per SIMD: 131 -> 196 ms on the sphere)
+            // per-slot copies: loads that do not wait on the face label (2-D; the records in 3-D
+            // with FP32 storage; the other 3-D copies cost registers that push pass B to one
+            // wave per SIMD: 131 -> 196 ms on the sphere)
             const d3 Sf = D == 2 ? a.fs_sf[own_slot] : g.Sf[f];
-            const EqCoef k = D == 2 ? a.fs_coef[own_slot] : a.fc[f];
+            const EqCoef k = (D == 2 || sizeof(S) == 4) ? a.fs_coef[own_slot] : a.fc[f];
             if (ck.new_stripe) stripe_factors<VC>(g, k, ck.ix0, a.n1, fex[s]);
             const double eyz = row_factor(k, xr);
             const int b = f - g.nIF;
@@ -1189,7 +1190,9 @@ void Fused<S>::face_macros(const Geo &g, FusedPlan &p, Macros fm, double *rho_b,
     else
         hipLaunchKernelGGL(k_face_macros<2>, grid1(g.nF), dim3(kBlk), 0, st, g, fargs(p), fm, rho_b, dt);
     hipLaunchKernelGGL(k_face_coef, grid1(g.nF), dim3(kBlk), 0, st, g, fm, p.fc, dt);
-    if (g.D == 2) hipLaunchKernelGGL(k_slot_coef, grid1(p.n_slots), dim3(kBlk), 0, st, fargs(p), p.n_slots);
+    // pass B reads the records per face slot (2-D; 3-D with FP32 storage)
+    if (g.D == 2 || sizeof(S) == 4)
+        hipLaunchKernelGGL(k_slot_coef, grid1(p.n_slots), dim3(kBlk), 0, st, fargs(p), p.n_slots);
     launch_cfl_partials(g, fm, cfl_part, st);
 }
 
