@@ -38,7 +38,8 @@ namespace {
 
 constexpr int kNTA = kFusedThreadsA;
 constexpr int kSlotsA = kFusedSlotsA;
-static_assert(kFusedThreadsB <= kFusedThreadsA, "pass A takes one gradient (block) cell per thread");
+static_assert(2 * kFusedThreadsB <= kFusedThreadsA,
+              "pass A can take each gradient (block) cell on two threads, one per half of the chunk");
 constexpr int kNTB = kFusedThreadsB;
 constexpr int kNb = kFusedMaxNb;
 constexpr int kBlk = 256;
@@ -399,33 +400,30 @@ __device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &
     }
 }
 
-// pass A stage 2: gradients of the block cells
-// stencil range (first entry, count) of the thread's gradient cells, fixed for the kernel
-template <int NT, int GPT>
-__device__ inline void grad_ranges(const FArgs &a, const BlockInfo &bi, int ng, int2 (&gk)[GPT])
+// pass A stage 2: gradients of the block cells.  SPLIT: block cell l is taken by threads l
+// and l + NT/2, which split the chunk's velocities (even / odd), so the whole work-group
+// works; otherwise by thread l alone (FP64 3-D: the split costs spills there).
+// Stencil range (first entry, count) of the thread's gradient cell, fixed for the kernel
+template <int NT, bool SPLIT>
+__device__ inline int2 grad_range(const FArgs &a, const BlockInfo &bi)
 {
-#pragma unroll
-    for (int s = 0; s < GPT; ++s) {
-        const int l = threadIdx.x + s * NT;
-        gk[s] = int2{0, 0};
-        if (l < ng) {
-            const int k0 = a.gs_beg[bi.gsb_off + l];
-            gk[s] = int2{k0, a.gs_beg[bi.gsb_off + l + 1] - k0};
-        }
-    }
+    const int l = SPLIT ? threadIdx.x % (NT / 2) : threadIdx.x;
+    if (l >= bi.nc) return int2{0, 0};
+    const int k0 = a.gs_beg[bi.gsb_off + l];
+    return int2{k0, a.gs_beg[bi.gsb_off + l + 1] - k0};
 }
 
-template <class S, int D, int VC, int NT, int GPT>
-__device__ inline void stage_grad(const FArgs &a, const BlockInfo &bi, int nvv, const int2 (&gk)[GPT],
-                                  const P2<S> *fb, P2<S> *gr)
+template <class S, int D, int VC, int NT, bool SPLIT>
+__device__ inline void stage_grad(const FArgs &a, const BlockInfo &bi, int nvv, int2 gk, const P2<S> *fb,
+                                  P2<S> *gr)
 {
     const V3<S> *ls = (const V3<S> *)a.gs_ls;
-#pragma unroll
-    for (int s = 0; s < GPT; ++s) {
-        const int l = threadIdx.x + s * NT;
-        if (l >= bi.nc) break;
-        const int k0 = gk[s].x;
-        const int n = gk[s].y;
+    const int l = SPLIT ? threadIdx.x % (NT / 2) : threadIdx.x;
+    const int v0 = SPLIT ? threadIdx.x / (NT / 2) : 0;
+    constexpr int kStep = SPLIT ? 2 : 1;
+    if (l < bi.nc) {
+        const int k0 = gk.x;
+        const int n = gk.y;
         int nb[kNb];
         V3<S> L[kNb];
 #pragma unroll
@@ -436,7 +434,7 @@ __device__ inline void stage_grad(const FArgs &a, const BlockInfo &bi, int nvv, 
             }
         }
 #pragma unroll 1
-        for (int vv = 0; vv < nvv; ++vv) {
+        for (int vv = v0; vv < nvv; vv += kStep) {
             const P2<S> *p = fb + vv * a.max_nl;
             const P2<S> c = p[l];
             S gx = 0, gy = 0, gz = 0, hx = 0, hy = 0, hz = 0;
@@ -534,8 +532,8 @@ __global__ void __launch_bounds__(kNTA)
         const int l = threadIdx.x + s * kNTA;
         lc[s] = l < bi.nl ? a.lcell[bi.l_off + l] : 0;
     }
-    int2 gk[1];  // block cells <= 256 < kNTA: one gradient cell per thread
-    grad_ranges<kNTA, 1>(a, bi, bi.nc, gk);
+    constexpr bool kSplitGrad = !(D == 3 && sizeof(S) == 8);
+    const int2 gk = grad_range<kNTA, kSplitGrad>(a, bi);
     // 2-D: the next chunk's g~, h~ are loaded before the chunk's last barrier, so their
     // latency overlaps the barrier wait (3-D has no registers to spare)
     constexpr bool kAhead = D == 2;
@@ -560,7 +558,7 @@ __global__ void __launch_bounds__(kNTA)
         const Chunk ck = chunk_of(a, ch, VC, stripe);
         stage_bar<S, VC, kNTA, CPT, D == 3, kAhead, kHoldK>(g, a, bi, gT, hT, dt, ck, lc, gt, ht, kc, cex, fb);
         __syncthreads();
-        stage_grad<S, D, VC, kNTA, 1>(a, bi, ck.nvv, gk, fb, gr);
+        stage_grad<S, D, VC, kNTA, kSplitGrad>(a, bi, ck.nvv, gk, fb, gr);
         __syncthreads();
         const d3 xr = g.xi[ck.vrow];
 #pragma unroll

@@ -386,49 +386,62 @@ __device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &
     }
 }
 
-// stage 2: gradients of block + ring-1 cells
+// gradient of local cell l for the chunk velocities [v0, v1)
+template <class S, int D>
+__device__ inline void grad_cell(const FArgs &a, const BlockInfo &bi, int l, int v0, int v1, const P2<S> *fb,
+                                 P2<S> *gr)
+{
+    const V3<S> *ls = (const V3<S> *)a.gs_ls;
+    const int k0 = (bi.gsb_off + l) * kNb;
+    const int n = a.gs_cnt[bi.gsb_off + l];
+    // the first NU entries (faces of a quad / hex) are read without waiting for the count;
+    // padded entries are local cell 0 with a zero vector and are not used
+    constexpr int NU = D == 2 ? 4 : 6;
+    int nb[kNb];
+    V3<S> L[kNb];
+#pragma unroll
+    for (int j = 0; j < kNb; ++j) {
+        if (j < NU || j < n) {
+            nb[j] = a.gs_nb[k0 + j];
+            L[j] = ls[k0 + j];
+        }
+    }
+#pragma unroll 1
+    for (int vv = v0; vv < v1; ++vv) {
+        const P2<S> *p = fb + vv * a.max_nl;
+        const P2<S> c = p[l];
+        S gx = 0, gy = 0, gz = 0, hx = 0, hy = 0, hz = 0;
+#pragma unroll
+        for (int j = 0; j < kNb; ++j) {
+            if (j < n) {
+                const P2<S> q = p[nb[j]];
+                const S dg = q.g - c.g, dh = q.h - c.h;
+                gx += L[j].x * dg; gy += L[j].y * dg;
+                hx += L[j].x * dh; hy += L[j].y * dh;
+                if (D == 3) {
+                    gz += L[j].z * dg;
+                    hz += L[j].z * dh;
+                }
+            }
+        }
+        P2<S> *o = gr + (vv * a.max_ng + l) * D;
+        o[0] = P2<S>{gx, hx};
+        o[1] = P2<S>{gy, hy};
+        if (D == 3) o[2] = P2<S>{gz, hz};
+    }
+}
+
+// stage 2: gradients of block + ring-1 cells.  Cells l < NT take one thread each for all
+// chunk velocities; the (cell, velocity) pairs of the cells beyond are dealt over all
+// threads, so no thread does two whole cells while others wait at the barrier
 template <class S, int D, int VC, int NT>
 __device__ inline void stage_grad(const FArgs &a, const BlockInfo &bi, int nvv, const P2<S> *fb, P2<S> *gr)
 {
-    const V3<S> *ls = (const V3<S> *)a.gs_ls;
-    for (int l = threadIdx.x; l < bi.ng; l += NT) {
-        const int k0 = (bi.gsb_off + l) * kNb;
-        const int n = a.gs_cnt[bi.gsb_off + l];
-        // the first NU entries (faces of a quad / hex) are read without waiting for the count;
-        // padded entries are local cell 0 with a zero vector and are not used
-        constexpr int NU = D == 2 ? 4 : 6;
-        int nb[kNb];
-        V3<S> L[kNb];
-#pragma unroll
-        for (int j = 0; j < kNb; ++j) {
-            if (j < NU || j < n) {
-                nb[j] = a.gs_nb[k0 + j];
-                L[j] = ls[k0 + j];
-            }
-        }
-#pragma unroll 1
-        for (int vv = 0; vv < nvv; ++vv) {
-            const P2<S> *p = fb + vv * a.max_nl;
-            const P2<S> c = p[l];
-            S gx = 0, gy = 0, gz = 0, hx = 0, hy = 0, hz = 0;
-#pragma unroll
-            for (int j = 0; j < kNb; ++j) {
-                if (j < n) {
-                    const P2<S> q = p[nb[j]];
-                    const S dg = q.g - c.g, dh = q.h - c.h;
-                    gx += L[j].x * dg; gy += L[j].y * dg;
-                    hx += L[j].x * dh; hy += L[j].y * dh;
-                    if (D == 3) {
-                        gz += L[j].z * dg;
-                        hz += L[j].z * dh;
-                    }
-                }
-            }
-            P2<S> *o = gr + (vv * a.max_ng + l) * D;
-            o[0] = P2<S>{gx, hx};
-            o[1] = P2<S>{gy, hy};
-            if (D == 3) o[2] = P2<S>{gz, hz};
-        }
+    if ((int)threadIdx.x < bi.ng) grad_cell<S, D>(a, bi, threadIdx.x, 0, nvv, fb, gr);
+    const int rest = (bi.ng - NT) * nvv;
+    for (int i = threadIdx.x; i < rest; i += NT) {
+        const int vv = i % nvv;
+        grad_cell<S, D>(a, bi, NT + i / nvv, vv, vv + 1, fb, gr);
     }
 }
 
