@@ -731,6 +731,35 @@ __device__ inline double face_value(S stored, const double *ref, int v)
     return (double)stored;
 }
 
+// flux sigma * f of velocity v through face slot own_slot (face f, area vector Sf): the
+// stored face value relaxed towards the face equilibrium (fvDVM.C:1362-1460), or the
+// incoming half of a wall / inlet face; gM is the Maxwellian factor A exp(-|xi - U|^2 / 2RT)
+// of the face record k.  Tangent velocities of internal faces carry no flux.
+template <class S>
+__device__ inline P2<double> slot_flux(const Geo &g, const FArgs &a, const P2<S> *__restrict__ F, int own_slot,
+                                       int f, int v, const d3 &Sf, const EqCoef &k, double gM, const double *rho_b)
+{
+    const d3 xi = g.xi[v];
+    const double xs = sigma_dot(xi, Sf);
+    const int b = f - g.nIF;
+    if (b < 0 ? (xs >= kVSmall || xs < -kVSmall) : xs > 0) {
+        const size_t kf = (size_t)v * a.n_slots + own_slot;
+        double gEq, hEq;
+        shakhov_poly(g, k, xi, gM, gEq, hEq);
+        const P2<S> fv = F[kf];
+        return P2<double>{xs * ((1.0 - k.tr) * face_value<S>(fv.g, g.ref_g, v) + k.tr * gEq),
+                          xs * ((1.0 - k.tr) * face_value<S>(fv.h, g.ref_h, v) + k.tr * hEq)};
+    }
+    if (b >= 0) {  // incoming half of a wall or inlet face
+        const BndCoef kb = a.bc[b];
+        const double scale = g.bf_type[b] == kMaxwellWall ? rho_b[b] : 1.0;
+        const double cx = xi.x - kb.ux, cy = xi.y - kb.uy, cz = xi.z - kb.uz;
+        const double gv = scale * (kb.A * exp(-((cx * cx + cy * cy + cz * cz) * kb.invRT) / 2.0));
+        return P2<double>{xs * gv, xs * (gv * kb.hfac)};
+    }
+    return P2<double>{0.0, 0.0};
+}
+
 template <class S, int D, int VC>
 __global__ void __launch_bounds__(kNTB)
     k_pass_b(Geo g, FArgs a, S *gT, S *hT, const P2<S> *__restrict__ F, const double *rho_b,
@@ -769,7 +798,8 @@ __global__ void __launch_bounds__(kNTB)
         }
     }
 
-    double cex[VC], fex[FPT][VC];
+    constexpr int FW = FPT - 1;  // whole faces per thread in stage 2
+    double cex[VC], fex[FW][VC];
     int stripe = -1;
     for (int ch = ch0; ch < ch1; ++ch) {
         const Chunk ck = chunk_of(a, ch, VC, stripe);
@@ -781,45 +811,46 @@ __global__ void __launch_bounds__(kNTB)
             const EqCoef k = a.cc[cell];
             cell_bar<S, VC>(g, a, k, cell, gT, hT, dt, ck, xr, cex, gt, ht, gb, hb);
         }
-        // stage 2: face values -> flux sigma * f_f in LDS
+        // stage 2: face values -> flux sigma * f_f in LDS.  Slots j < FW * kNTB: whole faces,
+        // FW per thread, x-factors kept for the stripe; the (slot, velocity) pairs of the
+        // slots beyond are dealt over all threads, so no thread takes one whole face more
+        // than the others while they wait at the barrier.
+        // Per-slot copies are loads that do not wait on the face label (2-D; the records in
+        // 3-D with FP32 storage; the other 3-D copies cost registers that push pass B to one
+        // wave per SIMD: 131 -> 196 ms on the sphere).  Pass A wrote the upwind value into
+        // the slots of both blocks of a shared face.
 #pragma unroll
-        for (int s = 0; s < FPT; ++s) {
+        for (int s = 0; s < FW; ++s) {
             const int j = threadIdx.x + s * kNTB;
             if (j >= bi.nfs) break;
-            const int f = a.fs_face[bi.fs_off + j];
-            // pass A wrote the upwind value into the slots of both blocks of a shared face
             const int own_slot = bi.fs_off + j;
-            // per-slot copies: loads that do not wait on the face label (2-D; the records in 3-D
-            // with FP32 storage; the other 3-D copies cost registers that push pass B to one
-            // wave per SIMD: 131 -> 196 ms on the sphere)
+            const int f = a.fs_face[own_slot];
             const d3 Sf = D == 2 ? a.fs_sf[own_slot] : g.Sf[f];
             const EqCoef k = (D == 2 || sizeof(S) == 4) ? a.fs_coef[own_slot] : a.fc[f];
             if (ck.new_stripe) stripe_factors<VC>(g, k, ck.ix0, a.n1, fex[s]);
             const double eyz = row_factor(k, xr);
-            const int b = f - g.nIF;
 #pragma unroll
             for (int vv = 0; vv < VC; ++vv) {
                 if (vv >= ck.nvv) break;
-                const int v = ck.vrow + ck.ix0 + vv;
-                const d3 xi = g.xi[v];
-                const double xs = sigma_dot(xi, Sf);
-                double fg = 0.0, fh = 0.0;
-                if (b < 0 ? (xs >= kVSmall || xs < -kVSmall) : xs > 0) {
-                    const size_t kf = (size_t)v * a.n_slots + own_slot;
-                    double gEq, hEq;
-                    shakhov_poly(g, k, xi, fex[s][vv] * eyz, gEq, hEq);
-                    const P2<S> fv = F[kf];
-                    fg = xs * ((1.0 - k.tr) * face_value<S>(fv.g, g.ref_g, v) + k.tr * gEq);
-                    fh = xs * ((1.0 - k.tr) * face_value<S>(fv.h, g.ref_h, v) + k.tr * hEq);
-                } else if (b >= 0) {  // incoming half of a wall or inlet face
-                    const BndCoef kb = a.bc[b];
-                    const double scale = g.bf_type[b] == kMaxwellWall ? rho_b[b] : 1.0;
-                    const double cx = xi.x - kb.ux, cy = xi.y - kb.uy, cz = xi.z - kb.uz;
-                    const double gv = scale * (kb.A * exp(-((cx * cx + cy * cy + cz * cz) * kb.invRT) / 2.0));
-                    fg = xs * gv;
-                    fh = xs * (gv * kb.hfac);
-                }
-                fl[vv * a.max_nfs + j] = P2<double>{fg, fh};
+                fl[vv * a.max_nfs + j] = slot_flux<S>(g, a, F, own_slot, f, ck.vrow + ck.ix0 + vv, Sf, k,
+                                                      fex[s][vv] * eyz, rho_b);
+            }
+        }
+        {
+            const int rest = (bi.nfs - FW * kNTB) * ck.nvv;
+            for (int i = threadIdx.x; i < rest; i += kNTB) {
+                const int q = ck.nvv == VC ? i / VC : i / ck.nvv;
+                const int vv = i - q * ck.nvv;
+                const int j = FW * kNTB + q;
+                const int own_slot = bi.fs_off + j;
+                const int f = a.fs_face[own_slot];
+                const d3 Sf = D == 2 ? a.fs_sf[own_slot] : g.Sf[f];
+                const EqCoef k = (D == 2 || sizeof(S) == 4) ? a.fs_coef[own_slot] : a.fc[f];
+                // the x-factor of this velocity alone, formed as stripe_factors forms it
+                const double dx = g.xi[ck.ix0 + vv].x - k.ux;
+                const double ex = k.A * exp(-((dx * dx) * k.invRT) / 2.0);
+                fl[vv * a.max_nfs + j] = slot_flux<S>(g, a, F, own_slot, f, ck.vrow + ck.ix0 + vv, Sf, k,
+                                                      ex * row_factor(k, xr), rho_b);
             }
         }
         __syncthreads();

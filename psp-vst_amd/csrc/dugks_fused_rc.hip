@@ -665,6 +665,58 @@ __global__ void __launch_bounds__(kBlk) k_slot_coef(FArgs a, int n_slots)
 }
 
 // ------------------------------------------------------------------------- pass B
+// flux sigma * f of velocity v through block face f (local owner lo, neighbour ln or -1):
+// upwind reconstruction at x_f - xi dt/2 relaxed towards the face equilibrium
+// (fvDVM.C:1362-1460), or the incoming half of a wall / inlet face; gM is the Maxwellian
+// factor A exp(-|xi - U|^2 / 2RT) of the face record k; pf, pq: the velocity's LDS rows
+template <class S, int D>
+__device__ inline P2<double> face_flux(const Geo &g, const FArgs &a, const P2<S> *pf, const P2<S> *pq, int v,
+                                       int f, int lo, int ln, const FaceGeo &fgeo, const EqCoef &k, double gM,
+                                       double hx, const double *rho_b)
+{
+    const d3 xi = g.xi[v];
+    const double xs = sigma_dot(xi, fgeo.Sf);
+    const double ox = fgeo.ao.x - xi.x * hx, oy = fgeo.ao.y - xi.y * hx, oz = fgeo.ao.z - xi.z * hx;
+    const int b = f - g.nIF;
+    double gv, hv;
+    if (b < 0) {
+        const double nx = fgeo.an.x - xi.x * hx, ny = fgeo.an.y - xi.y * hx, nz = fgeo.an.z - xi.z * hx;
+        if (xs >= kVSmall) {
+            const P2<S> r = recon_l<S, D>(pf, pq, lo, ox, oy, oz);
+            gv = (double)r.g;
+            hv = (double)r.h;
+        } else if (xs < -kVSmall) {
+            const P2<S> r = recon_l<S, D>(pf, pq, ln, nx, ny, nz);
+            gv = (double)r.g;
+            hv = (double)r.h;
+        } else {
+            const P2<S> rn = recon_l<S, D>(pf, pq, ln, nx, ny, nz);
+            const P2<S> ro = recon_l<S, D>(pf, pq, lo, ox, oy, oz);
+            gv = 0.5 * ((double)rn.g + (double)ro.g);
+            hv = 0.5 * ((double)rn.h + (double)ro.h);
+        }
+    } else if (xs > 0) {
+        const P2<S> r = recon_l<S, D>(pf, pq, lo, ox, oy, oz);
+        gv = (double)r.g;
+        hv = (double)r.h;
+    } else {
+        const BndCoef kb = a.bc[b];
+        const double scale = g.bf_type[b] == kMaxwellWall ? rho_b[b] : 1.0;
+        const double cx = xi.x - kb.ux, cy = xi.y - kb.uy, cz = xi.z - kb.uz;
+        gv = scale * (kb.A * exp(-((cx * cx + cy * cy + cz * cz) * kb.invRT) / 2.0));
+        return P2<double>{xs * gv, xs * (gv * kb.hfac)};
+    }
+    if (sizeof(S) == 4) {  // reconstructed deviation -> value
+        gv += g.ref_g[v];
+        hv += g.ref_h[v];
+    }
+    double gEq, hEq;
+    shakhov_poly(g, k, xi, gM, gEq, hEq);
+    gv = (1.0 - k.tr) * gv + k.tr * gEq;
+    hv = (1.0 - k.tr) * hv + k.tr * hEq;
+    return P2<double>{xs * gv, xs * hv};
+}
+
 template <class S, int D, int VC>
 __global__ void __launch_bounds__(kNTB, kWavesB)
     k_pass_b(Geo g, FArgs a, const S *__restrict__ gT, const S *__restrict__ hT, S *__restrict__ gN,
@@ -708,7 +760,8 @@ __global__ void __launch_bounds__(kNTB, kWavesB)
         }
     }
 
-    double cex[CPT][VC], fex[FPT][VC];
+    constexpr int FW = FPT - 1;  // whole faces per thread in stage 3
+    double cex[CPT][VC], fex[FW][VC];
     int lc[CPT];
 #pragma unroll
     for (int s = 0; s < CPT; ++s) {
@@ -731,71 +784,45 @@ __global__ void __launch_bounds__(kNTB, kWavesB)
         __syncthreads();
         stage_grad<S, D, VC, kNTB>(a, bi, ck.nvv, fb, gr);
         __syncthreads();
-        // stage 3: face values -> flux sigma * f_f in LDS
+        // stage 3: face values -> flux sigma * f_f in LDS.  Faces j < FW * kNTB: whole faces,
+        // FW per thread, x-factors kept for the stripe; the (face, velocity) pairs of the
+        // faces beyond are dealt over all threads, so no thread takes a third whole face
+        // while the others wait at the barrier
         const d3 xr = g.xi[ck.vrow];
 #pragma unroll
-        for (int s = 0; s < FPT; ++s) {
+        for (int s = 0; s < FW; ++s) {
             const int j = threadIdx.x + s * kNTB;
             if (j >= bi.nfb) break;
+            const FaceGeo fgeo = a.fb_geo[bi.fb_off + j];
+            const EqCoef k = a.fb_coef[bi.fb_off + j];
             const int f = a.fb_face[bi.fb_off + j];
             const int lo = a.fb_own[bi.fb_off + j], ln = a.fb_nei[bi.fb_off + j];
-            const FaceGeo fgeo = a.fb_geo[bi.fb_off + j];
-            const d3 Sf = fgeo.Sf;
-            const double aox = fgeo.ao.x, aoy = fgeo.ao.y, aoz = fgeo.ao.z;
-            const double anx = fgeo.an.x, any = fgeo.an.y, anz = fgeo.an.z;
-            const EqCoef k = a.fb_coef[bi.fb_off + j];
             if (ck.new_stripe) stripe_factors<VC>(g, k, ck.ix0, a.n1, fex[s]);
             const double eyz = row_factor(k, xr);
-            const int b = f - g.nIF;
 #pragma unroll
             for (int vv = 0; vv < VC; ++vv) {
                 if (vv >= ck.nvv) break;
-                const d3 xi = g.xi[ck.vrow + ck.ix0 + vv];
-                const double xs = sigma_dot(xi, Sf);
-                const P2<S> *pf = fb + vv * a.max_nl, *pq = gr + vv * a.max_ng * D;
-                const double ox = aox - xi.x * hx, oy = aoy - xi.y * hx, oz = aoz - xi.z * hx;
-                double gv, hv;
-                bool relax = true;
-                if (b < 0) {
-                    const double nx = anx - xi.x * hx, ny = any - xi.y * hx, nz = anz - xi.z * hx;
-                    if (xs >= kVSmall) {
-                        const P2<S> r = recon_l<S, D>(pf, pq, lo, ox, oy, oz);
-                        gv = (double)r.g;
-                        hv = (double)r.h;
-                    } else if (xs < -kVSmall) {
-                        const P2<S> r = recon_l<S, D>(pf, pq, ln, nx, ny, nz);
-                        gv = (double)r.g;
-                        hv = (double)r.h;
-                    } else {
-                        const P2<S> rn = recon_l<S, D>(pf, pq, ln, nx, ny, nz);
-                        const P2<S> ro = recon_l<S, D>(pf, pq, lo, ox, oy, oz);
-                        gv = 0.5 * ((double)rn.g + (double)ro.g);
-                        hv = 0.5 * ((double)rn.h + (double)ro.h);
-                    }
-                } else if (xs > 0) {
-                    const P2<S> r = recon_l<S, D>(pf, pq, lo, ox, oy, oz);
-                    gv = (double)r.g;
-                    hv = (double)r.h;
-                } else {
-                    relax = false;
-                    const BndCoef kb = a.bc[b];
-                    const double scale = g.bf_type[b] == kMaxwellWall ? rho_b[b] : 1.0;
-                    const double cx = xi.x - kb.ux, cy = xi.y - kb.uy, cz = xi.z - kb.uz;
-                    gv = scale * (kb.A * exp(-((cx * cx + cy * cy + cz * cz) * kb.invRT) / 2.0));
-                    hv = gv * kb.hfac;
-                }
-                if (relax) {
-                    if (sizeof(S) == 4) {  // reconstructed deviation -> value
-                        const int v = ck.vrow + ck.ix0 + vv;
-                        gv += g.ref_g[v];
-                        hv += g.ref_h[v];
-                    }
-                    double gEq, hEq;
-                    shakhov_poly(g, k, xi, fex[s][vv] * eyz, gEq, hEq);
-                    gv = (1.0 - k.tr) * gv + k.tr * gEq;
-                    hv = (1.0 - k.tr) * hv + k.tr * hEq;
-                }
-                fl[vv * a.max_nfb + j] = P2<double>{xs * gv, xs * hv};
+                fl[vv * a.max_nfb + j] = face_flux<S, D>(g, a, fb + vv * a.max_nl, gr + vv * a.max_ng * D,
+                                                        ck.vrow + ck.ix0 + vv, f, lo, ln, fgeo, k,
+                                                        fex[s][vv] * eyz, hx, rho_b);
+            }
+        }
+        {
+            const int rest = (bi.nfb - FW * kNTB) * ck.nvv;
+            for (int i = threadIdx.x; i < rest; i += kNTB) {
+                const int q = ck.nvv == VC ? i / VC : i / ck.nvv;
+                const int vv = i - q * ck.nvv;
+                const int j = FW * kNTB + q;
+                const FaceGeo fgeo = a.fb_geo[bi.fb_off + j];
+                const EqCoef k = a.fb_coef[bi.fb_off + j];
+                const int f = a.fb_face[bi.fb_off + j];
+                const int lo = a.fb_own[bi.fb_off + j], ln = a.fb_nei[bi.fb_off + j];
+                // the x-factor of this velocity alone, formed as stripe_factors forms it
+                const double dx = g.xi[ck.ix0 + vv].x - k.ux;
+                const double ex = k.A * exp(-((dx * dx) * k.invRT) / 2.0);
+                fl[vv * a.max_nfb + j] = face_flux<S, D>(g, a, fb + vv * a.max_nl, gr + vv * a.max_ng * D,
+                                                        ck.vrow + ck.ix0 + vv, f, lo, ln, fgeo, k,
+                                                        ex * row_factor(k, xr), hx, rho_b);
             }
         }
         __syncthreads();
