@@ -492,6 +492,15 @@ __global__ void __launch_bounds__(kNTA)
     const double dt = *dtp;
     const double hx = 0.5 * dt;
 
+    // Face slots: slot 0 of each thread takes block slot j = thread.  Slot 1 takes j = kNTA +
+    // thread in general; in 2-D blocks with at most kNTA / VC slots beyond kNTA (a 16 x 16
+    // block has 544) it takes velocity offset vo = thread % VC of slot kNTA + thread / VC
+    // instead, so that the few slots beyond kNTA spread over VC lanes each rather than
+    // giving a whole second face to some threads while the rest wait at the barrier.  The
+    // VC lanes of such a slot add their partial moments at the end, in lane order.
+    const bool items = D == 2 && bi.nfs > kNTA && bi.nfs <= kNTA + kNTA / VC;
+    const int vo = threadIdx.x % VC;
+    auto slot_of = [&](int s) { return (s == 1 && items) ? kNTA + (int)threadIdx.x / VC : (int)threadIdx.x + s * kNTA; };
     double acc[kSlotsA][NMA];
     // fmeta: (twin slot + 1) << 2 | slot flags; the twin is the adjacent block's slot of a
     // shared face (-1: none)
@@ -501,7 +510,7 @@ __global__ void __launch_bounds__(kNTA)
     for (int s = 0; s < kSlotsA; ++s) {
 #pragma unroll
         for (int k = 0; k < NMA; ++k) acc[s][k] = 0.0;
-        const int j = threadIdx.x + s * kNTA;
+        const int j = slot_of(s);
         fid[s] = -1;
         flo[s] = fln[s] = fmeta[s] = ftyp[s] = 0;
         fS[s] = armO[s] = armN[s] = d3{0, 0, 0};
@@ -566,8 +575,10 @@ __global__ void __launch_bounds__(kNTA)
             if (fid[s] < 0) continue;
             RowSums rs{};
             const bool own_in = fmeta[s] & kSlotOwnIn, nei_in = fmeta[s] & kSlotNeiIn;
+            const bool split = s == 1 && items;
+            const int vb = split ? vo : 0, ve = split ? min(vo + 1, ck.nvv) : ck.nvv;
 #pragma unroll 1
-            for (int vv = 0; vv < ck.nvv; ++vv) {
+            for (int vv = vb; vv < ve; ++vv) {
                 const int v = ck.vrow + ck.ix0 + vv;
                 const d3 xi = g.xi[v];
                 const double w = g.w[v];
@@ -619,7 +630,7 @@ __global__ void __launch_bounds__(kNTA)
                     // into this block's slot and, for a face shared with an adjacent block,
                     // into that block's slot too: pass B then reads its own slots only
                     const P2<S> val{(S)gf, (S)hf};
-                    F[(size_t)v * a.n_slots + (bi.fs_off + threadIdx.x + s * kNTA)] = val;  // own slot
+                    F[(size_t)v * a.n_slots + (bi.fs_off + slot_of(s))] = val;  // own slot
                     const int twin = (fmeta[s] >> 2) - 1;
                     if (twin >= 0) F[(size_t)v * a.n_slots + twin] = val;
                 }
@@ -636,6 +647,17 @@ __global__ void __launch_bounds__(kNTA)
                 if (threadIdx.x + s * kNTA < bi.nl) load_cell<S, VC>(g, lc[s], gT, hT, nx, gt[s], ht[s]);
         }
         __syncthreads();
+    }
+    if (items) {  // uniform over the work-group: every lane takes part in the shuffles
+#pragma unroll
+        for (int k = 0; k < NMA; ++k) {
+            const double t = acc[1][k];
+            double sum = t;
+#pragma unroll
+            for (int d = 1; d < VC; ++d) sum += __shfl_down(t, d, VC);
+            acc[1][k] = sum;
+        }
+        if (vo != 0) fid[1] = -1;
     }
 #pragma unroll
     for (int s = 0; s < kSlotsA; ++s) {

@@ -120,8 +120,10 @@ def test_determinism_and_checkpoint_resume(tmp_path, fused):
 
 @pytest.mark.parametrize("fused", [2, 3])
 def test_fused_and_staged_agree_on_multiblock_mesh(fused):
-    """A 48x48 cavity gives nine RCB blocks, so block halos, ring-2 cells and faces shared
-    between blocks (twin face slots) are all exercised; FP64 fused vs staged after 50 steps."""
+    """A 48x48 cavity gives nine 16x16 RCB blocks, so block halos, ring-2 cells, faces shared
+    between blocks (twin face slots) and, in the face-value schedule, the 32 face slots
+    beyond 512 per block that pass A spreads over velocity lanes (13 velocities per axis:
+    a short last stripe) are all exercised; FP64 fused vs staged after 50 steps."""
     from dugks_amd import hip
     c = _cases().cavity_case(48, "NC38", 13, 4.0, 0.05)
     a = hip.FvDVM(c, fused=0)
