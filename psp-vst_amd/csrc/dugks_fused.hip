@@ -837,18 +837,18 @@ __global__ void __launch_bounds__(kNTB)
         // FW per thread, x-factors kept for the stripe; the (slot, velocity) pairs of the
         // slots beyond are dealt over all threads, so no thread takes one whole face more
         // than the others while they wait at the barrier.
-        // Per-slot copies are loads that do not wait on the face label (2-D; the records in
-        // 3-D with FP32 storage; the other 3-D copies cost registers that push pass B to one
-        // wave per SIMD: 131 -> 196 ms on the sphere).  Pass A wrote the upwind value into
-        // the slots of both blocks of a shared face.
+        // Per-slot copies are loads that do not wait on the face label: the face records
+        // always, the area vectors except in 3-D with FP64 storage (there they cost the
+        // registers of the second wave per SIMD).  Pass A wrote the upwind value into the
+        // slots of both blocks of a shared face.
 #pragma unroll
         for (int s = 0; s < FW; ++s) {
             const int j = threadIdx.x + s * kNTB;
             if (j >= bi.nfs) break;
             const int own_slot = bi.fs_off + j;
             const int f = a.fs_face[own_slot];
-            const d3 Sf = D == 2 ? a.fs_sf[own_slot] : g.Sf[f];
-            const EqCoef k = (D == 2 || sizeof(S) == 4) ? a.fs_coef[own_slot] : a.fc[f];
+            const d3 Sf = (D == 2 || sizeof(S) == 4) ? a.fs_sf[own_slot] : g.Sf[f];
+            const EqCoef k = a.fs_coef[own_slot];
             if (ck.new_stripe) stripe_factors<VC>(g, k, ck.ix0, a.n1, fex[s]);
             const double eyz = row_factor(k, xr);
 #pragma unroll
@@ -866,8 +866,8 @@ __global__ void __launch_bounds__(kNTB)
                 const int j = FW * kNTB + q;
                 const int own_slot = bi.fs_off + j;
                 const int f = a.fs_face[own_slot];
-                const d3 Sf = D == 2 ? a.fs_sf[own_slot] : g.Sf[f];
-                const EqCoef k = (D == 2 || sizeof(S) == 4) ? a.fs_coef[own_slot] : a.fc[f];
+                const d3 Sf = (D == 2 || sizeof(S) == 4) ? a.fs_sf[own_slot] : g.Sf[f];
+                const EqCoef k = a.fs_coef[own_slot];
                 // the x-factor of this velocity alone, formed as stripe_factors forms it
                 const double dx = g.xi[ck.ix0 + vv].x - k.ux;
                 const double ex = k.A * exp(-((dx * dx) * k.invRT) / 2.0);
@@ -1241,9 +1241,8 @@ void Fused<S>::face_macros(const Geo &g, FusedPlan &p, Macros fm, double *rho_b,
     else
         hipLaunchKernelGGL(k_face_macros<2>, grid1(g.nF), dim3(kBlk), 0, st, g, fargs(p), fm, rho_b, dt);
     hipLaunchKernelGGL(k_face_coef, grid1(g.nF), dim3(kBlk), 0, st, g, fm, p.fc, dt);
-    // pass B reads the records per face slot (2-D; 3-D with FP32 storage)
-    if (g.D == 2 || sizeof(S) == 4)
-        hipLaunchKernelGGL(k_slot_coef, grid1(p.n_slots), dim3(kBlk), 0, st, fargs(p), p.n_slots);
+    // pass B reads the records per face slot
+    hipLaunchKernelGGL(k_slot_coef, grid1(p.n_slots), dim3(kBlk), 0, st, fargs(p), p.n_slots);
     launch_cfl_partials(g, fm, cfl_part, st);
 }
 
