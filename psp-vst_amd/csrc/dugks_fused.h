@@ -36,7 +36,9 @@ constexpr int kFusedXcds = 8;         // accelerator dies; work-group ids are de
 #define DUGKS_PASS_B_VC 4
 #endif
 constexpr int kFusedVcB = DUGKS_PASS_B_VC;  // max velocities per pass-B chunk (register budget)
-// local cells (block + halo ring) per pass-A thread in stage 1, face slots per pass-B thread
+// local cells (block + halo ring) per pass-A thread in stage 1; face slots per pass-B
+// thread, the capacity the planner checks (pass B takes all but one of them as whole faces
+// and deals the (slot, velocity) pairs of the rest over the work-group)
 constexpr int fused_cells_per_thread_a(int D) { return D == 2 ? 1 : 2; }
 constexpr int fused_faces_per_thread_b(int D) { return D == 2 ? 3 : 4; }
 

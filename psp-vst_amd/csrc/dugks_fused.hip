@@ -39,7 +39,7 @@ namespace {
 constexpr int kNTA = kFusedThreadsA;
 constexpr int kSlotsA = kFusedSlotsA;
 static_assert(2 * kFusedThreadsB <= kFusedThreadsA,
-              "pass A can take each gradient (block) cell on two threads, one per half of the chunk");
+              "pass A takes each gradient (block) cell on two threads, one per half of the chunk");
 constexpr int kNTB = kFusedThreadsB;
 constexpr int kNb = kFusedMaxNb;
 constexpr int kBlk = 256;
@@ -151,13 +151,13 @@ __device__ inline double row_factor(const EqCoef &k, const d3 &xr)
 // A exp(-(xi_x - U_x)^2 / 2RT) for the abscissae ix0 .. ix0+VC-1 of a stripe; xi[ix] is the
 // velocity (x[ix], x[0], x[0]) of the tensor set (ix fastest)
 template <int VC>
-__device__ inline void stripe_factors(const Geo &g, const EqCoef &k, int ix0, int n1, double *ex)
+__device__ inline void stripe_factors(const Geo &g, const EqCoef &k, int ix0, int n1, double *ex, int es = 1)
 {
 #pragma unroll
     for (int vv = 0; vv < VC; ++vv) {
         const int ix = min(ix0 + vv, n1 - 1);
         const double dx = g.xi[ix].x - k.ux;
-        ex[vv] = k.A * exp(-((dx * dx) * k.invRT) / 2.0);
+        ex[vv * es] = k.A * exp(-((dx * dx) * k.invRT) / 2.0);
     }
 }
 
@@ -335,21 +335,22 @@ __device__ inline void load_cell(const Geo &g, int c, const S *__restrict__ gT, 
 }
 
 // f-bar-plus of cell c (equilibrium record k) for the chunk's velocities (FP64), x-factors
-// cex of the stripe kept by the caller; LOAD: g~, h~ are loaded here, else gt, ht hold them
+// cex[vv * cs] of the stripe kept by the caller (registers, or LDS with a stride); LOAD:
+// g~, h~ are loaded here, else gt, ht hold them
 template <class S, int VC, bool LOAD = true>
 __device__ inline void cell_bar(const Geo &g, const FArgs &a, const EqCoef &k, int c, const S *__restrict__ gT,
                                 const S *__restrict__ hT, double dt, const Chunk &ck, const d3 &xr, double *cex,
-                                S *gt, S *ht, double *gb, double *hb)
+                                S *gt, S *ht, double *gb, double *hb, int cs = 1)
 {
     if (LOAD) load_cell<S, VC>(g, c, gT, hT, ck, gt, ht);
-    if (ck.new_stripe) stripe_factors<VC>(g, k, ck.ix0, a.n1, cex);
+    if (ck.new_stripe) stripe_factors<VC>(g, k, ck.ix0, a.n1, cex, cs);
     const double eyz = row_factor(k, xr);
     const double af = 1.5 * dt / (2.0 * k.tr + dt);
 #pragma unroll
     for (int vv = 0; vv < VC; ++vv) {
         const int v = ck.vrow + ck.ix0 + min(vv, ck.nvv - 1);
         double gEq, hEq;
-        shakhov_poly(g, k, g.xi[v], cex[vv] * eyz, gEq, hEq);
+        shakhov_poly(g, k, g.xi[v], cex[vv * cs] * eyz, gEq, hEq);
         // with FP32 storage, stored values and gb, hb are deviations from the per-velocity
         // reference; FP64 storage keeps values (its reference is zero)
         if (sizeof(S) == 4) {
@@ -364,11 +365,13 @@ __device__ inline void cell_bar(const Geo &g, const FArgs &a, const EqCoef &k, i
 // pass A stage 1: f-bar-plus of the local cells (block + ring) into LDS.  PRE: g~, h~ of
 // the thread's cells were loaded into pg, ph beforehand, else each cell loads its own.
 // HK: the cells' equilibrium records are held in kc for the kernel, else loaded per chunk
-template <class S, int VC, int NT, int CPT, bool D3, bool PRE, bool HK>
+// CL: the x-factors live in LDS, cexL[vv * max_nl + l] (each thread touches only its own
+// cells' entries), else in cex
+template <class S, int VC, int NT, int CPT, bool D3, bool PRE, bool HK, bool CL>
 __device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &bi, const S *__restrict__ gT,
                                  const S *__restrict__ hT, double dt, const Chunk &ck, const int (&lc)[CPT],
                                  const S (&pg)[CPT][VC], const S (&ph)[CPT][VC], const EqCoef (&kc)[CPT],
-                                 double (&cex)[CPT][VC], P2<S> *fb)
+                                 double (&cex)[CPT][VC], double *cexL, P2<S> *fb)
 {
     const d3 xr = g.xi[ck.vrow];
 #pragma unroll
@@ -386,11 +389,13 @@ __device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &
                 ht[vv] = ph[s][vv];
             }
         }
+        double *cx = CL ? cexL + l : cex[s];
+        const int cs = CL ? a.max_nl : 1;
         if (HK) {
-            cell_bar<S, VC, !PRE>(g, a, kc[s], c, gT, hT, dt, ck, xr, cex[s], gt, ht, gb, hb);
+            cell_bar<S, VC, !PRE>(g, a, kc[s], c, gT, hT, dt, ck, xr, cx, gt, ht, gb, hb, cs);
         } else {
             const EqCoef k = a.cc[c];
-            cell_bar<S, VC, !PRE>(g, a, k, c, gT, hT, dt, ck, xr, cex[s], gt, ht, gb, hb);
+            cell_bar<S, VC, !PRE>(g, a, k, c, gT, hT, dt, ck, xr, cx, gt, ht, gb, hb, cs);
         }
 #pragma unroll
         for (int vv = 0; vv < VC; ++vv) {
@@ -400,27 +405,25 @@ __device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &
     }
 }
 
-// pass A stage 2: gradients of the block cells.  SPLIT: block cell l is taken by threads l
-// and l + NT/2, which split the chunk's velocities (even / odd), so the whole work-group
-// works; otherwise by thread l alone (FP64 3-D: the split costs spills there).
+// pass A stage 2: gradients of the block cells.  Block cell l is taken by threads l and
+// l + NT/2, which split the chunk's velocities (even / odd), so the whole work-group works.
 // Stencil range (first entry, count) of the thread's gradient cell, fixed for the kernel
-template <int NT, bool SPLIT>
+template <int NT>
 __device__ inline int2 grad_range(const FArgs &a, const BlockInfo &bi)
 {
-    const int l = SPLIT ? threadIdx.x % (NT / 2) : threadIdx.x;
+    const int l = threadIdx.x % (NT / 2);
     if (l >= bi.nc) return int2{0, 0};
     const int k0 = a.gs_beg[bi.gsb_off + l];
     return int2{k0, a.gs_beg[bi.gsb_off + l + 1] - k0};
 }
 
-template <class S, int D, int VC, int NT, bool SPLIT>
+template <class S, int D, int VC, int NT>
 __device__ inline void stage_grad(const FArgs &a, const BlockInfo &bi, int nvv, int2 gk, const P2<S> *fb,
                                   P2<S> *gr)
 {
     const V3<S> *ls = (const V3<S> *)a.gs_ls;
-    const int l = SPLIT ? threadIdx.x % (NT / 2) : threadIdx.x;
-    const int v0 = SPLIT ? threadIdx.x / (NT / 2) : 0;
-    constexpr int kStep = SPLIT ? 2 : 1;
+    const int l = threadIdx.x % (NT / 2);
+    const int v0 = threadIdx.x / (NT / 2);
     if (l < bi.nc) {
         const int k0 = gk.x;
         const int n = gk.y;
@@ -434,7 +437,7 @@ __device__ inline void stage_grad(const FArgs &a, const BlockInfo &bi, int nvv, 
             }
         }
 #pragma unroll 1
-        for (int vv = v0; vv < nvv; vv += kStep) {
+        for (int vv = v0; vv < nvv; vv += 2) {
             const P2<S> *p = fb + vv * a.max_nl;
             const P2<S> c = p[l];
             S gx = 0, gy = 0, gz = 0, hx = 0, hy = 0, hz = 0;
@@ -489,6 +492,10 @@ __global__ void __launch_bounds__(kNTA)
     const int ch0 = vg * a.cpg, ch1 = min(a.nchunk, ch0 + a.cpg);
     P2<S> *fb = (P2<S> *)smem;        // [VC][max_nl]
     P2<S> *gr = fb + VC * a.max_nl;    // [VC][max_nc][D]
+    // FP64 3-D: the local cells' x-factors in LDS, [VC][max_nl] (registers are short there;
+    // the 512-thread work-group is alone on its CU, so the LDS is free)
+    constexpr bool kCexLds = D == 3 && sizeof(S) == 8;
+    double *cexL = (double *)(gr + VC * a.max_nc * D);
     const double dt = *dtp;
     const double hx = 0.5 * dt;
 
@@ -541,8 +548,7 @@ __global__ void __launch_bounds__(kNTA)
         const int l = threadIdx.x + s * kNTA;
         lc[s] = l < bi.nl ? a.lcell[bi.l_off + l] : 0;
     }
-    constexpr bool kSplitGrad = !(D == 3 && sizeof(S) == 8);
-    const int2 gk = grad_range<kNTA, kSplitGrad>(a, bi);
+    const int2 gk = grad_range<kNTA>(a, bi);
     // 2-D: the next chunk's g~, h~ are loaded before the chunk's last barrier, so their
     // latency overlaps the barrier wait (3-D has no registers to spare)
     constexpr bool kAhead = D == 2;
@@ -565,9 +571,10 @@ __global__ void __launch_bounds__(kNTA)
     }
     for (int ch = ch0; ch < ch1; ++ch) {
         const Chunk ck = chunk_of(a, ch, VC, stripe);
-        stage_bar<S, VC, kNTA, CPT, D == 3, kAhead, kHoldK>(g, a, bi, gT, hT, dt, ck, lc, gt, ht, kc, cex, fb);
+        stage_bar<S, VC, kNTA, CPT, D == 3, kAhead, kHoldK, kCexLds>(g, a, bi, gT, hT, dt, ck, lc, gt, ht, kc, cex,
+                                                                     cexL, fb);
         __syncthreads();
-        stage_grad<S, D, VC, kNTA, kSplitGrad>(a, bi, ck.nvv, gk, fb, gr);
+        stage_grad<S, D, VC, kNTA>(a, bi, ck.nvv, gk, fb, gr);
         __syncthreads();
         const d3 xr = g.xi[ck.vrow];
 #pragma unroll
@@ -822,16 +829,33 @@ __global__ void __launch_bounds__(kNTB)
 
     constexpr int FW = FPT - 1;  // whole faces per thread in stage 2
     double cex[VC], fex[FW][VC];
+    // 2-D FP64: the next chunk's g~, h~ of the cell are loaded before the chunk's last
+    // barrier (7.52 -> 7.42 ms on the 256^2 cavity; with FP32 storage it measured slower, and
+    // 3-D has no registers to spare).  The update below writes only this chunk's velocities
+    // of the cell, so the early loads read values no one writes first.
+    constexpr bool kAhead = D == 2 && sizeof(S) == 8;
+    S pg[VC] = {}, ph[VC] = {};
     int stripe = -1;
+    if (kAhead && has_cell && ch0 < ch1) {
+        int st = stripe;
+        load_cell<S, VC>(g, cell, gT, hT, chunk_of(a, ch0, VC, st), pg, ph);
+    }
     for (int ch = ch0; ch < ch1; ++ch) {
         const Chunk ck = chunk_of(a, ch, VC, stripe);
         const d3 xr = g.xi[ck.vrow];
         // stage 1: f-bar-plus of the thread's cell
         S gt[VC], ht[VC];
         double gb[VC], hb[VC];
+        if (kAhead) {
+#pragma unroll
+            for (int vv = 0; vv < VC; ++vv) {
+                gt[vv] = pg[vv];
+                ht[vv] = ph[vv];
+            }
+        }
         if (has_cell) {
             const EqCoef k = a.cc[cell];
-            cell_bar<S, VC>(g, a, k, cell, gT, hT, dt, ck, xr, cex, gt, ht, gb, hb);
+            cell_bar<S, VC, !kAhead>(g, a, k, cell, gT, hT, dt, ck, xr, cex, gt, ht, gb, hb);
         }
         // stage 2: face values -> flux sigma * f_f in LDS.  Slots j < FW * kNTB: whole faces,
         // FW per thread, x-factors kept for the stripe; the (slot, velocity) pairs of the
@@ -908,6 +932,10 @@ __global__ void __launch_bounds__(kNTB)
                 row_add(rs, w * (double)sg, w * (double)sh, g.xi[v].x);  // deviations
             }
             fold_row<D>(mom, rs, xr.y, xr.z);
+            if (kAhead && ch + 1 < ch1) {
+                int st = stripe;
+                load_cell<S, VC>(g, cell, gT, hT, chunk_of(a, ch + 1, VC, st), pg, ph);
+            }
         }
         __syncthreads();
     }
@@ -1013,10 +1041,18 @@ __global__ void k_ref_moments(Geo g, double *cref)
     for (int j = 0; j < NMA; ++j) cref[j] = r[j];
 }
 
+// pass-A LDS: f-bar-plus and gradient tiles (what the budget limits), plus the x-factors
+// of FP64 3-D
+template <class S>
+size_t lds_tiles_a(const FusedPlan &p, int D, int VC)
+{
+    return (size_t)VC * (2 * p.max_nl + 2 * p.max_nc * D) * sizeof(S);
+}
 template <class S>
 size_t lds_pass_a(const FusedPlan &p, int D, int VC)
 {
-    return (size_t)VC * (2 * p.max_nl + 2 * p.max_nc * D) * sizeof(S);
+    const size_t cex = (D == 3 && sizeof(S) == 8) ? (size_t)VC * p.max_nl * sizeof(double) : 0;
+    return lds_tiles_a<S>(p, D, VC) + cex;
 }
 size_t lds_pass_b(const FusedPlan &p, int VC) { return (size_t)VC * 2 * p.max_nfs * sizeof(double); }
 inline int vc_for_b(int vc) { return std::min(vc, kFusedVcB); }
@@ -1027,7 +1063,7 @@ template <class S>
 int choose_vc(const FusedPlan &p, int D)
 {
     for (int vc : {4, 2, 1})
-        if (lds_pass_a<S>(p, D, vc) <= kLdsBudget && lds_pass_b(p, vc_for_b(vc)) <= kLdsBudget) return vc;
+        if (lds_tiles_a<S>(p, D, vc) <= kLdsBudget && lds_pass_b(p, vc_for_b(vc)) <= kLdsBudget) return vc;
     return 0;
 }
 

@@ -34,7 +34,9 @@ constexpr int kFusedSlotsA = 3;       // owned faces per pass-A thread
 constexpr int kFusedThreadsB = 256;   // pass B threads = max cells per block
 constexpr int kFusedMaxNb = 8;        // max internal faces per gradient cell / faces per block cell
 constexpr int kFusedXcds = 8;         // accelerator dies; work-group ids are dealt round-robin over them
-// local cells per thread in stage 1 (pass A / pass B) and block faces per thread in pass B
+// local cells per thread in stage 1 (pass A / pass B) and block faces per thread in pass B,
+// the capacity the planner checks (pass B takes all but one of them as whole faces and
+// deals the (face, velocity) pairs of the rest over the work-group)
 constexpr int fused_cells_per_thread_a(int D) { return D == 2 ? 2 : 4; }
 constexpr int fused_cells_per_thread_b(int D) { return D == 2 ? 2 : 4; }
 constexpr int fused_faces_per_thread_b(int D) { return D == 2 ? 3 : 4; }
