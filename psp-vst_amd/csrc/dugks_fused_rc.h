@@ -85,6 +85,7 @@ struct FusedPlan {
     int wg_target = 4096;              // work-groups per pass the velocity grouping aims at
     int max_nl = 0, max_ng = 0, max_nfa = 0, max_nfb = 0, max_nc = 0;
     int nm = 0, nma = 0;               // raw moments per cell / per face (+ outgoing wall flux)
+    int nbuf_a = 1;                    // pass-A LDS tile buffers (1 or 2)
     size_t lds_a = 0, lds_b = 0;       // dynamic LDS bytes of the two passes
     // host tables
     std::vector<BlockInfo> blk;

@@ -74,6 +74,7 @@ struct FArgs {
     const BndCoef *bc;
     int nblk, b8, nvg, n1, nrow, nchunk, cpg;
     int max_nl, max_ng, max_nfb, max_nc, nm, nma;
+    int nbuf_a;        // pass-A LDS tile buffers (2: alternate per chunk, one barrier fewer)
 };
 
 FArgs fargs(const FusedPlan &p)
@@ -87,6 +88,7 @@ FArgs fargs(const FusedPlan &p)
     a.nblk = p.nblk; a.b8 = p.b8; a.nvg = p.nvg; a.n1 = p.n1; a.nrow = p.nrow; a.nchunk = p.nchunk; a.cpg = p.cpg;
     a.max_nl = p.max_nl; a.max_ng = p.max_ng; a.max_nfb = p.max_nfb; a.max_nc = p.max_nc;
     a.nm = p.nm; a.nma = p.nma;
+    a.nbuf_a = p.nbuf_a;
     return a;
 }
 
@@ -473,8 +475,12 @@ __global__ void __launch_bounds__(kNTA, kWavesA)
     extern __shared__ __align__(16) unsigned char smem[];
     const BlockInfo bi = a.blk[blkid];
     const int ch0 = vg * a.cpg, ch1 = min(a.nchunk, ch0 + a.cpg);
-    P2<S> *fb = (P2<S> *)smem;        // [VC][max_nl]
-    P2<S> *gr = fb + VC * a.max_nl;    // [VC][max_ng][D]
+    // LDS: nbuf_a buffers of {f-bar-plus [VC][max_nl], gradients [VC][max_ng][D]}.  With two,
+    // chunk c uses buffer c % 2: the next chunk's stage 1 writes the other buffer while slow
+    // threads still reconstruct from this one, so the barrier closing each chunk is not needed
+    // (a thread reaches the next chunk's first barrier only after its own faces are done).
+    P2<S> *const tiles = (P2<S> *)smem;
+    const int tile = VC * (a.max_nl + a.max_ng * D);
     const double dt = *dtp;
     const double hx = 0.5 * dt;
 
@@ -519,6 +525,8 @@ __global__ void __launch_bounds__(kNTA, kWavesA)
     int stripe = -1;
     for (int ch = ch0; ch < ch1; ++ch) {
         const Chunk ck = chunk_of(a, ch, VC, stripe);
+        P2<S> *fb = tiles + (a.nbuf_a == 2 ? ((ch - ch0) & 1) * tile : 0);  // [VC][max_nl]
+        P2<S> *gr = fb + VC * a.max_nl;                                      // [VC][max_ng][D]
         stage_bar<S, VC, kNTA, CPT, false>(g, a, bi, gT, hT, dt, ck, lc, unused, unused, cex, fb,
                                            (P2<S> *)nullptr);
         __syncthreads();
@@ -565,7 +573,7 @@ __global__ void __launch_bounds__(kNTA, kWavesA)
             }
             fold_row<D>(acc[s], rs, xr.y, xr.z);
         }
-        __syncthreads();
+        if (a.nbuf_a == 1) __syncthreads();
     }
 #pragma unroll
     for (int s = 0; s < kSlotsA; ++s) {
@@ -965,15 +973,21 @@ __global__ void k_ref_moments(Geo g, double *cref)
     for (int j = 0; j < NMA; ++j) cref[j] = r[j];
 }
 
+// f-bar-plus and gradient tiles of one chunk
 template <class S>
-size_t lds_pass_a(const FusedPlan &p, int D, int VC)
+size_t lds_tiles(const FusedPlan &p, int D, int VC)
 {
     return (size_t)VC * (2 * p.max_nl + 2 * p.max_ng * D) * sizeof(S);
 }
 template <class S>
+size_t lds_pass_a(const FusedPlan &p, int D, int VC)
+{
+    return (size_t)p.nbuf_a * lds_tiles<S>(p, D, VC);
+}
+template <class S>
 size_t lds_pass_b(const FusedPlan &p, int D, int VC)
 {
-    return (size_t)VC * 2 * p.max_nfb * sizeof(double) + lds_pass_a<S>(p, D, VC) +
+    return (size_t)VC * 2 * p.max_nfb * sizeof(double) + lds_tiles<S>(p, D, VC) +
            (size_t)VC * 2 * p.max_nc * sizeof(S);
 }
 
@@ -1078,6 +1092,11 @@ bool fused_plan_upload(FusedPlan &p, const Geo &g, size_t es, std::string &err)
     if (vc == 0) {
         err = "block tables exceed the LDS budget";
         return false;
+    }
+    // two pass-A tile buffers where they fit the budget that keeps two work-groups per CU
+    {
+        const size_t t = es == 4 ? lds_tiles<float>(p, g.D, vc) : lds_tiles<double>(p, g.D, vc);
+        p.nbuf_a = 2 * t <= kLdsBudget ? 2 : 1;
     }
     if (g.n1d < 1 || g.Nv != (g.D == 3 ? g.n1d * g.n1d * g.n1d : g.n1d * g.n1d)) {
         err = "fused path needs a tensor-product velocity set";
