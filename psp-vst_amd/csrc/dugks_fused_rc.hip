@@ -788,7 +788,8 @@ __global__ void __launch_bounds__(kNTB, kWavesB)
     }
     for (int ch = ch0; ch < ch1; ++ch) {
         const Chunk ck = chunk_of(a, ch, VC, stripe);
-        stage_bar<S, VC, kNTB, CPT, kAhead>(g, a, bi, gT, hT, dt, ck, lc, gt, ht, cex, fb, ft);
+        stage_bar<S, VC, kNTB, CPT, kAhead>(g, a, bi, gT, hT, dt, ck, lc, gt, ht, cex, fb,
+                                            kAhead ? (P2<S> *)nullptr : ft);
         __syncthreads();
         stage_grad<S, D, VC, kNTB>(a, bi, ck.nvv, fb, gr);
         __syncthreads();
@@ -836,10 +837,14 @@ __global__ void __launch_bounds__(kNTB, kWavesB)
         __syncthreads();
         // stage 4: cell update in ascending face order, new values to the other buffer
         if (has_cell) {
-#pragma unroll 1
-            for (int vv = 0; vv < ck.nvv; ++vv) {
+#pragma unroll
+            for (int vv = 0; vv < VC; ++vv) {
+                if (vv >= ck.nvv) break;
                 const int v = ck.vrow + ck.ix0 + vv;
-                const P2<S> t = ft[vv * a.max_nc + my], bb = fb[vv * a.max_nl + my];
+                // g~ of the thread's own cell: still in the registers stage 1 took it from
+                // (slot 0 is local cell l = thread, the block cell it updates)
+                const P2<S> t = kAhead ? P2<S>{gt[0][vv], ht[0][vv]} : ft[vv * a.max_nc + my];
+                const P2<S> bb = fb[vv * a.max_nl + my];
                 double ag = -1.0 / 3 * (double)t.g + 4.0 / 3 * (double)bb.g;
                 double ah = -1.0 / 3 * (double)t.h + 4.0 / 3 * (double)bb.h;
 #pragma unroll
