@@ -537,8 +537,9 @@ __global__ void __launch_bounds__(kNTA, kWavesA)
         for (int s = 0; s < kSlotsA; ++s) {
             if (fid[s] < 0) continue;
             RowSums rs{};
-#pragma unroll 1
-            for (int vv = 0; vv < ck.nvv; ++vv) {
+#pragma unroll
+            for (int vv = 0; vv < VC; ++vv) {
+                if (vv >= ck.nvv) break;
                 const int v = ck.vrow + ck.ix0 + vv;
                 const d3 xi = g.xi[v];
                 const double w = g.w[v];
