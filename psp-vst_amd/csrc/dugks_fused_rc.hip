@@ -545,29 +545,29 @@ __global__ void __launch_bounds__(kNTA, kWavesA)
                 const double w = g.w[v];
                 const double xs = sigma_dot(xi, fS[s]);
                 const P2<S> *pf = fb + vv * a.max_nl, *pq = gr + vv * a.max_ng * D;
-                const double ox = armO[s].x - xi.x * hx, oy = armO[s].y - xi.y * hx, oz = armO[s].z - xi.z * hx;
+                if (ftyp[s] != 0 && !(xs > 0)) continue;  // incoming half added from constants
+                // upwind side: the owner for xs >= VSMALL (and outgoing boundary halves), the
+                // neighbour for xs < -VSMALL, chosen before one reconstruction (the lanes of a
+                // wave hold different faces, so two branches would both run); tangent internal
+                // faces average both sides
+                const bool tangent = ftyp[s] == 0 && xs < kVSmall && xs >= -kVSmall;
                 double gf, hf;
-                if (ftyp[s] == 0) {
-                    const double nx = armN[s].x - xi.x * hx, ny = armN[s].y - xi.y * hx, nz = armN[s].z - xi.z * hx;
-                    if (xs >= kVSmall) {
-                        const P2<S> r = recon_l<S, D>(pf, pq, fown[s], ox, oy, oz);
-                        gf = (double)r.g;
-                        hf = (double)r.h;
-                    } else if (xs < -kVSmall) {
-                        const P2<S> r = recon_l<S, D>(pf, pq, fnei[s], nx, ny, nz);
-                        gf = (double)r.g;
-                        hf = (double)r.h;
-                    } else {
-                        const P2<S> rn = recon_l<S, D>(pf, pq, fnei[s], nx, ny, nz);
-                        const P2<S> ro = recon_l<S, D>(pf, pq, fown[s], ox, oy, oz);
-                        gf = 0.5 * ((double)rn.g + (double)ro.g);
-                        hf = 0.5 * ((double)rn.h + (double)ro.h);
-                    }
-                } else {
-                    if (!(xs > 0)) continue;  // incoming half added from constants
-                    const P2<S> r = recon_l<S, D>(pf, pq, fown[s], ox, oy, oz);
+                if (!tangent) {
+                    const bool own = ftyp[s] != 0 || xs >= kVSmall;
+                    const int l = own ? fown[s] : fnei[s];
+                    const double ax = (own ? armO[s].x : armN[s].x) - xi.x * hx;
+                    const double ay = (own ? armO[s].y : armN[s].y) - xi.y * hx;
+                    const double az = (own ? armO[s].z : armN[s].z) - xi.z * hx;
+                    const P2<S> r = recon_l<S, D>(pf, pq, l, ax, ay, az);
                     gf = (double)r.g;
                     hf = (double)r.h;
+                } else {
+                    const double ox = armO[s].x - xi.x * hx, oy = armO[s].y - xi.y * hx, oz = armO[s].z - xi.z * hx;
+                    const double nx = armN[s].x - xi.x * hx, ny = armN[s].y - xi.y * hx, nz = armN[s].z - xi.z * hx;
+                    const P2<S> rn = recon_l<S, D>(pf, pq, fnei[s], nx, ny, nz);
+                    const P2<S> ro = recon_l<S, D>(pf, pq, fown[s], ox, oy, oz);
+                    gf = 0.5 * ((double)rn.g + (double)ro.g);
+                    hf = 0.5 * ((double)rn.h + (double)ro.h);
                 }
                 if (ftyp[s] == kMaxwellWall) acc[s][NMA - 1] += w * xs * gf;
                 row_add(rs, w * gf, w * hf, xi.x);
@@ -685,35 +685,34 @@ __device__ inline P2<double> face_flux(const Geo &g, const FArgs &a, const P2<S>
 {
     const d3 xi = g.xi[v];
     const double xs = sigma_dot(xi, fgeo.Sf);
-    const double ox = fgeo.ao.x - xi.x * hx, oy = fgeo.ao.y - xi.y * hx, oz = fgeo.ao.z - xi.z * hx;
     const int b = f - g.nIF;
-    double gv, hv;
-    if (b < 0) {
-        const double nx = fgeo.an.x - xi.x * hx, ny = fgeo.an.y - xi.y * hx, nz = fgeo.an.z - xi.z * hx;
-        if (xs >= kVSmall) {
-            const P2<S> r = recon_l<S, D>(pf, pq, lo, ox, oy, oz);
-            gv = (double)r.g;
-            hv = (double)r.h;
-        } else if (xs < -kVSmall) {
-            const P2<S> r = recon_l<S, D>(pf, pq, ln, nx, ny, nz);
-            gv = (double)r.g;
-            hv = (double)r.h;
-        } else {
-            const P2<S> rn = recon_l<S, D>(pf, pq, ln, nx, ny, nz);
-            const P2<S> ro = recon_l<S, D>(pf, pq, lo, ox, oy, oz);
-            gv = 0.5 * ((double)rn.g + (double)ro.g);
-            hv = 0.5 * ((double)rn.h + (double)ro.h);
-        }
-    } else if (xs > 0) {
-        const P2<S> r = recon_l<S, D>(pf, pq, lo, ox, oy, oz);
-        gv = (double)r.g;
-        hv = (double)r.h;
-    } else {
+    if (b >= 0 && !(xs > 0)) {  // incoming half of a wall or inlet face
         const BndCoef kb = a.bc[b];
         const double scale = g.bf_type[b] == kMaxwellWall ? rho_b[b] : 1.0;
         const double cx = xi.x - kb.ux, cy = xi.y - kb.uy, cz = xi.z - kb.uz;
-        gv = scale * (kb.A * exp(-((cx * cx + cy * cy + cz * cz) * kb.invRT) / 2.0));
+        const double gv = scale * (kb.A * exp(-((cx * cx + cy * cy + cz * cz) * kb.invRT) / 2.0));
         return P2<double>{xs * gv, xs * (gv * kb.hfac)};
+    }
+    // upwind side: the owner for xs >= VSMALL (and outgoing boundary halves), the neighbour
+    // for xs < -VSMALL, chosen before one reconstruction (the lanes of a wave hold different
+    // faces, so two branches would both run); tangent internal faces average both sides
+    const bool tangent = b < 0 && xs < kVSmall && xs >= -kVSmall;
+    double gv, hv;
+    if (!tangent) {
+        const bool own = b >= 0 || xs >= kVSmall;
+        const double ax = (own ? fgeo.ao.x : fgeo.an.x) - xi.x * hx;
+        const double ay = (own ? fgeo.ao.y : fgeo.an.y) - xi.y * hx;
+        const double az = (own ? fgeo.ao.z : fgeo.an.z) - xi.z * hx;
+        const P2<S> r = recon_l<S, D>(pf, pq, own ? lo : ln, ax, ay, az);
+        gv = (double)r.g;
+        hv = (double)r.h;
+    } else {
+        const double ox = fgeo.ao.x - xi.x * hx, oy = fgeo.ao.y - xi.y * hx, oz = fgeo.ao.z - xi.z * hx;
+        const double nx = fgeo.an.x - xi.x * hx, ny = fgeo.an.y - xi.y * hx, nz = fgeo.an.z - xi.z * hx;
+        const P2<S> rn = recon_l<S, D>(pf, pq, ln, nx, ny, nz);
+        const P2<S> ro = recon_l<S, D>(pf, pq, lo, ox, oy, oz);
+        gv = 0.5 * ((double)rn.g + (double)ro.g);
+        hv = 0.5 * ((double)rn.h + (double)ro.h);
     }
     if (sizeof(S) == 4) {  // reconstructed deviation -> value
         gv += g.ref_g[v];
