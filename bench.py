@@ -87,7 +87,9 @@ def pmc_summary(fetch_csv: str, write_csv: str, tag: str, mib: int) -> dict:
     # the probe's copy dispatches (the copy may be split); the step itself copies only small arrays
     fcopy = sum(v for k, vs in f.items() if COPY_SYMBOL in k for v in vs if v > probe / 64)
     wcopy = sum(v for k, vs in w.items() if COPY_SYMBOL in k for v in vs if v > probe / 64)
-    if not (0.5 * probe < fcopy < 2 * probe and 0.5 * probe < wcopy < 2 * probe):
+    # FETCH_SIZE reads about half of a device-to-device copy's bytes on this hardware, so the
+    # accepted window around the probe size is wide (a factor of 4 either way)
+    if not (0.25 * probe < fcopy < 4 * probe and 0.25 * probe < wcopy < 4 * probe):
         raise SystemExit(f"copy probe dispatches not found in the counter files ({fcopy:.3e}, {wcopy:.3e} B)")
     kf, kw = probe / fcopy, probe / wcopy
     out = {"workload": tag, "copy_probe_bytes": probe, "fetch_scale": kf, "write_scale": kw, "kernels": {}}
