@@ -668,22 +668,32 @@ static void update_macro_vol(orc_state *s, double dt)
 /* fvDVM::evolution (F:2128-2174).  GHbarPvolComm only exchanges processor halos and
  * evaluates fixedGradient patches whose values equal the cell values; symmetry and
  * pressure in/out patches are outside the supported set. */
-int orc_evolution(orc_state *s, double dt)
+int orc_evolution_stages(orc_state *s, double dt, int first, int last)
 {
     if (!(dt > 0)) {
         set_err("orc_evolution: dt must be positive");
         return 1;
     }
-    update_gh_bar_pvol(s, dt);
-    update_gh_bar_surf(s, dt);
-    update_maxwell_wall_rho(s);
-    update_gh_bar_surf_maxwell_wall_in(s);
-    update_macro_surf(s, dt);
-    update_gh_surf(s, dt);
-    update_gh_tilde_vol(s, dt);
-    update_macro_vol(s, dt);
+    if (first < 0 || last > ORC_N_STAGES || first > last) {
+        set_err("orc_evolution_stages: stage range outside [0, 8]");
+        return 1;
+    }
+    for (int k = first; k < last; ++k) {
+        switch (k) {
+        case 0: update_gh_bar_pvol(s, dt); break;
+        case 1: update_gh_bar_surf(s, dt); break;
+        case 2: update_maxwell_wall_rho(s); break;
+        case 3: update_gh_bar_surf_maxwell_wall_in(s); break;
+        case 4: update_macro_surf(s, dt); break;
+        case 5: update_gh_surf(s, dt); break;
+        case 6: update_gh_tilde_vol(s, dt); break;
+        case 7: update_macro_vol(s, dt); break;
+        }
+    }
     return 0;
 }
+
+int orc_evolution(orc_state *s, double dt) { return orc_evolution_stages(s, dt, 0, ORC_N_STAGES); }
 
 /* getCoNum (F:2177-2185): internal faces, previous Usurf (Q10) */
 int orc_co_num(const orc_state *s, double dt, double *co_max, double *co_mean)

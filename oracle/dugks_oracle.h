@@ -57,6 +57,15 @@ int orc_init(orc_state *s, const double *rho, const double *U, const double *T);
 int orc_co_num(const orc_state *s, double dt, double *co_max, double *co_mean);
 int orc_evolution(orc_state *s, double dt);
 
+/* The sub-steps of evolution in order (F:2128-2174): 0 updateGHbarPvol, 1 updateGHbarSurf
+ * (gradients, reconstruction, wall outGoing), 2 updateMaxwellWallRho, 3
+ * updateGHbarSurfMaxwellWallIn, 4 updateMacroSurf, 5 updateGHsurf, 6 updateGHtildeVol,
+ * 7 updateMacroVol.  orc_evolution_stages runs sub-steps [first, last) so that tests can
+ * read intermediate states (face values before relaxation, f-bar-plus, ...);
+ * orc_evolution(s, dt) is orc_evolution_stages(s, dt, 0, 8). */
+#define ORC_N_STAGES 8
+int orc_evolution_stages(orc_state *s, double dt, int first, int last);
+
 /* getters (host copies) */
 int orc_nv(const orc_state *s);
 int orc_get_cell_macros(const orc_state *s, double *rho, double *U, double *T, double *q, double *tau);

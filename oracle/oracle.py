@@ -52,6 +52,7 @@ def lib():
             getattr(L, name).argtypes = [ct.c_void_p] + [_D] * {"orc_init": 3, "orc_set_df": 2,
                                                                 "orc_set_cell_macros": 4}[name]
         L.orc_evolution.argtypes = [ct.c_void_p, ct.c_double]
+        L.orc_evolution_stages.argtypes = [ct.c_void_p, ct.c_double, ct.c_int, ct.c_int]
         L.orc_co_num.argtypes = [ct.c_void_p, ct.c_double, _D, _D]
         L.orc_nv.argtypes = [ct.c_void_p]
         L.orc_set_threads.argtypes = [ct.c_int]
@@ -111,6 +112,12 @@ class Oracle:
 
     def evolution(self, dt):
         if self.L.orc_evolution(self._h, float(dt)) != 0:
+            raise RuntimeError(self.L.orc_last_error().decode())
+
+    def stages(self, dt, first, last):
+        """Sub-steps [first, last) of evolution (0 f-bar-plus, 1 reconstruction, 2 wall rho,
+        3 wall incoming, 4 face macros, 5 face relaxation, 6 cell update, 7 cell macros)."""
+        if self.L.orc_evolution_stages(self._h, float(dt), int(first), int(last)) != 0:
             raise RuntimeError(self.L.orc_last_error().decode())
 
     def step(self, n=1):

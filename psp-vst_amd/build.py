@@ -1,4 +1,5 @@
-"""Build libdugks_hip.so for gfx950 in-tree (psp-vst_amd/dugks_amd/libdugks_hip.so).
+"""Build libdugks_hip.so for gfx950 in-tree (psp-vst_amd/dugks_amd/libdugks_hip.so), and the
+streaming probe libdugks_probe.so beside it (HBM counter calibration for the benchmark).
 
 Usage: python psp-vst_amd/build.py [--jobs N] [--verbose]
 
@@ -18,6 +19,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "dugks_amd")
 LIB = os.path.join(OUT_DIR, "libdugks_hip.so")
+PROBE_LIB = os.path.join(OUT_DIR, "libdugks_probe.so")
+PROBE_SOURCES = ["dugks_probe.hip"]
 BUILD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("DUGKS_OFFLOAD_ARCH", "gfx950")
 
@@ -49,18 +52,23 @@ def compile_one(src: str, verbose: bool) -> str:
     return obj
 
 
-def build(jobs: int = 4, verbose: bool = False) -> str:
-    os.makedirs(BUILD, exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
-        objs = list(ex.map(lambda s: compile_one(s, verbose), SOURCES))
-    tmp = LIB + ".tmp"
+def link(lib: str, objs: list, verbose: bool) -> None:
+    tmp = lib + ".tmp"
     cmd = [hipcc(), "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError(f"link failed\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, LIB)
+        raise RuntimeError(f"link failed: {lib}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, lib)
+
+
+def build(jobs: int = 4, verbose: bool = False) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        objs = list(ex.map(lambda s: compile_one(s, verbose), SOURCES + PROBE_SOURCES))
+    link(LIB, objs[:len(SOURCES)], verbose)
+    link(PROBE_LIB, objs[len(SOURCES):], verbose)
     return LIB
 
 
