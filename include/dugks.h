@@ -142,7 +142,15 @@ int dugks_set_df(dugks_handle *h, const double *g, const double *hh);
 int dugks_set_cell_macros(dugks_handle *h, const double *rho, const double *U, const double *T,
                           const double *q);
 
-/* binary checkpoint of DFs + macros + wall state (exact resume) */
+/* Binary checkpoint of DFs + macros + wall state (exact resume).  Cell-indexed arrays are
+ * written in the caller's cell order, so a file saved by one handle loads into any handle of
+ * the same mesh, velocity set, storage precision and gas reference, whatever path or
+ * schedule it runs.  Layout: a 64-byte header (magic "DUGKSCK", version 3, nC, nF, nB, Nv,
+ * bytes per stored value, the reference sums and a checksum of the caller's cell centres
+ * and volumes), then g~ and h~ [Nv][nC] (FP64 values, or FP32 deviations from the
+ * reference Maxwellian with MIXED storage), cell rho, U, T, q, tau, face rho, U, T, q, tau,
+ * boundary rho, Courant partials and the device time-step record.  A load refuses a file of
+ * another version, size, precision, reference or mesh. */
 int dugks_checkpoint_save(dugks_handle *h, const char *path);
 int dugks_checkpoint_load(dugks_handle *h, const char *path);
 

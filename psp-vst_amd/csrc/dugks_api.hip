@@ -69,6 +69,7 @@ struct dugks_handle {
     FusedPlan plan;                // face-value schedule data
     rc::FusedPlan plan_rc;         // recompute schedule data
     std::vector<int> perm;         // internal cell -> caller's cell label
+    double geo_sum = 0.0;          // order-sensitive checksum of the caller's cell centres / volumes
     std::vector<double> ref_g, ref_h;  // host copies of the storage reference per velocity
     int hist_cap = 0;
     std::vector<void *> allocs;
@@ -380,6 +381,9 @@ int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patch
     h->device = o.device;
     h->es = o.precision == DUGKS_MIXED ? 4 : 8;
     h->perm = perm;
+    for (int c = 0; c < nC; ++c)  // caller's numbering: a mesh with renumbered cells differs
+        h->geo_sum += (1.0 + (c % 97)) * (mesh->V[c] + mesh->C[3 * (size_t)c] + 0.5 * mesh->C[3 * (size_t)c + 1] +
+                                          0.25 * mesh->C[3 * (size_t)c + 2]);
     auto bail = [&](int rc) {
         std::string m = g_err;
         dugks_destroy(h);
@@ -714,13 +718,18 @@ int dugks_set_cell_macros(dugks_handle *h, const double *rho, const double *U, c
 
 // ------------------------------------------------------------------- checkpoints
 namespace {
-// Version 2: the stored distributions are deviations from the per-velocity reference; the
-// header carries the sums of that reference so a file is only loaded where it means the same.
+// Version 3.  The stored distributions are deviations from the per-velocity reference (the
+// header carries the sums of that reference), and every cell-indexed array is written in
+// the caller's cell order, so a file does not depend on the internal block numbering (which
+// differs between the staged path and the two fused schedules); the header's geometry
+// checksum, taken over the caller's cell centres and volumes in their order, refuses a mesh
+// with the same counts but other cells.
 struct CkptHeader {
     char magic[8];
     int version, nC, nF, nB, Nv, es;
-    double ref_sum_g, ref_sum_h, reserved[2];
+    double ref_sum_g, ref_sum_h, geo_sum, reserved;
 };
+constexpr int kCkptVersion = 3;
 
 int ckpt_io(dugks_handle *h, const char *path, bool save)
 {
@@ -728,50 +737,86 @@ int ckpt_io(dugks_handle *h, const char *path, bool save)
     if (!path) return fail("checkpoint: NULL path");
     const Geo &g = h->g;
     DevState &d = h->d;
-    std::FILE *fp = std::fopen(path, save ? "wb" : "rb");
-    if (!fp) return fail(std::string("checkpoint: cannot open ") + path);
     CkptHeader hd{};
     std::memcpy(hd.magic, "DUGKSCK", 8);
-    hd.version = 2; hd.nC = g.nC; hd.nF = g.nF; hd.nB = g.nB; hd.Nv = g.Nv; hd.es = (int)h->es;
+    hd.version = kCkptVersion; hd.nC = g.nC; hd.nF = g.nF; hd.nB = g.nB; hd.Nv = g.Nv; hd.es = (int)h->es;
     for (int v = 0; v < g.Nv; ++v) {
         hd.ref_sum_g += h->ref_g[v];
         hd.ref_sum_h += h->ref_h[v];
     }
-    const size_t ndf = (size_t)g.Nv * g.nC * h->es;
-    struct Blk { void *p; size_t bytes; };
+    hd.geo_sum = h->geo_sum;
+    if (save) HIPCHK(hipStreamSynchronize(h->stream));
+    std::FILE *fp = std::fopen(path, save ? "wb" : "rb");
+    if (!fp) return fail(std::string("checkpoint: cannot open ") + path);
+    // rows x nC elements of `elem` bytes, cell-indexed (permuted), or plain bytes
+    struct Blk { void *p; size_t rows, elem; bool cells; };
+    const size_t nC = g.nC;
     std::vector<Blk> blocks = {
-        {d.gT, ndf}, {d.hT, ndf},
-        {d.cell.rho, 8ull * g.nC}, {d.cell.U, 24ull * g.nC}, {d.cell.T, 8ull * g.nC}, {d.cell.q, 24ull * g.nC},
-        {d.cell.tau, 8ull * g.nC},
-        {d.face.rho, 8ull * g.nF}, {d.face.U, 24ull * g.nF}, {d.face.T, 8ull * g.nF}, {d.face.q, 24ull * g.nF},
-        {d.face.tau, 8ull * g.nF}, {d.rho_b, 8ull * g.nB}, {d.cfl_part, 16ull * cfl_blocks(g.nIF)}, {d.dt, 32}};
+        {d.gT, (size_t)g.Nv, h->es, true}, {d.hT, (size_t)g.Nv, h->es, true},
+        {d.cell.rho, 1, 8, true}, {d.cell.U, 1, 24, true}, {d.cell.T, 1, 8, true}, {d.cell.q, 1, 24, true},
+        {d.cell.tau, 1, 8, true},
+        {d.face.rho, 1, 8ull * g.nF, false}, {d.face.U, 1, 24ull * g.nF, false}, {d.face.T, 1, 8ull * g.nF, false},
+        {d.face.q, 1, 24ull * g.nF, false}, {d.face.tau, 1, 8ull * g.nF, false}, {d.rho_b, 1, 8ull * g.nB, false},
+        {d.cfl_part, 1, 16ull * cfl_blocks(g.nIF), false}, {d.dt, 1, 32, false}};
     int rc = 0;
     if (save) {
-        HIPCHK(hipStreamSynchronize(h->stream));
         if (std::fwrite(&hd, sizeof hd, 1, fp) != 1) rc = fail("checkpoint: write failed");
     } else {
         CkptHeader in{};
-        if (std::fread(&in, sizeof in, 1, fp) != 1 || std::memcmp(in.magic, "DUGKSCK", 8) != 0 || in.version != 2)
+        if (std::fread(&in, sizeof in, 1, fp) != 1 || std::memcmp(in.magic, "DUGKSCK", 8) != 0)
             rc = fail("checkpoint: not a checkpoint file");
+        else if (in.version != kCkptVersion)
+            rc = fail("checkpoint: file format version " + std::to_string(in.version) + ", this library reads " +
+                      std::to_string(kCkptVersion));
         else if (in.nC != hd.nC || in.nF != hd.nF || in.nB != hd.nB || in.Nv != hd.Nv || in.es != hd.es)
             rc = fail("checkpoint: sizes or precision differ from this handle");
         else if (in.ref_sum_g != hd.ref_sum_g || in.ref_sum_h != hd.ref_sum_h)
             rc = fail("checkpoint: storage reference (Tref, R, K) differs from this handle");
+        else if (in.geo_sum != hd.geo_sum)
+            rc = fail("checkpoint: the mesh (cell centres / volumes in the caller's order) differs from this handle's");
     }
-    const size_t chunk = 64ull << 20;
-    std::vector<char> buf;
+    std::vector<char> buf, perm_buf;
     for (size_t bi = 0; rc == 0 && bi < blocks.size(); ++bi) {
-        if (!blocks[bi].p || blocks[bi].bytes == 0) continue;
-        for (size_t off = 0; off < blocks[bi].bytes && rc == 0; off += chunk) {
-            const size_t n = std::min(chunk, blocks[bi].bytes - off);
-            buf.resize(n);
-            char *dev = (char *)blocks[bi].p + off;
+        const Blk &b = blocks[bi];
+        if (!b.p) continue;
+        if (!b.cells) {  // one plain byte range
+            buf.resize(b.elem);
             if (save) {
-                if (hipMemcpy(buf.data(), dev, n, hipMemcpyDeviceToHost) != hipSuccess) rc = fail("checkpoint: copy");
-                else if (std::fwrite(buf.data(), 1, n, fp) != n) rc = fail("checkpoint: write failed");
+                if (hipMemcpy(buf.data(), b.p, b.elem, hipMemcpyDeviceToHost) != hipSuccess) rc = fail("checkpoint: copy");
+                else if (std::fwrite(buf.data(), 1, b.elem, fp) != b.elem) rc = fail("checkpoint: write failed");
             } else {
-                if (std::fread(buf.data(), 1, n, fp) != n) rc = fail("checkpoint: truncated file");
-                else if (hipMemcpy(dev, buf.data(), n, hipMemcpyHostToDevice) != hipSuccess) rc = fail("checkpoint: copy");
+                if (std::fread(buf.data(), 1, b.elem, fp) != b.elem) rc = fail("checkpoint: truncated file");
+                else if (hipMemcpy(b.p, buf.data(), b.elem, hipMemcpyHostToDevice) != hipSuccess) rc = fail("checkpoint: copy");
+            }
+            continue;
+        }
+        // cell-indexed rows, a few rows per transfer (about 64 MiB)
+        const size_t row = nC * b.elem;
+        const size_t rpc = std::max<size_t>(1, (64ull << 20) / row);
+        for (size_t r0 = 0; r0 < b.rows && rc == 0; r0 += rpc) {
+            const size_t nr = std::min(rpc, b.rows - r0);
+            const size_t n = nr * row;
+            char *dev = (char *)b.p + r0 * row;
+            buf.resize(n);
+            perm_buf.resize(n);
+            if (save) {
+                if (hipMemcpy(buf.data(), dev, n, hipMemcpyDeviceToHost) != hipSuccess) {
+                    rc = fail("checkpoint: copy");
+                    break;
+                }
+                for (size_t r = 0; r < nr; ++r)
+                    for (size_t i = 0; i < nC; ++i)
+                        std::memcpy(&perm_buf[r * row + (size_t)h->perm[i] * b.elem], &buf[r * row + i * b.elem], b.elem);
+                if (std::fwrite(perm_buf.data(), 1, n, fp) != n) rc = fail("checkpoint: write failed");
+            } else {
+                if (std::fread(perm_buf.data(), 1, n, fp) != n) {
+                    rc = fail("checkpoint: truncated file");
+                    break;
+                }
+                for (size_t r = 0; r < nr; ++r)
+                    for (size_t i = 0; i < nC; ++i)
+                        std::memcpy(&buf[r * row + i * b.elem], &perm_buf[r * row + (size_t)h->perm[i] * b.elem], b.elem);
+                if (hipMemcpy(dev, buf.data(), n, hipMemcpyHostToDevice) != hipSuccess) rc = fail("checkpoint: copy");
             }
         }
     }

@@ -61,16 +61,20 @@ def test_tiny_cavity_fp64_matches_oracle(fused):
     assert field_rel_err(hg_, ho_) < 1e-12
 
 
-@pytest.mark.parametrize("fused", [0, 2, 3])
-def test_tiny_cavity_odd_grid_zero_normal_velocity(fused):
+@pytest.mark.parametrize("fused,precision", [(0, 0), (2, 0), (3, 0), (2, 1), (3, 1)])
+def test_tiny_cavity_odd_grid_zero_normal_velocity(fused, precision):
     """Odd Newton-Cotes grid: xi_d = 0 exists, so xi.Sf = 0 on axis-aligned faces and the
-    averaging branch of the internal reconstruction (fvDVM.C:847-863) is exercised."""
+    averaging branch of the internal reconstruction (fvDVM.C:847-863) is exercised; with
+    FP32 storage (precision 1) the tangent branch of each fused schedule's FP32 pass A runs
+    too (halved moments, no stored value), checked against the mixed budget."""
     c = _cases().cavity_case(5, "NC", 9, 4.0, 0.3)
-    o, g, ho, hg = _run_pair(c, 20, 0, fused)
+    o, g, ho, hg = _run_pair(c, 20, precision, fused)
     e = _errs(o, g)
-    print("odd grid", e)
-    for k, v in e.items():
-        assert v < 1e-12, (k, v)
+    print("odd grid", fused, precision, e)
+    tol = 1e-12 if precision == 0 else 1e-5
+    for k in ("rho", "U", "T"):
+        assert e[k] < tol, (k, e[k])
+    assert e["q"] < (1e-12 if precision == 0 else 1e-3)
 
 
 @pytest.mark.parametrize("fused", [0, 2, 3])
@@ -188,6 +192,39 @@ def test_small_sphere_mixed_matches_oracle():
         assert e[k] < 1e-5, (k, e[k])
 
 
+def test_small_sphere_odd_velocity_grid_mixed_matches_oracle():
+    """3-D with an odd Simpson 3/8 set (xi_d = 0 on every axis): velocities tangent to the
+    axis-aligned box faces and to the O-grid shell faces reach the FP32 tangent branch of
+    the face-value pass A."""
+    c = _cases().sphere_case(SMALL_SPHERE, "NC38", 7, 4.0, 0.1)
+    o, g, ho, hg = _run_pair(c, 10, 1, 1)
+    e = _errs(o, g)
+    print("small sphere odd grid mixed", e)
+    for k in ("rho", "U", "T"):
+        assert e[k] < 1e-5, (k, e[k])
+    assert e["q"] < 1e-3
+
+
+@pytest.mark.parametrize("fused", [0, 1])
+def test_high_mach_sphere_fp64_matches_oracle(fused):
+    """Ma = 5 freestream (U = 6.45 sqrt(R T0)) past the small sphere, 22^3 trapezoid set on
+    [-14, 14]: a bow shock forms at the wall and q reaches ~20 while rho U^3 ~ 270, so the
+    raw power sums the passes accumulate (sum w g xi^3) cancel strongly when q is formed
+    about U.  FP64 GPU vs oracle after 5 steps: <= 1e-10 (max-norm relative) in rho, U, T, q,
+    cell and face fields, and the wall density."""
+    c = _cases().sphere_case(SMALL_SPHERE, "trapezoid", 22, 14.0, 0.1, Ma=5.0)
+    o, g, ho, hg = _run_pair(c, 5, 0, fused)
+    e = _errs(o, g)
+    fo, fg = o.face_macros(), g.face_macros()
+    ef = {k: field_rel_err(fg[k], fo[k]) for k in ("rho", "U", "T", "q")}
+    print("high-Ma sphere fp64", fused, e, ef, flush=True)
+    assert np.max(np.abs(o.cell_macros()["q"])) > 10.0
+    for k in ("rho", "U", "T", "q"):
+        assert e[k] < 1e-10, (k, e[k])
+        assert ef[k] < 1e-10, (k, ef[k])
+    assert field_rel_err(g.boundary_rho(), o.wall_rho()) < 1e-10
+
+
 def test_full_size_cavity_matches_oracle():
     """The 256^2 cavity with the 64^2 trapezoid set on [-5, 5] (Kn 0.1) at full size, 3 steps
     from the same state: both default schedules (FP64: stored face values, 1e-10; FP32
@@ -260,6 +297,48 @@ def test_mixed_storage_drift_stays_small(fused):
     print("mixed drift", fused, e)
     for k in ("rho", "U", "T"):
         assert e[k] < 1e-6, (k, e[k])
+
+
+def test_checkpoint_is_in_caller_cell_order_and_moves_between_paths(tmp_path):
+    """A checkpoint holds g~ in the caller's cell order (the RCB block numbering inside a
+    handle depends on its path and schedule), so a file saved by the staged path loads into
+    both fused schedules with the same fields, and the runs continue alike; a mesh with the
+    same counts but other cells is refused."""
+    from dugks_amd import hip, mesh as meshmod
+    from test_oracle_pins import _distorted_cavity
+    c = _cases().cavity_case(36, "NC38", 10, 4.0, 0.1)
+    a = hip.FvDVM(c, fused=0)
+    a.run(6)
+    path = tmp_path / "ck.bin"
+    a.checkpoint_save(path)
+    g0, h0 = a.df()
+    m0 = a.cell_macros()
+    nC, Nv = a.nC, a.Nv
+    raw = np.fromfile(path, dtype=np.float64, count=2 * Nv * nC, offset=64).reshape(2, Nv, nC)
+    assert np.array_equal(raw[0], g0) and np.array_equal(raw[1], h0)
+    a.dt_saved = a.dt
+    a.run(4)
+    ma = a.cell_macros()
+    for fused in (2, 3):
+        b = hip.FvDVM(c, fused=fused)
+        b.checkpoint_load(path)
+        gb, hb = b.df()
+        assert np.array_equal(gb, g0) and np.array_equal(hb, h0)
+        mb = b.cell_macros()
+        for k in m0:
+            assert np.array_equal(mb[k], m0[k]), k
+        b.dt = a.dt_saved
+        b.run(4)
+        mb = b.cell_macros()
+        for k in ("rho", "U", "T", "q"):
+            assert field_rel_err(mb[k], ma[k]) < 1e-12, (fused, k)
+        b.close()
+    c2 = _cases().cavity_case(36, "NC38", 10, 4.0, 0.1)
+    c2.mesh = _distorted_cavity(36, seed=2)
+    c2.geo = meshmod.geometry(c2.mesh)
+    other = hip.FvDVM(c2, fused=0)
+    with pytest.raises(hip.DugksError, match="mesh"):
+        other.checkpoint_load(path)
 
 
 def test_checkpoint_refuses_other_storage_reference(tmp_path):
