@@ -127,6 +127,19 @@ int dugks_evolution(dugks_handle *h, double dt);
 int dugks_advance(dugks_handle *h, int n_steps, int adjust, double max_co, double max_delta_t,
                   double *dt, double *dt_history);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * dugks_advance with the same arguments and results, which calls no hipMalloc, hipFree or
+ * allocating library routine.  dugks_advance keeps the device time-step history in a
+ * handle buffer that it reallocates when a call asks for more steps than before; this
+ * variant keeps it in a fixed per-GPU array of 65536 steps (512 KiB, held by each GPU
+ * the library runs on until the process ends) and fails with an error when dt_history is
+ * requested for more steps.  Every call on one GPU shares that array: calls that overlap
+ * on one GPU (two handles, two host threads) race on it.  All other device memory the
+ * step uses is the handle's, allocated by dugks_create. */
+int dugks_advance_noalloc(dugks_handle *h, int n_steps, int adjust, double max_co, double max_delta_t,
+                          double *dt, double *dt_history);
+
 int dugks_synchronize(dugks_handle *h);
 
 /* field output (host copies; blocking).  Cell fields [n_cells], face fields [nIF + nB]. */

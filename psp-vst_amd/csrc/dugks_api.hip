@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -216,6 +217,9 @@ int do_step(dugks_handle *h)
     return h->es == 4 ? staged_step<float>(h) : staged_step<double>(h);
 }
 
+// resolves the per-device address of the allocation-free entry point's step-history array
+int resolve_scratch(int device);
+
 int check_handle(dugks_handle *h, bool need_init = true)
 {
     if (!h) return fail("null handle");
@@ -391,6 +395,7 @@ int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patch
         return rc;
     };
     if (hipSetDevice(h->device) != hipSuccess) return bail(fail("dugks_create: hipSetDevice failed"));
+    if (resolve_scratch(h->device)) return bail(1);
     if (hipStreamCreate(&h->stream) != hipSuccess) return bail(fail("dugks_create: hipStreamCreate failed"));
     DevState &d = h->d;
     hipStream_t st = h->stream;
@@ -557,13 +562,76 @@ int dugks_evolution(dugks_handle *h, double dt)
     return do_step(h);
 }
 
+}  // extern "C"
+
+namespace {
+
+int advance_check(int n_steps, int adjust, double max_co, double max_delta_t, const double *dt, const char *who)
+{
+    if (n_steps < 0) return fail(std::string(who) + ": negative step count");
+    if (!dt || !(*dt > 0)) return fail(std::string(who) + ": *dt must be a positive time step");
+    if (adjust && (!(max_co > 0) || !(max_delta_t > 0)))
+        return fail(std::string(who) + ": maxCo and maxDeltaT must be > 0");
+    return 0;
+}
+
+// The loop of dugks_advance with the device time-step history in `hist` (NULL: none)
+int advance_loop(dugks_handle *h, int n_steps, int adjust, double max_co, double max_delta_t, double *dt,
+                 double *dt_history, double *hist)
+{
+    DevState &d = h->d;
+    hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->stream, d.dt, *dt);
+    for (int i = 0; i < n_steps; ++i) {
+        PHASE(kPhDt, launch_dt_update(h->g, d.cfl_part, d.dt, adjust, max_co, max_delta_t, hist, i, h->stream));
+        if (do_step(h)) return 1;
+    }
+    HIPCHK(hipMemcpyAsync(dt, d.dt, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (dt_history && n_steps > 0)
+        HIPCHK(hipMemcpyAsync(dt_history, hist, sizeof(double) * n_steps, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+// Time-step history of dugks_advance_noalloc: a file-scope device array that every GPU the
+// library runs on holds for the life of the process (512 KiB).  In host code its name is
+// only a stand-in for the symbol: the address each device holds it at comes from
+// hipGetSymbolAddress, called once per device while dugks_create has that device current,
+// and only that address is passed to kernels or copies.
+constexpr int kNoAllocMaxSteps = 65536;
+static __device__ __align__(256) double scratch[kNoAllocMaxSteps];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+int resolve_scratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return fail("device ordinal beyond the library's table of 64");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return 0;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(std::string("hipGetSymbolAddress of the step-history array failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = (double *)ptr;
+    return 0;
+}
+
+double *scratch_of(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
+}  // namespace
+
+extern "C" {
+
 int dugks_advance(dugks_handle *h, int n_steps, int adjust, double max_co, double max_delta_t, double *dt,
                   double *dt_history)
 {
     if (check_handle(h)) return 1;
-    if (n_steps < 0) return fail("dugks_advance: negative step count");
-    if (!dt || !(*dt > 0)) return fail("dugks_advance: *dt must be a positive time step");
-    if (adjust && (!(max_co > 0) || !(max_delta_t > 0))) return fail("dugks_advance: maxCo and maxDeltaT must be > 0");
+    if (advance_check(n_steps, adjust, max_co, max_delta_t, dt, "dugks_advance")) return 1;
     DevState &d = h->d;
     if (dt_history && n_steps > h->hist_cap) {
         if (d.dt_hist) {
@@ -575,20 +643,23 @@ int dugks_advance(dugks_handle *h, int n_steps, int adjust, double max_co, doubl
         if (dalloc(h, &d.dt_hist, n_steps)) return 1;
         h->hist_cap = n_steps;
     }
-    hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->stream, d.dt, *dt);
-    for (int i = 0; i < n_steps; ++i) {
-        PHASE(kPhDt, launch_dt_update(h->g, d.cfl_part, d.dt, adjust, max_co, max_delta_t,
-                                      dt_history ? d.dt_hist : nullptr, i, h->stream));
-        if (do_step(h)) return 1;
-    }
-    HIPCHK(hipMemcpyAsync(dt, d.dt, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    return advance_loop(h, n_steps, adjust, max_co, max_delta_t, dt, dt_history, dt_history ? d.dt_hist : nullptr);
+}
+
+int dugks_advance_noalloc(dugks_handle *h, int n_steps, int adjust, double max_co, double max_delta_t, double *dt,
+                          double *dt_history)
+{
+    if (check_handle(h)) return 1;
+    if (advance_check(n_steps, adjust, max_co, max_delta_t, dt, "dugks_advance_noalloc")) return 1;
+    double *hist = nullptr;
     if (dt_history) {
-        if (n_steps > 0)
-            HIPCHK(hipMemcpyAsync(dt_history, d.dt_hist, sizeof(double) * n_steps, hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(hipStreamSynchronize(h->stream));
+        if (n_steps > kNoAllocMaxSteps)
+            return fail("dugks_advance_noalloc: " + std::to_string(n_steps) + " steps with a time-step history exceed "
+                        "its fixed buffer of " + std::to_string(kNoAllocMaxSteps) + " steps; split the call");
+        hist = scratch_of(h->device);
+        if (!hist) return fail("dugks_advance_noalloc: the step-history array of this device was not resolved");
     }
-    HIPCHK(hipStreamSynchronize(h->stream));
-    return 0;
+    return advance_loop(h, n_steps, adjust, max_co, max_delta_t, dt, dt_history, hist);
 }
 
 int dugks_synchronize(dugks_handle *h)

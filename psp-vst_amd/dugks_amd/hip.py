@@ -53,6 +53,7 @@ class Options(ct.Structure):
 
 EXPORTS = [
     "dugks_create", "dugks_init_from_macros", "dugks_co_num", "dugks_evolution", "dugks_advance",
+    "dugks_advance_noalloc",
     "dugks_synchronize", "dugks_get_cell_macros", "dugks_get_face_macros", "dugks_get_boundary_rho",
     "dugks_get_df", "dugks_set_df", "dugks_set_cell_macros", "dugks_checkpoint_save", "dugks_checkpoint_load",
     "dugks_num_velocities", "dugks_velocity_set", "dugks_kernel_count", "dugks_kernel_name", "dugks_profile",
@@ -78,6 +79,7 @@ def load_library(path: str = LIB_PATH):
     L.dugks_co_num.argtypes = [vp, ct.c_double, _D, _D]
     L.dugks_evolution.argtypes = [vp, ct.c_double]
     L.dugks_advance.argtypes = [vp, ct.c_int, ct.c_int, ct.c_double, ct.c_double, _D, _D]
+    L.dugks_advance_noalloc.argtypes = [vp, ct.c_int, ct.c_int, ct.c_double, ct.c_double, _D, _D]
     L.dugks_synchronize.argtypes = [vp]
     L.dugks_get_cell_macros.argtypes = [vp, _D, _D, _D, _D, _D]
     L.dugks_get_face_macros.argtypes = [vp, _D, _D, _D, _D, _D]
@@ -217,6 +219,20 @@ class FvDVM:
             hist.append(self.dt)
             self.evolution()
         return np.array(hist)
+
+    def run_noalloc(self, n_steps: int, record_dt: bool = False):
+        """run() through dugks_advance_noalloc: no device allocation in the call; the time-step
+        history (record_dt) is limited to 65536 steps per call."""
+        c = self.controls
+        dt = ct.c_double(self.dt)
+        hist = np.empty(n_steps) if record_dt else None
+        self._check(self.L.dugks_advance_noalloc(self._h, int(n_steps), int(bool(c.adjustTimeStep)), c.maxCo,
+                                                 c.maxDeltaT, ct.byref(dt), _p(hist)))
+        self.dt = dt.value
+        self.step_index += n_steps
+        if hist is not None:
+            self.time += float(hist.sum())
+        return hist
 
     def run_fixed(self, dts):
         """evolution() with the given time steps (no Courant-number control)."""
