@@ -1,5 +1,7 @@
-"""Build libdugks_hip.so for gfx950 in-tree (psp-vst_amd/dugks_amd/libdugks_hip.so), and the
-streaming probe libdugks_probe.so beside it (HBM counter calibration for the benchmark).
+"""Build libdugks_hip.so for gfx950 in-tree (psp-vst_amd/dugks_amd/libdugks_hip.so), the
+streaming probe libdugks_probe.so beside it (HBM counter calibration for the benchmark), the
+OpenFOAM case library libdugks_foam.so (host C++, g++) and the dugksFoam driver executable
+that runs a case directory through both.
 
 Usage: python psp-vst_amd/build.py [--jobs N] [--verbose]
 
@@ -21,6 +23,12 @@ OUT_DIR = os.path.join(HERE, "dugks_amd")
 LIB = os.path.join(OUT_DIR, "libdugks_hip.so")
 PROBE_LIB = os.path.join(OUT_DIR, "libdugks_probe.so")
 PROBE_SOURCES = ["dugks_probe.hip"]
+FOAM_DIR = os.path.join(HERE, "foam")
+FOAM_LIB = os.path.join(OUT_DIR, "libdugks_foam.so")
+FOAM_SOURCES = ["foam_dict.cpp", "foam_case.cpp", "foam_capi.cpp"]
+DRIVER = os.path.join(OUT_DIR, "dugksFoam")
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-I", os.path.join(HERE, "..", "include")]
+ROCM_LIB = os.environ.get("ROCM_PATH", "/opt/rocm") + "/lib"
 BUILD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("DUGKS_OFFLOAD_ARCH", "gfx950")
 
@@ -63,12 +71,39 @@ def link(lib: str, objs: list, verbose: bool) -> None:
     os.replace(tmp, lib)
 
 
+def run(cmd: list, what: str, verbose: bool) -> None:
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{what} failed\n{r.stdout}\n{r.stderr}")
+
+
+def gxx() -> str:
+    return os.environ.get("CXX") or shutil.which("g++") or "g++"
+
+
+def build_foam(verbose: bool = False) -> None:
+    """libdugks_foam.so (OpenFOAM case I/O, host only) and the dugksFoam driver, which links
+    it and libdugks_hip.so from the same directory (rpath $ORIGIN)."""
+    srcs = [os.path.join(FOAM_DIR, f) for f in FOAM_SOURCES]
+    tmp = FOAM_LIB + ".tmp"
+    run([gxx()] + HOST_FLAGS + ["-shared", "-o", tmp] + srcs + ["-lz"], "libdugks_foam.so build", verbose)
+    os.replace(tmp, FOAM_LIB)
+    tmp = DRIVER + ".tmp"
+    run([gxx()] + HOST_FLAGS + [os.path.join(FOAM_DIR, "dugksFoam.C"), "-o", tmp, "-L", OUT_DIR, "-ldugks_foam",
+                                "-ldugks_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link," + ROCM_LIB],
+        "dugksFoam build", verbose)
+    os.replace(tmp, DRIVER)
+
+
 def build(jobs: int = 4, verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         objs = list(ex.map(lambda s: compile_one(s, verbose), SOURCES + PROBE_SOURCES))
     link(LIB, objs[:len(SOURCES)], verbose)
     link(PROBE_LIB, objs[len(SOURCES):], verbose)
+    build_foam(verbose)
     return LIB
 
 

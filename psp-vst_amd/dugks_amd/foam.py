@@ -1,0 +1,303 @@
+"""OpenFOAM case directories: read through libdugks_foam.so (include/dugks_foam.h), write from
+a :class:`dugks_amd.cases.Case`.
+
+:func:`read_case` returns a Case built from what dugksFoam reads at start-up
+(system/controlDict, constant/polyMesh, constant/DVMProperties, constant/Xis,
+constant/weights and <time>/{rho,U,T}; src/dugksFoam.C:48-54,
+src/fvDVM/fvDVM/fvDVM.C:1928-1948, 62-123), with the mesh geometry computed by the C++
+library (OpenFOAM's primitiveMesh / surfaceInterpolation formulas).  :func:`write_case`
+writes a Case as such a directory (ASCII, optionally gzip-compressed polyMesh files), so a
+generated case can be run by the dugksFoam driver; :func:`read_fields` reads the fields a
+run wrote at one time.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import gzip
+import os
+
+import numpy as np
+
+from . import cases as casesmod
+from . import mesh as meshmod
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libdugks_foam.so")
+DRIVER_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "dugksFoam")
+
+_lib = None
+
+
+class Controls(ct.Structure):
+    _fields_ = [("start_time", ct.c_double), ("end_time", ct.c_double), ("delta_t", ct.c_double),
+                ("write_interval", ct.c_double), ("max_co", ct.c_double), ("max_delta_t", ct.c_double),
+                ("converge_tol", ct.c_double), ("adjust_time_step", ct.c_int), ("converge_check_steps", ct.c_int),
+                ("write_precision", ct.c_int), ("time_precision", ct.c_int), ("write_control", ct.c_int),
+                ("time", ct.c_double)]
+
+
+class Output(ct.Structure):
+    _fields_ = [("time", ct.c_double), ("delta_t", ct.c_double), ("delta_t0", ct.c_double), ("index", ct.c_int),
+                ("rho", ct.POINTER(ct.c_double)), ("U", ct.POINTER(ct.c_double)), ("T", ct.POINTER(ct.c_double)),
+                ("q", ct.POINTER(ct.c_double)), ("rho_b", ct.POINTER(ct.c_double)),
+                ("q_surf", ct.POINTER(ct.c_double))]
+
+
+def load_library(path: str = LIB_PATH):
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python psp-vst_amd/build.py`")
+    L = ct.CDLL(path)
+    vp = ct.c_void_p
+    L.dugks_foam_read.argtypes = [ct.c_char_p, ct.c_char_p, ct.POINTER(vp)]
+    L.dugks_foam_free.argtypes = [vp]
+    L.dugks_foam_free.restype = None
+    L.dugks_foam_last_error.restype = ct.c_char_p
+    L.dugks_foam_array_d.argtypes = [vp, ct.c_char_p, ct.POINTER(ct.POINTER(ct.c_double)), ct.POINTER(ct.c_size_t)]
+    L.dugks_foam_array_i.argtypes = [vp, ct.c_char_p, ct.POINTER(ct.POINTER(ct.c_int)), ct.POINTER(ct.c_size_t)]
+    L.dugks_foam_scalar.argtypes = [vp, ct.c_char_p, ct.POINTER(ct.c_double)]
+    L.dugks_foam_patch.argtypes = [vp, ct.c_int, ct.POINTER(ct.c_char_p), ct.POINTER(ct.c_char_p),
+                                   ct.POINTER(ct.c_char_p), ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]
+    L.dugks_foam_controls_get.argtypes = [vp, ct.POINTER(Controls)]
+    L.dugks_foam_write.argtypes = [vp, ct.POINTER(Output)]
+    L.dugks_foam_write_df.argtypes = [vp, ct.c_double, ct.c_int, ct.POINTER(ct.c_double), ct.c_int]
+    _lib = L
+    return L
+
+
+class FoamError(RuntimeError):
+    pass
+
+
+class _Handle:
+    def __init__(self, case_dir: str, time_name: str | None):
+        self.L = load_library()
+        self.h = ct.c_void_p()
+        if self.L.dugks_foam_read(case_dir.encode(), (time_name or "").encode(), ct.byref(self.h)) != 0:
+            raise FoamError(self.L.dugks_foam_last_error().decode())
+
+    def close(self):
+        if self.h:
+            self.L.dugks_foam_free(self.h)
+            self.h = ct.c_void_p()
+
+    def d(self, name, shape=None):
+        p, n = ct.POINTER(ct.c_double)(), ct.c_size_t()
+        if self.L.dugks_foam_array_d(self.h, name.encode(), ct.byref(p), ct.byref(n)) != 0:
+            raise FoamError(self.L.dugks_foam_last_error().decode())
+        a = np.ctypeslib.as_array(p, shape=(n.value,)).copy() if n.value else np.zeros(0)
+        return a.reshape(shape) if shape else a
+
+    def i(self, name):
+        p, n = ct.POINTER(ct.c_int)(), ct.c_size_t()
+        if self.L.dugks_foam_array_i(self.h, name.encode(), ct.byref(p), ct.byref(n)) != 0:
+            raise FoamError(self.L.dugks_foam_last_error().decode())
+        return np.ctypeslib.as_array(p, shape=(n.value,)).astype(np.int64) if n.value else np.zeros(0, np.int64)
+
+    def s(self, name):
+        v = ct.c_double()
+        if self.L.dugks_foam_scalar(self.h, name.encode(), ct.byref(v)) != 0:
+            raise FoamError(self.L.dugks_foam_last_error().decode())
+        return v.value
+
+    def patch(self, k):
+        nm, gt, rt = ct.c_char_p(), ct.c_char_p(), ct.c_char_p()
+        s, z = ct.c_int(), ct.c_int()
+        if self.L.dugks_foam_patch(self.h, k, ct.byref(nm), ct.byref(gt), ct.byref(rt), ct.byref(s), ct.byref(z)):
+            raise FoamError(self.L.dugks_foam_last_error().decode())
+        return nm.value.decode(), gt.value.decode(), rt.value.decode(), s.value, z.value
+
+    def controls(self):
+        c = Controls()
+        if self.L.dugks_foam_controls_get(self.h, ct.byref(c)) != 0:
+            raise FoamError(self.L.dugks_foam_last_error().decode())
+        return c
+
+
+def read_case(case_dir: str, time_name: str | None = None) -> casesmod.Case:
+    """The case in case_dir as a Case (nondimensional or SI values as the files hold them)."""
+    h = _Handle(case_dir, time_name)
+    try:
+        nC = int(h.s("n_cells"))
+        nIF = int(h.s("n_internal_faces"))
+        pm = meshmod.PolyMesh(points=h.d("points", (-1, 3)), face_offsets=h.i("face_offsets"),
+                              face_points=h.i("face_points"), owner=h.i("owner"), neighbour=h.i("neighbour"),
+                              n_cells=nC)
+        bcs, off = [], 0
+        bf_rho, bf_U, bf_T = h.d("bf_rho"), h.d("bf_U", (-1, 3)), h.d("bf_T")
+        for k in range(int(h.s("n_patches"))):
+            name, gtype, rtype, start, size = h.patch(k)
+            pm.patches.append(meshmod.Patch(name=name, geometric_type="empty" if gtype == "empty" else gtype,
+                                            start=start, size=size))
+            if gtype == "empty" or size == 0:
+                continue
+            bcs.append(casesmod.PatchBC(name, rtype, bf_rho[off:off + size].copy(), bf_U[off:off + size].copy(),
+                                        bf_T[off:off + size].copy()))
+            off += size
+        Sf, Cf = h.d("Sf", (-1, 3)), h.d("Cf", (-1, 3))
+        geo = meshmod.Geometry(Sf=Sf, Cf=Cf, magSf=np.linalg.norm(Sf, axis=1), C=h.d("C", (-1, 3)), V=h.d("V"),
+                               weights=h.d("weights"), delta_coeffs=h.d("delta_coeffs"))
+        gas = casesmod.Gas(R=h.s("R"), omega=h.s("omega"), Tref=h.s("Tref"), muRef=h.s("muRef"), Pr=h.s("Pr"),
+                           KInner=int(h.s("KInner")))
+        ctl = h.controls()
+        controls = casesmod.Controls(deltaT=ctl.delta_t, adjustTimeStep=bool(ctl.adjust_time_step), maxCo=ctl.max_co,
+                                     maxDeltaT=ctl.max_delta_t)
+        c = casesmod.Case(name=os.path.basename(os.path.abspath(case_dir)), mesh=pm, geo=geo,
+                          D=int(h.s("n_solution_d")), xis=h.d("xis"), wts=h.d("xi_weights"), xi_max=h.s("xiMax"),
+                          xi_min=h.s("xiMin"), gas=gas, rho0=h.d("rho"), U0=h.d("U", (-1, 3)), T0=h.d("T"),
+                          bcs=bcs, controls=controls,
+                          meta=dict(case_dir=case_dir, time=h.s("time"), end_time=ctl.end_time,
+                                    converge_tol=ctl.converge_tol, converge_check_steps=ctl.converge_check_steps,
+                                    df_write_cells=h.i("df_write_cells").tolist()))
+        assert c.n_cells == nC and pm.n_internal_faces == nIF
+        return c
+    finally:
+        h.close()
+
+
+# ------------------------------------------------------------------------------- writing
+def _header(cls: str, location: str, obj: str) -> str:
+    return ("// DUGKS case (OpenFOAM ASCII format)\nFoamFile\n{\n    version     2.0;\n    format      ascii;\n"
+            f"    class       {cls};\n    location    \"{location}\";\n    object      {obj};\n}}\n\n")
+
+
+def _num(x) -> str:
+    return repr(float(x))
+
+
+def _open(path: str, compress: bool):
+    if compress:
+        return gzip.open(path + ".gz", "wt")
+    return open(path, "w")
+
+
+def _values(vals: np.ndarray, vtype: str) -> str:
+    vals = np.asarray(vals, float)
+    if vals.ndim == 1:
+        body = "\n".join(_num(v) for v in vals)
+    else:
+        body = "\n".join("(" + " ".join(_num(x) for x in row) + ")" for row in vals)
+    return f"nonuniform List<{vtype}> \n{len(vals)}\n(\n{body}\n)\n"
+
+
+def write_case(case: casesmod.Case, case_dir: str, end_time: float = 1.0e300, write_interval: int = 1000000,
+               delta_t: float | None = None, compress_mesh: bool = False, converge_tol: float = -1.0,
+               converge_check_steps: int = 1000000, write_precision: int = 17, df_write_cells=()) -> str:
+    """Writes `case` as an OpenFOAM case directory that dugksFoam (and read_case) takes:
+    polyMesh, DVMProperties (values as the case holds them, with dimension sets), Xis,
+    weights, controlDict (the case's time controls; writeControl timeStep) and 0/{rho,U,T}.
+    The default convergeTol of -1 never stops the loop on convergence."""
+    m = case.mesh
+    pmdir = os.path.join(case_dir, "constant", "polyMesh")
+    os.makedirs(pmdir, exist_ok=True)
+    os.makedirs(os.path.join(case_dir, "system"), exist_ok=True)
+    os.makedirs(os.path.join(case_dir, "0"), exist_ok=True)
+    with _open(os.path.join(pmdir, "points"), compress_mesh) as f:
+        f.write(_header("vectorField", "constant/polyMesh", "points"))
+        f.write(f"{len(m.points)}\n(\n" + "\n".join("(" + " ".join(_num(x) for x in p) + ")" for p in m.points)
+                + "\n)\n")
+    with _open(os.path.join(pmdir, "faces"), compress_mesh) as f:
+        f.write(_header("faceList", "constant/polyMesh", "faces"))
+        lines = []
+        for i in range(m.n_faces):
+            fp = m.face(i)
+            lines.append(f"{len(fp)}(" + " ".join(str(int(x)) for x in fp) + ")")
+        f.write(f"{m.n_faces}\n(\n" + "\n".join(lines) + "\n)\n")
+    for name, arr in (("owner", m.owner), ("neighbour", m.neighbour)):
+        with _open(os.path.join(pmdir, name), compress_mesh) as f:
+            f.write(_header("labelList", "constant/polyMesh", name))
+            f.write(f"{len(arr)}\n(\n" + "\n".join(str(int(x)) for x in arr) + "\n)\n")
+    with open(os.path.join(pmdir, "boundary"), "w") as f:
+        f.write(_header("polyBoundaryMesh", "constant/polyMesh", "boundary"))
+        f.write(f"{len(m.patches)}\n(\n")
+        for p in m.patches:
+            f.write(f"    {p.name}\n    {{\n        type            {p.geometric_type};\n"
+                    f"        nFaces          {p.size};\n        startFace       {p.start};\n    }}\n")
+        f.write(")\n")
+    g = case.gas
+    with open(os.path.join(case_dir, "constant", "DVMProperties"), "w") as f:
+        f.write(_header("dictionary", "constant", "DVMProperties"))
+        f.write("fvDVMparas\n{\n"
+                f"    xiMax       xiMax [0 1 -1 0 0 0 0] {_num(case.xi_max)};\n"
+                f"    xiMin       xiMin [0 1 -1 0 0 0 0] {_num(case.xi_min)};\n"
+                f"    nDV         {len(case.xis)};\n    macroFlux   no;\n}}\n\n"
+                "gasProperties\n{\n"
+                f"    R           R [0 2 -2 -1 0 0 0] {_num(g.R)};\n    omega       {_num(g.omega)};\n"
+                f"    Tref        Tref [0 0 0 1 0 0 0] {_num(g.Tref)};\n"
+                f"    muRef       muRef [1 -1 -1 0 0 0 0] {_num(g.muRef)};\n    Pr          {_num(g.Pr)};\n"
+                f"    KInner      {int(g.KInner)};\n}}\n")
+        if len(df_write_cells):
+            f.write(f"\nDFwriteCellList {len(df_write_cells)}(" + " ".join(str(int(x)) for x in df_write_cells)
+                    + ");\n")
+    for name, vals in (("Xis", case.xis), ("weights", case.wts)):
+        with open(os.path.join(case_dir, "constant", name), "w") as f:
+            f.write(_header("scalarList", "constant", name))
+            f.write(f"{len(vals)}\n(\n" + "\n".join(_num(v) for v in vals) + "\n)\n")
+    c = case.controls
+    with open(os.path.join(case_dir, "system", "controlDict"), "w") as f:
+        f.write(_header("dictionary", "system", "controlDict"))
+        f.write(f"application     dugksFoam;\nstartFrom       startTime;\nstartTime       0;\nstopAt          endTime;\n"
+                f"endTime         {_num(end_time)};\ndeltaT          {_num(delta_t if delta_t else c.deltaT)};\n"
+                f"writeControl    timeStep;\nwriteInterval   {int(write_interval)};\nwriteFormat     ascii;\n"
+                f"writePrecision  {int(write_precision)};\ntimeFormat      general;\ntimePrecision   12;\n"
+                f"adjustTimeStep  {'yes' if c.adjustTimeStep else 'no'};\nmaxCo           {_num(c.maxCo)};\n"
+                f"maxDeltaT       {_num(c.maxDeltaT)};\nconvergeTol     {_num(converge_tol)};\n"
+                f"convergeCheckSteps {int(converge_check_steps)};\n")
+    bc_by_name = {b.name: b for b in case.bcs}
+    for name, cls, vtype, dims, internal in (
+            ("rho", "volScalarField", "scalar", "[1 -3 0 0 0 0 0]", case.rho0),
+            ("U", "volVectorField", "vector", "[0 1 -1 0 0 0 0]", case.U0),
+            ("T", "volScalarField", "scalar", "[0 0 0 1 0 0 0]", case.T0)):
+        with open(os.path.join(case_dir, "0", name), "w") as f:
+            f.write(_header(cls, "0", name))
+            f.write(f"dimensions      {dims};\n\ninternalField   " + _values(internal, vtype) + ";\n\nboundaryField\n{\n")
+            for p in m.patches:
+                if p.geometric_type == "empty" or p.size == 0:
+                    f.write(f"    {p.name}\n    {{\n        type            empty;\n    }}\n")
+                    continue
+                b = bc_by_name[p.name]
+                vals = {"rho": np.broadcast_to(b.rho, (p.size,)), "U": np.broadcast_to(b.U, (p.size, 3)),
+                        "T": np.broadcast_to(b.T, (p.size,))}[name]
+                ptype = b.rho_type if name == "rho" else "fixedValue"
+                f.write(f"    {p.name}\n    {{\n        type            {ptype};\n        value           "
+                        + _values(vals, vtype) + ";\n    }\n")
+            f.write("}\n")
+    return case_dir
+
+
+def read_fields(case_dir: str, time_name: str) -> dict:
+    """rho, U, T (cells) of a written time, read through the library (the time directory
+    must hold rho, U, T; mesh and dictionaries from the case)."""
+    h = _Handle(case_dir, time_name)
+    try:
+        return dict(rho=h.d("rho"), U=h.d("U", (-1, 3)), T=h.d("T"), time=h.s("time"))
+    finally:
+        h.close()
+
+
+def write_fields(case_dir: str, time: float, index: int, delta_t: float, delta_t0: float, rho, U, T, q,
+                 rho_b=None, q_surf=None, time_name: str | None = None) -> None:
+    """Writes one output time of the case in case_dir through the library (the files a
+    dugksFoam run writes): cell fields in the polyMesh numbering, rho_b [nB] and q_surf
+    [nIF + nB, 3] in the C-ABI face numbering (or None)."""
+    h = _Handle(case_dir, time_name)
+    try:
+        f = lambda x: None if x is None else np.ascontiguousarray(x, dtype=np.float64)
+        keep = [f(rho), f(U), f(T), f(q), f(rho_b), f(q_surf)]
+        P = lambda x: None if x is None else x.ctypes.data_as(ct.POINTER(ct.c_double))
+        o = Output(time=time, delta_t=delta_t, delta_t0=delta_t0, index=index, rho=P(keep[0]), U=P(keep[1]),
+                   T=P(keep[2]), q=P(keep[3]), rho_b=P(keep[4]), q_surf=P(keep[5]))
+        if h.L.dugks_foam_write(h.h, ct.byref(o)) != 0:
+            raise FoamError(h.L.dugks_foam_last_error().decode())
+    finally:
+        h.close()
+
+
+def run_driver(case_dir: str, *args: str, timeout: float = 600.0):
+    """Runs the dugksFoam executable on a case directory; returns (returncode, stdout, stderr)."""
+    import subprocess
+    if not os.path.exists(DRIVER_PATH):
+        raise RuntimeError(f"{DRIVER_PATH} is missing: build it with `python psp-vst_amd/build.py`")
+    r = subprocess.run([DRIVER_PATH, "-case", case_dir, *args], capture_output=True, text=True, timeout=timeout)
+    return r.returncode, r.stdout, r.stderr
