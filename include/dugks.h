@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define DUGKS_ABI_VERSION 1
+#define DUGKS_ABI_VERSION 2
 
 /* storage precision of the distributions */
 enum dugks_precision {
@@ -33,8 +33,23 @@ enum dugks_precision {
 /* rho boundary types (0/rho "type" entries) understood by the path.  The DF face type
  * follows the map of src/fvDVM/discreteVelocity/discreteVelocity.C:188-195. */
 enum dugks_patch_type {
-    DUGKS_PATCH_MAXWELL_WALL = 1,  /* rho calculatedMaxwell -> DF maxwellWall (diffuse wall) */
-    DUGKS_PATCH_MIXED = 2          /* rho fixedValue        -> DF mixed (frozen incoming) */
+    DUGKS_PATCH_MAXWELL_WALL = 1,   /* rho calculatedMaxwell -> DF maxwellWall (diffuse wall) */
+    DUGKS_PATCH_MIXED = 2,          /* rho fixedValue        -> DF mixed (frozen incoming) */
+    DUGKS_PATCH_ZERO_GRADIENT = 3,  /* rho zeroGradient      -> DF zeroGradient: every entry of
+                                       the face takes the owner cell's f-bar-plus (fvDVM.C:887-891) */
+    DUGKS_PATCH_FAR_FIELD = 4,      /* rho farField          -> DF farField: incoming entries are
+                                       rho_p M(U_owner, T_p) (fvDVM.C:910-940) */
+    DUGKS_PATCH_PRESSURE_IN = 5,    /* rho pressureIn        -> DF farField, rho_p = p/(R T_p) set
+                                       after every step (fvDVM.C:1758-1784) */
+    DUGKS_PATCH_PRESSURE_OUT = 6,   /* rho pressureOut       -> DF farField, rho_p and T_p from the
+                                       outlet characteristic after every step (fvDVM.C:1785-1819) */
+    DUGKS_PATCH_SYMMETRY = 7,       /* rho symmetryMod       -> DF DVMsymmetry: incoming entries copy
+                                       the mirror velocity's face value; the face relaxation then
+                                       relaxes outgoing entries and the whole face once more
+                                       (discreteVelocity.C:725-749, fvDVM.C:1431-1456) */
+    DUGKS_PATCH_SYMMETRY_PLANE = 8  /* geometric symmetryPlane -> DF symmetryPlane: incoming entries
+                                       copy the mirror velocity's face value, outgoing entries are
+                                       relaxed (discreteVelocity.C:750-770) */
 };
 
 /* Mesh geometry as OpenFOAM's fvMesh provides it (mesh.owner(), neighbour(), Sf(), Cf(),
@@ -65,9 +80,12 @@ typedef struct dugks_patch {
     int type;                  /* enum dugks_patch_type */
     int start;                 /* first face, counted within the boundary faces */
     int size;
-    const double *rho;         /* [size] patch rho (mixed: fixed value; wall: initial) */
-    const double *U;           /* [size][3] fixed U */
-    const double *T;           /* [size] fixed T */
+    const double *rho;         /* [size] patch rho (mixed, farField: fixed value; wall: initial;
+                                  pressure patches: initial, then updated) */
+    const double *U;           /* [size][3] fixed U (walls, mixed) */
+    const double *T;           /* [size] fixed T (walls, mixed, farField, pressureIn; pressureOut:
+                                  initial, then updated) */
+    double pressure;           /* pressureIn / pressureOut patches: the patch pressure */
 } dugks_patch;
 
 /* constant/Xis, constant/weights and the fvDVMparas entries (fvDVM.C:1928-1933). */

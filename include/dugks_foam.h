@@ -59,7 +59,8 @@ int dugks_foam_controls_get(const dugks_foam_case *c, dugks_foam_controls *ctl);
 /* Introspection (used by bindings): named arrays and scalars of the case.
  * double arrays: "points" [nP][3], "Sf", "Cf" [nF][3] (polyMesh faces), "C" [nC][3], "V",
  *   "weights", "delta_coeffs" [nIF], "xis", "xi_weights" [nDV], "rho", "T" [nC], "U" [nC][3],
- *   "bf_rho", "bf_T" [nB], "bf_U" [nB][3] (C-ABI boundary faces);
+ *   "bf_rho", "bf_T" [nB], "bf_U" [nB][3] (C-ABI boundary faces), "patch_pressure" [non-empty
+ *   patches] (pressureIn / pressureOut);
  * int arrays: "face_offsets" [nF+1], "face_points", "owner" [nF], "neighbour" [nIF],
  *   "abi_faces" [nIF + nB] (C-ABI face -> polyMesh face), "df_write_cells";
  * scalars: "n_cells", "n_faces", "n_internal_faces", "n_boundary_faces", "n_solution_d",
@@ -68,6 +69,9 @@ int dugks_foam_controls_get(const dugks_foam_case *c, dugks_foam_controls *ctl);
 int dugks_foam_array_d(const dugks_foam_case *c, const char *name, const double **data, size_t *count);
 int dugks_foam_array_i(const dugks_foam_case *c, const char *name, const int **data, size_t *count);
 int dugks_foam_scalar(const dugks_foam_case *c, const char *name, double *value);
+/* gradSchemes default of system/fvSchemes ("leastSquares" or "Gauss linear"; the solver
+ * inputs carry the Gauss linear gradient as ls_own / ls_nei vectors) */
+const char *dugks_foam_grad_scheme(const dugks_foam_case *c);
 /* polyMesh patch i (all patches, empty ones included): name, geometric type, the rho
  * boundary type of the time read, face range; strings live until dugks_foam_free */
 int dugks_foam_patch(const dugks_foam_case *c, int i, const char **name, const char **geom_type,

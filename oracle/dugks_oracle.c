@@ -52,6 +52,10 @@ struct orc_state {
     double *in_by_rho, *out_going;  /* [nB] */
     double *own_ls, *nei_ls, *patch_ls;
     int *bf_type, *bf_local;        /* [nB] */
+    double *bf_p;                   /* [nB] pressure of pressureIn / pressureOut faces */
+    int *bf_mirror;                 /* [nB] symmetry faces: patch axis (0, 1, 2), else -1 */
+    double *bf_sf0;                 /* [nB][3] symmetry faces: Sf of the patch's first face */
+    int *mirror;                    /* [3][Nv] velocity mirrored about each axis (symmetric sets) */
     /* owned copies of the case arrays */
     int *owner, *neighbour, *ptype, *pstart, *psize;
     double *Sf, *Cf, *C, *V, *wts_if, *dc, *bf_rho, *bf_U, *bf_T, *xis, *wts;
@@ -212,10 +216,17 @@ orc_state *orc_create(const orc_case *cs)
 
     s->bf_type = calloc((size_t)nB + 1, sizeof(int));
     s->bf_local = calloc((size_t)nB + 1, sizeof(int));
-    for (int b = 0; b < nB; ++b) s->bf_type[b] = -1;
+    s->bf_p = calloc((size_t)nB + 1, sizeof(double));
+    s->bf_mirror = calloc((size_t)nB + 1, sizeof(int));
+    s->bf_sf0 = calloc(3 * (size_t)nB + 3, sizeof(double));
+    for (int b = 0; b < nB; ++b) {
+        s->bf_type[b] = -1;
+        s->bf_mirror[b] = -1;
+    }
+    int any_sym = 0;
     for (int p = 0; p < cs->n_patches; ++p) {
         int t = s->ptype[p];
-        if (t != ORC_PATCH_MAXWELL_WALL && t != ORC_PATCH_MIXED) {
+        if (t < ORC_PATCH_MAXWELL_WALL || t > ORC_PATCH_SYMMETRY_PLANE) {
             set_err("orc_create: unsupported patch type");
             orc_destroy(s);
             return NULL;
@@ -230,9 +241,29 @@ orc_state *orc_create(const orc_case *cs)
             orc_destroy(s);
             return NULL;
         }
+        int axis = -1;
+        if ((t == ORC_PATCH_SYMMETRY || t == ORC_PATCH_SYMMETRY_PLANE) && s->psize[p] > 0) {
+            /* the mirror velocity is taken about the normal of the patch's first face, which
+             * must be a coordinate axis (discreteVelocity.C:731-737, 760-764) */
+            const double *S0 = s->Sf + 3 * (nIF + s->pstart[p]);
+            const double m = sqrt(magsqr3(S0));
+            for (int k = 0; k < 3; ++k)
+                if (fabs(fabs(S0[k]) / m - 1.0) < 1e-12) axis = k;
+            if (axis < 0) {
+                set_err("orc_create: a symmetry patch must be normal to a coordinate axis");
+                orc_destroy(s);
+                return NULL;
+            }
+            any_sym = 1;
+        }
         for (int i = 0; i < s->psize[p]; ++i) {
-            s->bf_type[s->pstart[p] + i] = t;
-            s->bf_local[s->pstart[p] + i] = i;
+            const int b = s->pstart[p] + i;
+            s->bf_type[b] = t;
+            s->bf_local[b] = i;
+            s->bf_p[b] = cs->patch_pressure ? cs->patch_pressure[p] : 0.0;
+            s->bf_mirror[b] = axis;
+            if (axis >= 0)
+                for (int k = 0; k < 3; ++k) s->bf_sf0[3 * b + k] = s->Sf[3 * (nIF + s->pstart[p]) + k];
         }
     }
     for (int b = 0; b < nB; ++b)
@@ -269,6 +300,23 @@ orc_state *orc_create(const orc_case *cs)
         }
     }
 
+    /* mirror velocities of a symmetric 1-D set (x[n-1-i] = -x[i], equal weights) */
+    s->mirror = calloc(3 * (size_t)Nv, sizeof(int));
+    if (any_sym) {
+        for (int k = 0; k < n; ++k)
+            if (fabs(s->xis[n - 1 - k] + s->xis[k]) > 1e-12 * fabs(s->xis[0]) || s->wts[n - 1 - k] != s->wts[k]) {
+                set_err("orc_create: symmetry patches need a velocity set symmetric about 0");
+                orc_destroy(s);
+                return NULL;
+            }
+    }
+    for (int v = 0; v < Nv; ++v) {
+        const int ix = v % n, iy = (v / n) % n, iz = v / (n * n);
+        s->mirror[v] = (n - 1 - ix) + n * (iy + n * iz);
+        s->mirror[Nv + v] = D >= 2 ? ix + n * ((n - 1 - iy) + n * iz) : v;
+        s->mirror[2 * Nv + v] = D == 3 ? ix + n * (iy + n * (n - 1 - iz)) : v;
+    }
+
     size_t nvc = (size_t)Nv * nC, nvf = (size_t)Nv * nF;
     s->gT = calloc(nvc, sizeof(double)); s->hT = calloc(nvc, sizeof(double));
     s->gB = calloc(nvc, sizeof(double)); s->hB = calloc(nvc, sizeof(double));
@@ -291,6 +339,10 @@ orc_state *orc_create(const orc_case *cs)
         return NULL;
     }
     compute_ls_vectors(s);
+    if (cs->ls_own && cs->ls_nei) {  /* another gradient scheme in the same form (Gauss linear) */
+        memcpy(s->own_ls, cs->ls_own, sizeof(double) * 3 * (size_t)nIF);
+        memcpy(s->nei_ls, cs->ls_nei, sizeof(double) * 3 * (size_t)nIF);
+    }
     return s;
 }
 
@@ -300,6 +352,7 @@ void orc_destroy(orc_state *s)
     void *p[] = {s->xi, s->w, s->gT, s->hT, s->gB, s->hB, s->gS, s->hS, s->rho, s->U, s->T, s->q,
                  s->tau, s->rhoS, s->US, s->TS, s->qS, s->tauS, s->rho_b, s->in_by_rho,
                  s->out_going, s->own_ls, s->nei_ls, s->patch_ls, s->bf_type, s->bf_local,
+                 s->bf_p, s->bf_mirror, s->bf_sf0, s->mirror,
                  s->owner, s->neighbour, s->ptype, s->pstart, s->psize, s->Sf, s->Cf, s->C, s->V,
                  s->wts_if, s->dc, s->bf_rho, s->bf_U, s->bf_T, s->xis, s->wts};
     for (size_t i = 0; i < sizeof p / sizeof p[0]; ++i) free(p[i]);
@@ -318,6 +371,27 @@ static double maxwell_by_rho(const orc_state *s, const double *xi, const double 
     double R = s->cs.R;
     double c[3] = {U[0] - xi[0], U[1] - xi[1], U[2] - xi[2]};
     return 1.0 / pow(sqrt(2.0 * ORC_PI * R * T), (double)s->cs.D) * exp(-magsqr3(c) / (2.0 * R * T));
+}
+
+/* updatePressureInOutBC (fvDVM.C:1745-1821): patch rho (and T at outlets) from the cell
+ * state by the characteristic relations; the patch U the reference also sets there is not
+ * read by the DF boundary (farField incoming takes the owner cell's U, fvDVM.C:927-933) */
+static void update_pressure_bc(orc_state *s)
+{
+    const orc_case *cs = &s->cs;
+    const double R = cs->R;
+    for (int b = 0; b < cs->nB; ++b) {
+        const int t = s->bf_type[b];
+        if (t == ORC_PATCH_PRESSURE_IN) {
+            s->rho_b[b] = s->bf_p[b] / R / s->bf_T[b];
+        } else if (t == ORC_PATCH_PRESSURE_OUT) {
+            const int own = s->owner[cs->nIF + b];
+            const double Ti = s->T[own], rhoi = s->rho[own];
+            const double ai = sqrt(R * Ti * (cs->K + 5) / (cs->K + 3));
+            s->rho_b[b] = rhoi + (s->bf_p[b] - rhoi * R * Ti) / ai / ai;
+            s->bf_T[b] = s->bf_p[b] / (R * rhoi);
+        }
+    }
 }
 
 /* ------------------------------------------------------------------------- init */
@@ -373,6 +447,7 @@ int orc_init(orc_state *s, const double *rho, const double *U, const double *T)
             }
         }
     for (int c = 0; c < nC; ++c) s->tau[c] = tau_of(cs, s->T[c], s->rho[c]);
+    update_pressure_bc(s);  /* "set initial rho in pressureIn/Out BC" (F:2112-2113) */
     /* Usurf = linear interpolate(U) for the first CFL (F:2116) */
     for (int f = 0; f < nIF; ++f) {
         double lam = s->wts_if[f];
@@ -471,13 +546,29 @@ static void update_gh_bar_surf(orc_state *s, double dt)
                     hS[f] = 0.5 * (recon(hB, gh, nei, Cf, s->C, xi, dt) + recon(hB, gh, own, Cf, s->C, xi, dt));
                 }
             }
-            /* boundary faces: maxwellWall and mixed reconstruct outgoing DFs (F:892-963) */
+            /* boundary faces (F:883-1027) */
             for (int b = 0; b < nB; ++b) {
                 int f = nIF + b, own = s->owner[f];
+                const int t = s->bf_type[b];
                 double xs = dot3(xi, s->Sf + 3 * f);
-                if (xs > 0) {
+                if (t == ORC_PATCH_ZERO_GRADIENT) {
+                    /* every entry takes the owner's f-bar-plus, no reconstruction (F:887-891) */
+                    gS[f] = gB[own];
+                    hS[f] = hB[own];
+                } else if (t == ORC_PATCH_SYMMETRY || t == ORC_PATCH_SYMMETRY_PLANE) {
+                    /* outgoing and tangent entries reconstructed; incoming ones copied below (F:1009-1025) */
+                    if (xs > -ORC_VSMALL) {
+                        gS[f] = recon(gB, gg, own, s->Cf + 3 * f, s->C, xi, dt);
+                        hS[f] = recon(hB, gh, own, s->Cf + 3 * f, s->C, xi, dt);
+                    }
+                } else if (xs > 0) {  /* maxwellWall, mixed, farField: outgoing reconstructed */
                     gS[f] = recon(gB, gg, own, s->Cf + 3 * f, s->C, xi, dt);
                     hS[f] = recon(hB, gh, own, s->Cf + 3 * f, s->C, xi, dt);
+                } else if (t == ORC_PATCH_FAR_FIELD || t == ORC_PATCH_PRESSURE_IN || t == ORC_PATCH_PRESSURE_OUT) {
+                    /* incoming: patch rho times the Maxwellian of the owner cell's U and the patch T (F:923-937) */
+                    const double Tp = s->bf_T[b];
+                    gS[f] = s->rho_b[b] * maxwell_by_rho(s, xi, s->U + 3 * own, Tp);
+                    hS[f] = gS[f] * (cs->R * Tp) * (cs->K + 3 - cs->D);
                 }
             }
         }
@@ -521,6 +612,25 @@ static void update_gh_bar_surf_maxwell_wall_in(orc_state *s)
                 s->gS[k] = s->rho_b[b] * maxwell_by_rho(s, xi, s->bf_U + 3 * b, s->bf_T[b]);
                 s->hS[k] = s->gS[k] * (cs->R * s->bf_T[b]) * (cs->K + 3 - D);
             }
+        }
+    }
+}
+
+/* updateGHbarSurfSymmetryIn (F:1100-1179, discreteVelocity.C:691-775): on symmetry
+ * faces every incoming entry (xi . Sf0 <= 0, Sf0 the patch's first face) takes the face
+ * value of the velocity mirrored about the patch axis */
+static void update_gh_bar_surf_symmetry_in(orc_state *s)
+{
+    const int nIF = s->cs.nIF, nB = s->cs.nB, nF = s->nF, Nv = s->Nv;
+#pragma omp parallel for schedule(dynamic)
+    for (int v = 0; v < Nv; ++v) {
+        const double *xi = s->xi + 3 * v;
+        for (int b = 0; b < nB; ++b) {
+            const int axis = s->bf_mirror[b];
+            if (axis < 0 || dot3(xi, s->bf_sf0 + 3 * b) > 0) continue;
+            const int f = nIF + b, vm = s->mirror[(size_t)axis * Nv + v];
+            s->gS[(size_t)v * nF + f] = s->gS[(size_t)vm * nF + f];
+            s->hS[(size_t)v * nF + f] = s->hS[(size_t)vm * nF + f];
         }
     }
 }
@@ -587,6 +697,24 @@ static void update_gh_surf(orc_state *s, double dt)
             double gEq = (1.0 + (1.0 - Pr) * cqBy5pRT * (cSqrByRT - D - 2.0)) * gEqBGK;
             /* Q1: on patches hEq takes T from the internal-face Tsurf at the patch-local index */
             double Th = (bnd && cs->q1_compat) ? s->TS[s->bf_local[f - nIF]] : T;
+            double hEq = ((K + 3.0 - D) + (1.0 - Pr) * cqBy5pRT * ((cSqrByRT - D) * (K + 3.0 - D) - 2 * K)) * gEqBGK * R * Th;
+            gS[f] = (1.0 - relaxFactor) * gS[f] + relaxFactor * gEq;
+            hS[f] = (1.0 - relaxFactor) * hS[f] + relaxFactor * hEq;
+        }
+        /* DVMsymmetry patches: the whole face is relaxed once more, outgoing entries
+         * included (F:1449-1456) */
+        for (int b = 0; b < cs->nB; ++b) {
+            if (s->bf_type[b] != ORC_PATCH_SYMMETRY) continue;
+            const int f = nIF + b;
+            const double rho = s->rhoS[f], T = s->TS[f], *U = s->US + 3 * f, *q = s->qS + 3 * f;
+            double relaxFactor = 0.5 * dt / (2.0 * s->tauS[f] + 0.5 * dt);
+            double um[3] = {U[0] - xi[0], U[1] - xi[1], U[2] - xi[2]};
+            double cSqrByRT = magsqr3(um) / (R * T);
+            double xu[3] = {xi[0] - U[0], xi[1] - U[1], xi[2] - U[2]};
+            double cqBy5pRT = dot3(xu, q) / (5.0 * rho * R * T * R * T);
+            double gEqBGK = rho / pow(sqrt(2.0 * ORC_PI * R * T), (double)D) * exp(-cSqrByRT / 2.0);
+            double gEq = (1.0 + (1.0 - Pr) * cqBy5pRT * (cSqrByRT - D - 2.0)) * gEqBGK;
+            double Th = cs->q1_compat ? s->TS[s->bf_local[b]] : T;
             double hEq = ((K + 3.0 - D) + (1.0 - Pr) * cqBy5pRT * ((cSqrByRT - D) * (K + 3.0 - D) - 2 * K)) * gEqBGK * R * Th;
             gS[f] = (1.0 - relaxFactor) * gS[f] + relaxFactor * gEq;
             hS[f] = (1.0 - relaxFactor) * hS[f] + relaxFactor * hEq;
@@ -666,8 +794,7 @@ static void update_macro_vol(orc_state *s, double dt)
 }
 
 /* fvDVM::evolution (F:2128-2174).  GHbarPvolComm only exchanges processor halos and
- * evaluates fixedGradient patches whose values equal the cell values; symmetry and
- * pressure in/out patches are outside the supported set. */
+ * evaluates fixedGradient patches whose values equal the cell values. */
 int orc_evolution_stages(orc_state *s, double dt, int first, int last)
 {
     if (!(dt > 0)) {
@@ -683,11 +810,17 @@ int orc_evolution_stages(orc_state *s, double dt, int first, int last)
         case 0: update_gh_bar_pvol(s, dt); break;
         case 1: update_gh_bar_surf(s, dt); break;
         case 2: update_maxwell_wall_rho(s); break;
-        case 3: update_gh_bar_surf_maxwell_wall_in(s); break;
+        case 3:
+            update_gh_bar_surf_maxwell_wall_in(s);
+            update_gh_bar_surf_symmetry_in(s);
+            break;
         case 4: update_macro_surf(s, dt); break;
         case 5: update_gh_surf(s, dt); break;
         case 6: update_gh_tilde_vol(s, dt); break;
-        case 7: update_macro_vol(s, dt); break;
+        case 7:
+            update_macro_vol(s, dt);
+            update_pressure_bc(s);  /* F:2172 */
+            break;
         }
     }
     return 0;

@@ -20,8 +20,15 @@ extern "C" {
 
 /* rho boundary types (0/rho) that the path supports; the DF face type follows the map
  * of src/fvDVM/discreteVelocity/discreteVelocity.C:188-195. */
-enum { ORC_PATCH_MAXWELL_WALL = 1, /* rho calculatedMaxwell -> DF maxwellWall */
-       ORC_PATCH_MIXED = 2 };      /* rho fixedValue       -> DF mixed       */
+enum { ORC_PATCH_MAXWELL_WALL = 1,   /* rho calculatedMaxwell -> DF maxwellWall  */
+       ORC_PATCH_MIXED = 2,          /* rho fixedValue        -> DF mixed        */
+       ORC_PATCH_ZERO_GRADIENT = 3,  /* rho zeroGradient      -> DF zeroGradient */
+       ORC_PATCH_FAR_FIELD = 4,      /* rho farField          -> DF farField     */
+       ORC_PATCH_PRESSURE_IN = 5,    /* rho pressureIn        -> DF farField     */
+       ORC_PATCH_PRESSURE_OUT = 6,   /* rho pressureOut       -> DF farField     */
+       ORC_PATCH_SYMMETRY = 7,       /* rho symmetryMod       -> DF DVMsymmetry  */
+       ORC_PATCH_SYMMETRY_PLANE = 8  /* geometric symmetryPlane -> DF symmetryPlane */
+};
 
 typedef struct orc_case {
     int nC, nIF, nB, D;
@@ -40,6 +47,8 @@ typedef struct orc_case {
     double R, omega, Tref, muRef, Pr;
     int K;                   /* KInner */
     int q1_compat;           /* replicate the Tsurf internal-field read of fvDVM.C:1413 */
+    const double *patch_pressure; /* [n_patches] pressureIn / pressureOut values (may be NULL) */
+    const double *ls_own, *ls_nei; /* [nIF][3] gradient vectors (NULL: least squares) */
 } orc_case;
 
 typedef struct orc_state orc_state;

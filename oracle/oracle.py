@@ -28,7 +28,8 @@ class OrcCase(ct.Structure):
         ("bf_rho", _D), ("bf_U", _D), ("bf_T", _D),
         ("n1d", ct.c_int), ("xis", _D), ("wts", _D), ("xi_max", ct.c_double),
         ("R", ct.c_double), ("omega", ct.c_double), ("Tref", ct.c_double), ("muRef", ct.c_double),
-        ("Pr", ct.c_double), ("K", ct.c_int), ("q1_compat", ct.c_int),
+        ("Pr", ct.c_double), ("K", ct.c_int), ("q1_compat", ct.c_int), ("patch_pressure", _D),
+        ("ls_own", _D), ("ls_nei", _D),
     ]
 
 
@@ -87,7 +88,9 @@ class Oracle:
             bf_rho=_p(a["bf_rho"]), bf_U=_p(a["bf_U"]), bf_T=_p(a["bf_T"]),
             n1d=len(a["xis"]), xis=_p(a["xis"]), wts=_p(a["wts"]), xi_max=a["xi_max"],
             R=a["R"], omega=a["omega"], Tref=a["Tref"], muRef=a["muRef"], Pr=a["Pr"],
-            K=a["K"], q1_compat=a["q1_compat"])
+            K=a["K"], q1_compat=a["q1_compat"], patch_pressure=_p(a["patch_pressure"]),
+            ls_own=_p(a["ls_own"]) if a.get("ls_own") is not None else None,
+            ls_nei=_p(a["ls_nei"]) if a.get("ls_nei") is not None else None)
         self._h = L.orc_create(ct.byref(cs))
         if not self._h:
             raise RuntimeError(L.orc_last_error().decode())

@@ -69,6 +69,8 @@ struct dugks_handle {
     int sched = 0;
     FusedPlan plan;                // face-value schedule data
     rc::FusedPlan plan_rc;         // recompute schedule data
+    bool has_far_field = false, has_symmetry = false, has_pressure = false;  // boundary types present
+    std::vector<double> bf_T0;     // patch temperatures as given (pressure outlets change theirs)
     std::vector<int> perm;         // internal cell -> caller's cell label
     double geo_sum = 0.0;          // order-sensitive checksum of the caller's cell centres / volumes
     std::vector<double> ref_g, ref_h;  // host copies of the storage reference per velocity
@@ -165,12 +167,15 @@ int staged_step(dugks_handle *h)
     S *gT = (S *)d.gT, *hT = (S *)d.hT, *gB = (S *)d.gB, *hB = (S *)d.hB, *gS = (S *)d.gS, *hS = (S *)d.hS;
     PHASE(kPhBar, Staged<S>::bar(g, d.cell, gT, hT, gB, hB, d.dt, st));
     PHASE(kPhFaceRecon, Staged<S>::face_recon(g, gB, hB, gS, hS, d.dt, st));
+    if (h->has_far_field) PHASE(kPhFaceRecon, Staged<S>::far_field_in(g, d.cell, d.rho_b, gS, hS, st));
     PHASE(kPhWallRho, Staged<S>::wall_rho(g, gS, d.rho_b, st));
     PHASE(kPhWallIn, Staged<S>::wall_in(g, d.rho_b, gS, hS, st));
+    if (h->has_symmetry) PHASE(kPhWallIn, Staged<S>::symmetry_in(g, gS, hS, st));
     PHASE(kPhFaceMom, Staged<S>::face_moments(g, gS, hS, d.face, d.dt, d.cfl_part, st));
     PHASE(kPhFaceRelax, Staged<S>::face_relax(g, d.face, gS, hS, d.dt, st));
     PHASE(kPhUpdate, Staged<S>::update(g, gB, hB, gS, hS, gT, hT, d.dt, st));
     PHASE(kPhCellMom, Staged<S>::cell_moments(g, gT, hT, d.cell, d.dt, st));
+    if (h->has_pressure) launch_pressure_bc(g, d.cell, d.rho_b, d.bf_T, st);  // fvDVM.C:2172
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -267,18 +272,42 @@ int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patch
     const int Nv = (int)nv_ll;
 
     // boundary-face patch map
-    std::vector<int> bf_type(nB, -1), bf_local(nB, 0);
-    std::vector<double> bf_rho(nB), bf_T(nB);
-    std::vector<d3> bf_U(nB);
+    std::vector<int> bf_type(nB, -1), bf_local(nB, 0), bf_axis(nB, -1);
+    std::vector<double> bf_rho(nB), bf_T(nB), bf_p(nB, 0.0);
+    std::vector<d3> bf_U(nB), bf_sf0(nB, d3{0, 0, 0});
+    bool any_sym = false, all_fusable = true;
     for (int p = 0; p < n_patches; ++p) {
         const dugks_patch &pp = patches[p];
-        if (pp.type != DUGKS_PATCH_MAXWELL_WALL && pp.type != DUGKS_PATCH_MIXED)
-            return fail("dugks_create: patch " + std::to_string(p) + " has unsupported type " + std::to_string(pp.type) +
-                        " (supported: calculatedMaxwell -> maxwellWall, fixedValue -> mixed)");
+        if (pp.type < DUGKS_PATCH_MAXWELL_WALL || pp.type > DUGKS_PATCH_SYMMETRY_PLANE)
+            return fail("dugks_create: patch " + std::to_string(p) + " has unknown type " + std::to_string(pp.type));
         if (pp.size < 0 || pp.start < 0 || pp.start + pp.size > nB)
             return fail("dugks_create: patch " + std::to_string(p) + " range outside the boundary faces");
         if (pp.size > 0 && (!pp.rho || !pp.U || !pp.T))
             return fail("dugks_create: patch " + std::to_string(p) + " lacks rho/U/T values");
+        if ((pp.type == DUGKS_PATCH_PRESSURE_IN || pp.type == DUGKS_PATCH_PRESSURE_OUT) && !(pp.pressure > 0))
+            return fail("dugks_create: pressure patch " + std::to_string(p) + " needs a positive pressure");
+        all_fusable = all_fusable && fused_patch_type(pp.type);
+        int axis = -1;
+        if (is_symmetry(pp.type) && pp.size > 0) {
+            // the mirror velocity is taken about the patch's first face normal, which must be a
+            // coordinate axis (discreteVelocity.C:731-737, 760-764)
+            const double *S0 = mesh->Sf + 3 * (size_t)(nIF + pp.start);
+            const double m = std::sqrt(S0[0] * S0[0] + S0[1] * S0[1] + S0[2] * S0[2]);
+            for (int k = 0; k < 3; ++k)
+                if (std::fabs(std::fabs(S0[k]) / m - 1.0) < 1e-12) axis = k;
+            if (axis < 0)
+                return fail("dugks_create: symmetry patch " + std::to_string(p) + " is not normal to a coordinate axis");
+            any_sym = true;
+        }
+        for (int i = 0; i < pp.size; ++i) {
+            const int b = pp.start + i;
+            bf_axis[b] = axis;
+            bf_p[b] = pp.pressure;
+            if (axis >= 0) {
+                const double *S0 = mesh->Sf + 3 * (size_t)(nIF + pp.start);
+                bf_sf0[b] = d3{S0[0], S0[1], S0[2]};
+            }
+        }
         if (o.q1_compat && pp.size > nIF)
             return fail("dugks_create: q1_compat reads the internal-face temperature at patch-local indices; patch " +
                         std::to_string(p) + " is larger than the internal-face count");
@@ -295,6 +324,18 @@ int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patch
     }
     for (int b = 0; b < nB; ++b)
         if (bf_type[b] == -1) return fail("dugks_create: boundary face " + std::to_string(b) + " is in no patch");
+    if (any_sym)
+        for (int k = 0; k < n1; ++k)
+            if (std::fabs(vset->xis[n1 - 1 - k] + vset->xis[k]) > 1e-12 * std::fabs(vset->xis[0]) ||
+                vset->weights[n1 - 1 - k] != vset->weights[k])
+                return fail("dugks_create: symmetry patches need a velocity set symmetric about 0");
+    // zeroGradient, farField, pressure and symmetry faces run on the staged path
+    if (!all_fusable) {
+        if (o.fused >= 1)
+            return fail("dugks_create: the fused path carries calculatedMaxwell and fixedValue patches only; "
+                        "use fused = 0 or -1 for this case");
+        o.fused = 0;
+    }
 
     for (int f = 0; f < nF; ++f)
         if (mesh->owner[f] < 0 || mesh->owner[f] >= nC) return fail("dugks_create: owner label out of range");
@@ -407,8 +448,28 @@ int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patch
         dalloc(h, &d.bf_type, nB) || dalloc(h, &d.bf_local, nB) || dalloc(h, &d.bf_rho, nB) ||
         dalloc(h, &d.bf_T, nB) || dalloc(h, &d.bf_U, nB) || dalloc(h, &d.in_by_rho, nB) || dalloc(h, &d.dc, nIF) ||
         dalloc(h, &d.weights, nIF) || dalloc(h, &d.xi, Nv) || dalloc(h, &d.w, Nv) || dalloc(h, &d.rho_b, nB) ||
-        dalloc(h, &d.cfl_part, 2 * (size_t)cfl_blocks(nIF)) || dalloc(h, &d.dt, 4))
+        dalloc(h, &d.cfl_part, 2 * (size_t)cfl_blocks(nIF)) || dalloc(h, &d.dt, 4) || dalloc(h, &d.bf_p, nB) ||
+        dalloc(h, &d.bf_axis, nB) || dalloc(h, &d.bf_sf0, nB) || dalloc(h, &d.mirror, 3 * (size_t)Nv))
         return bail(1);
+    {
+        // velocity mirrored about each axis (ix fastest: v = ix + n (iy + n iz))
+        std::vector<int> mir(3 * (size_t)Nv);
+        for (int v = 0; v < Nv; ++v) {
+            const int ix = v % n1, iy = (v / n1) % n1, iz = v / (n1 * n1);
+            mir[v] = (n1 - 1 - ix) + n1 * (iy + n1 * iz);
+            mir[Nv + v] = D >= 2 ? ix + n1 * ((n1 - 1 - iy) + n1 * iz) : v;
+            mir[2 * (size_t)Nv + v] = D == 3 ? ix + n1 * (iy + n1 * (n1 - 1 - iz)) : v;
+        }
+        if (upload(d.mirror, mir.data(), mir.size(), st) || upload(d.bf_p, bf_p.data(), nB, st) ||
+            upload(d.bf_axis, bf_axis.data(), nB, st) || upload(d.bf_sf0, bf_sf0.data(), nB, st))
+            return bail(1);
+    }
+    for (int b = 0; b < nB; ++b) {
+        h->has_far_field = h->has_far_field || is_far_field(bf_type[b]);
+        h->has_symmetry = h->has_symmetry || is_symmetry(bf_type[b]);
+        h->has_pressure = h->has_pressure || bf_type[b] == kPressureIn || bf_type[b] == kPressureOut;
+    }
+    h->bf_T0 = bf_T;
     if (dalloc(h, &d.cell.rho, nC) || dalloc(h, &d.cell.U, nC) || dalloc(h, &d.cell.T, nC) ||
         dalloc(h, &d.cell.q, nC) || dalloc(h, &d.cell.tau, nC) || dalloc(h, &d.face.rho, nF) ||
         dalloc(h, &d.face.U, nF) || dalloc(h, &d.face.T, nF) || dalloc(h, &d.face.q, nF) ||
@@ -456,6 +517,7 @@ int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patch
     g.bf_type = d.bf_type; g.bf_local = d.bf_local; g.bf_rho = d.bf_rho; g.bf_T = d.bf_T; g.bf_U = d.bf_U;
     g.in_by_rho = d.in_by_rho; g.delta_coeffs = d.dc; g.weights = d.weights; g.xi = d.xi; g.w = d.w;
     g.ref_g = d.ref_g; g.ref_h = d.ref_h;
+    g.bf_p = d.bf_p; g.bf_axis = d.bf_axis; g.bf_sf0 = d.bf_sf0; g.mirror = d.mirror;
 
     // distributions
     const size_t ncell = (size_t)Nv * nC, nface = (size_t)Nv * nF;
@@ -533,6 +595,9 @@ int dugks_init_from_macros(dugks_handle *h, const double *rho, const double *U, 
         if (!ok) return fail("dugks_init_from_macros: " + err);
     }
     HIPCHK(hipMemcpyAsync(d.rho_b, g.bf_rho, sizeof(double) * g.nB, hipMemcpyDeviceToDevice, st));
+    if (g.nB > 0 && upload(d.bf_T, h->bf_T0.data(), g.nB, st)) return 1;
+    // "set initial rho in pressureIn/Out BC" (fvDVM.C:2112-2113)
+    if (h->has_pressure) launch_pressure_bc(g, d.cell, d.rho_b, d.bf_T, st);
     launch_init_usurf(g, d.cell, d.face, st);
     launch_cfl_partials(g, d.face, d.cfl_part, st);
     HIPCHK(hipGetLastError());
@@ -903,6 +968,16 @@ int dugks_adjoint(dugks_handle *h, int n_steps, const double *dts, int checkpoin
     if (check_handle(h)) return 1;
     if (n_steps < 1 || !dts) return fail("dugks_adjoint: need n_steps >= 1 and dts");
     if (checkpoint_every < 1) return fail("dugks_adjoint: checkpoint_every must be >= 1");
+    if (h->has_far_field || h->has_symmetry)
+        return fail("dugks_adjoint: the reverse kernels carry diffuse-wall and fixedValue patches only");
+    {
+        std::vector<int> bt(h->g.nB);
+        if (h->g.nB > 0)
+            HIPCHK(hipMemcpy(bt.data(), h->g.bf_type, sizeof(int) * h->g.nB, hipMemcpyDeviceToHost));
+        for (int t : bt)
+            if (!fused_patch_type(t))
+                return fail("dugks_adjoint: the reverse kernels carry diffuse-wall and fixedValue patches only");
+    }
     for (int n = 0; n < n_steps; ++n)
         if (!(dts[n] > 0)) return fail("dugks_adjoint: non-positive time step " + std::to_string(n));
     const Geo &g = h->g;
@@ -948,6 +1023,7 @@ int dugks_checkpoint_load(dugks_handle *h, const char *path)
 {
     if (ckpt_io(h, path, false)) return 1;
     refresh_cell_coef(h);
+    if (h->has_pressure) launch_pressure_bc(h->g, h->d.cell, h->d.rho_b, h->d.bf_T, h->stream);
     HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
 }

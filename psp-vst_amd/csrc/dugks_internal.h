@@ -34,7 +34,22 @@ __device__ inline double sigma_dot(const d3 &xi, const d3 &Sf)
 }
 __host__ __device__ inline double magsqr(const d3 &a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
 
-enum PatchType : int { kMaxwellWall = 1, kMixed = 2 };
+// DF boundary types selected by the rho patch type (discreteVelocity.C:188-195); the
+// numbers are the dugks_patch_type codes of include/dugks.h
+enum PatchType : int {
+    kMaxwellWall = 1,   // calculatedMaxwell: diffuse wall
+    kMixed = 2,         // fixedValue: incoming half frozen at the patch Maxwellian
+    kZeroGradient = 3,  // every entry takes the owner cell's f-bar-plus
+    kFarField = 4,      // incoming entries: patch rho x Maxwellian(owner cell U, patch T)
+    kPressureIn = 5,    // farField with rho = p / (R T) set after every step
+    kPressureOut = 6,   // farField with rho, T from the outlet characteristic
+    kSymmetry = 7,      // DVMsymmetry: incoming entries copy the mirror velocity, whole face relaxed again
+    kSymmetryPlane = 8  // symmetryPlane: incoming entries copy the mirror velocity
+};
+__host__ __device__ inline bool is_far_field(int t) { return t == kFarField || t == kPressureIn || t == kPressureOut; }
+__host__ __device__ inline bool is_symmetry(int t) { return t == kSymmetry || t == kSymmetryPlane; }
+// boundary types the fused schedules carry (the others run on the staged path)
+inline bool fused_patch_type(int t) { return t == kMaxwellWall || t == kMixed; }
 
 // Read-only geometry and constants, passed by value to every kernel.
 struct Geo {
@@ -55,6 +70,10 @@ struct Geo {
     const double *bf_rho, *bf_T;
     const d3 *bf_U;
     const double *in_by_rho; // [nB] inComingByRho (walls)
+    const double *bf_p;      // [nB] patch pressure (pressureIn / pressureOut)
+    const int *bf_axis;      // [nB] symmetry faces: the axis of the patch normal, else -1
+    const d3 *bf_sf0;        // [nB] symmetry faces: Sf of the patch's first face
+    const int *mirror;       // [3][Nv] velocity mirrored about each axis
     // internal-face CFL data
     const double *delta_coeffs; // [nIF]
     const double *weights;      // [nIF] linear interpolation weights
@@ -82,7 +101,9 @@ struct DevState {
     int *cg_off = nullptr, *cg_nb = nullptr, *bf_type = nullptr, *bf_local = nullptr;
     d3 *Sf = nullptr, *Cf = nullptr, *C = nullptr, *cg_ls = nullptr, *bf_U = nullptr, *xi = nullptr;
     double *V = nullptr, *bf_rho = nullptr, *bf_T = nullptr, *in_by_rho = nullptr, *dc = nullptr, *w = nullptr,
-           *weights = nullptr, *ref_g = nullptr, *ref_h = nullptr;
+           *weights = nullptr, *ref_g = nullptr, *ref_h = nullptr, *bf_p = nullptr;
+    int *bf_axis = nullptr, *mirror = nullptr;
+    d3 *bf_sf0 = nullptr;
     // distributions (storage type chosen at run time; void* + element size)
     void *gT = nullptr, *hT = nullptr, *gB = nullptr, *hB = nullptr, *gS = nullptr, *hS = nullptr;
     // macros
@@ -116,6 +137,8 @@ void launch_init_df(const Geo &g, const Macros &cm, S *gT, S *hT, hipStream_t st
 template <class S>
 void launch_init_mixed_faces(const Geo &g, S *gS, S *hS, hipStream_t st);
 void launch_init_incoming(const Geo &g, double *in_by_rho, hipStream_t st);
+// updatePressureInOutBC (fvDVM.C:1745-1821): patch rho and (outlets) T from the cell state
+void launch_pressure_bc(const Geo &g, const Macros &cm, double *rho_b, double *bf_T, hipStream_t st);
 void launch_init_usurf(const Geo &g, const Macros &cm, Macros fm, hipStream_t st);
 void launch_tau(const Geo &g, Macros m, int n, hipStream_t st);
 void launch_cfl_partials(const Geo &g, const Macros &fm, double *part, hipStream_t st);
@@ -129,6 +152,10 @@ struct Staged {
     static void face_recon(const Geo &g, const S *gB, const S *hB, S *gS, S *hS, const double *dt, hipStream_t st);
     static void wall_rho(const Geo &g, const S *gS, double *rho_b, hipStream_t st);
     static void wall_in(const Geo &g, const double *rho_b, S *gS, S *hS, hipStream_t st);
+    // incoming entries of farField faces (fvDVM.C:923-937) and of symmetry faces
+    // (discreteVelocity.C:691-775)
+    static void far_field_in(const Geo &g, const Macros &cm, const double *rho_b, S *gS, S *hS, hipStream_t st);
+    static void symmetry_in(const Geo &g, S *gS, S *hS, hipStream_t st);
     static void face_moments(const Geo &g, const S *gS, const S *hS, Macros fm, const double *dt, double *cfl_part,
                              hipStream_t st);
     static void face_relax(const Geo &g, const Macros &fm, S *gS, S *hS, const double *dt, hipStream_t st);

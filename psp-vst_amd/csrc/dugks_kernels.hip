@@ -199,12 +199,56 @@ __global__ void __launch_bounds__(kBlock)
         }
         gS[k] = (S)gf;
         hS[k] = (S)hf;
-    } else if (xs > 0) {  // maxwellWall and mixed: outgoing reconstructed from the owner
-        const int own = g.owner[f];
-        const d3 r = recon_arm(Cf, g.C[own], xi, dt);
-        gS[k] = (S)recon(g, gv, own, r);
-        hS[k] = (S)recon(g, hv, own, r);
+    } else {
+        const int b = f - g.nIF, t = g.bf_type[b], own = g.owner[f];
+        if (t == kZeroGradient) {  // every entry takes the owner's f-bar-plus (fvDVM.C:887-891)
+            const size_t kc = (size_t)v * g.nC + own;
+            gS[k] = gB[kc];
+            hS[k] = hB[kc];
+        } else if (is_symmetry(t) ? xs > -kVSmall : xs > 0) {
+            // outgoing entries (symmetry faces: tangent ones too, fvDVM.C:1013) reconstructed
+            // from the owner; incoming ones are set by the wall / far-field / symmetry kernels
+            const d3 r = recon_arm(Cf, g.C[own], xi, dt);
+            gS[k] = (S)recon(g, gv, own, r);
+            hS[k] = (S)recon(g, hv, own, r);
+        }
     }
+}
+
+// incoming entries of farField / pressureIn / pressureOut faces: patch rho times the
+// Maxwellian of the owner cell's U and the patch T (fvDVM.C:923-937)
+template <class S>
+__global__ void __launch_bounds__(kBlock) k_far_field_in(Geo g, Macros cm, const double *rho_b, S *gS, S *hS)
+{
+    const int b = blockIdx.x * kBlock + threadIdx.x;
+    const int v = blockIdx.y;
+    if (b >= g.nB || !is_far_field(g.bf_type[b])) return;
+    const int f = g.nIF + b;
+    const d3 xi = g.xi[v];
+    if (sigma_dot(xi, g.Sf[f]) > 0) return;
+    const double Tp = g.bf_T[b];
+    const double gp = rho_b[b] * maxwell_by_rho(g, xi, cm.U[g.owner[f]], Tp);
+    const size_t k = (size_t)v * g.nF + f;
+    gS[k] = (S)(gp - g.ref_g[v]);
+    hS[k] = (S)(gp * (g.R * Tp) * (g.K + 3 - g.D) - g.ref_h[v]);
+}
+
+// updateGHbarSurfSymmetryIn (fvDVM.C:1100-1179, discreteVelocity.C:691-775): an incoming
+// entry (xi . Sf0 <= 0, Sf0 the patch's first face) takes the value of the velocity mirrored
+// about the patch axis.  The mirror of an incoming velocity is outgoing (or itself when
+// tangent), so no entry read here is written here.
+template <class S>
+__global__ void __launch_bounds__(kBlock) k_symmetry_in(Geo g, S *gS, S *hS)
+{
+    const int b = blockIdx.x * kBlock + threadIdx.x;
+    const int v = blockIdx.y;
+    if (b >= g.nB) return;
+    const int axis = g.bf_axis[b];
+    if (axis < 0 || sigma_dot(g.xi[v], g.bf_sf0[b]) > 0) return;
+    const int f = g.nIF + b, vm = g.mirror[(size_t)axis * g.Nv + v];
+    // deviations: the reference value is the same for v and its mirror
+    gS[(size_t)v * g.nF + f] = gS[(size_t)vm * g.nF + f];
+    hS[(size_t)v * g.nF + f] = hS[(size_t)vm * g.nF + f];
 }
 
 // outGoing accumulation (fvDVM.C:959-960, velocity order) and the wall density
@@ -295,7 +339,11 @@ __global__ void __launch_bounds__(kBlock) k_face_relax(Geo g, Macros fm, S *gS, 
     if (f >= g.nF) return;
     const d3 xi = g.xi[v];
     const bool bnd = f >= g.nIF;
-    if (bnd && !(sigma_dot(xi, g.Sf[f]) > 0)) return;  // patches: outgoing only (fvDVM.C:1431)
+    // patches: outgoing entries (fvDVM.C:1431); DVMsymmetry faces: then every entry once
+    // more, outgoing ones included (fvDVM.C:1449-1456)
+    const bool out = !bnd || sigma_dot(xi, g.Sf[f]) > 0;
+    const bool sym_all = bnd && g.bf_type[f - g.nIF] == kSymmetry;
+    if (!out && !sym_all) return;
     const double dt = *dtp;
     const double T = fm.T[f];
     const double Th = (bnd && g.q1) ? fm.T[g.bf_local[f - g.nIF]] : T;
@@ -303,8 +351,13 @@ __global__ void __launch_bounds__(kBlock) k_face_relax(Geo g, Macros fm, S *gS, 
     shakhov(g, fm.rho[f], fm.U[f], T, fm.q[f], xi, Th, gEq, hEq);
     const double b = 0.5 * dt / (2.0 * fm.tau[f] + 0.5 * dt);
     const size_t k = (size_t)v * g.nF + f;
-    gS[k] = (S)((1.0 - b) * (double)gS[k] + b * (gEq - g.ref_g[v]));
-    hS[k] = (S)((1.0 - b) * (double)hS[k] + b * (hEq - g.ref_h[v]));
+    double gv = (double)gS[k], hv = (double)hS[k];
+    for (int pass = (out ? 0 : 1); pass < (sym_all ? 2 : 1); ++pass) {
+        gv = (1.0 - b) * gv + b * (gEq - g.ref_g[v]);
+        hv = (1.0 - b) * hv + b * (hEq - g.ref_h[v]);
+    }
+    gS[k] = (S)gv;
+    hS[k] = (S)hv;
 }
 
 // updateGHtildeVol (fvDVM.C:1462-1519) as a cell-wise gather in ascending face order
@@ -451,6 +504,25 @@ __global__ void __launch_bounds__(kCflBlock)
     }
 }
 
+// updatePressureInOutBC (fvDVM.C:1745-1821): pressureIn faces rho_p = p / (R T_p); pressureOut
+// faces rho_p = rho_i + (p - rho_i R T_i) / a_i^2 and T_p = p / (R rho_i), with the owner
+// cell's state and a_i^2 = R T_i (K + 5) / (K + 3)
+__global__ void __launch_bounds__(kBlock) k_pressure_bc(Geo g, Macros cm, double *rho_b, double *bf_T)
+{
+    const int b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= g.nB) return;
+    const int t = g.bf_type[b];
+    if (t == kPressureIn) {
+        rho_b[b] = g.bf_p[b] / g.R / bf_T[b];
+    } else if (t == kPressureOut) {
+        const int own = g.owner[g.nIF + b];
+        const double Ti = cm.T[own], rhoi = cm.rho[own];
+        const double ai = sqrt(g.R * Ti * (g.K + 5) / (g.K + 3));
+        rho_b[b] = rhoi + (g.bf_p[b] - rhoi * g.R * Ti) / ai / ai;
+        bf_T[b] = g.bf_p[b] / (g.R * rhoi);
+    }
+}
+
 inline dim3 grid2(int n, int nv) { return dim3((n + kBlock - 1) / kBlock, nv); }
 inline dim3 grid1(int n) { return dim3((n + kBlock - 1) / kBlock); }
 
@@ -479,6 +551,10 @@ void launch_init_usurf(const Geo &g, const Macros &cm, Macros fm, hipStream_t st
 void launch_tau(const Geo &g, Macros m, int n, hipStream_t st)
 {
     if (n > 0) hipLaunchKernelGGL(k_tau, grid1(n), dim3(kBlock), 0, st, g, m, n);
+}
+void launch_pressure_bc(const Geo &g, const Macros &cm, double *rho_b, double *bf_T, hipStream_t st)
+{
+    if (g.nB > 0) hipLaunchKernelGGL(k_pressure_bc, grid1(g.nB), dim3(kBlock), 0, st, g, cm, rho_b, bf_T);
 }
 void launch_cfl_partials(const Geo &g, const Macros &fm, double *part, hipStream_t st)
 {
@@ -511,6 +587,16 @@ template <class S>
 void Staged<S>::wall_in(const Geo &g, const double *rho_b, S *gS, S *hS, hipStream_t st)
 {
     if (g.nB > 0) hipLaunchKernelGGL(k_wall_in<S>, grid2(g.nB, g.Nv), dim3(kBlock), 0, st, g, rho_b, gS, hS);
+}
+template <class S>
+void Staged<S>::far_field_in(const Geo &g, const Macros &cm, const double *rho_b, S *gS, S *hS, hipStream_t st)
+{
+    if (g.nB > 0) hipLaunchKernelGGL(k_far_field_in<S>, grid2(g.nB, g.Nv), dim3(kBlock), 0, st, g, cm, rho_b, gS, hS);
+}
+template <class S>
+void Staged<S>::symmetry_in(const Geo &g, S *gS, S *hS, hipStream_t st)
+{
+    if (g.nB > 0) hipLaunchKernelGGL(k_symmetry_in<S>, grid2(g.nB, g.Nv), dim3(kBlock), 0, st, g, gS, hS);
 }
 template <class S>
 void Staged<S>::face_moments(const Geo &g, const S *gS, const S *hS, Macros fm, const double *dt, double *cfl_part,

@@ -22,19 +22,20 @@ from . import mesh as meshmod
 from . import quadrature
 
 # rho patch types accepted by the path, with the DF face type they select
+# (src/fvDVM/discreteVelocity/discreteVelocity.C:188-195; symmetryPlane is a geometric
+# constraint type whose DF faces OpenFOAM sets by itself)
 RHO_BC_TO_DF = {
     "calculatedMaxwell": "maxwellWall",
     "fixedValue": "mixed",
-}
-# rho patch types the reference maps but this build does not carry
-RHO_BC_UNSUPPORTED = {
     "zeroGradient": "zeroGradient",
     "farField": "farField",
-    "symmetryMod": "DVMsymmetry",
     "pressureIn": "farField",
     "pressureOut": "farField",
+    "symmetryMod": "DVMsymmetry",
+    "symmetryPlane": "symmetryPlane",
 }
-PATCH_CODE = {"calculatedMaxwell": 1, "fixedValue": 2}
+PATCH_CODE = {"calculatedMaxwell": 1, "fixedValue": 2, "zeroGradient": 3, "farField": 4, "pressureIn": 5,
+              "pressureOut": 6, "symmetryMod": 7, "symmetryPlane": 8}
 
 
 @dataclass
@@ -44,6 +45,7 @@ class PatchBC:
     rho: np.ndarray   # [size]
     U: np.ndarray     # [size, 3]
     T: np.ndarray     # [size]
+    pressure: float = 0.0  # pressureIn / pressureOut patches
 
 
 @dataclass
@@ -82,6 +84,8 @@ class Case:
     controls: Controls = field(default_factory=Controls)
     q1_compat: bool = True
     meta: dict = field(default_factory=dict)
+    # gradient scheme of system/fvSchemes: "leastSquares" (dugksFoam's demos) or "Gauss linear"
+    grad_scheme: str = "leastSquares"
 
     @property
     def n_cells(self) -> int:
@@ -96,7 +100,7 @@ class Case:
         m, g = self.mesh, self.geo
         nIF = m.n_internal_faces
         bc_by_name = {b.name: b for b in self.bcs}
-        bfaces, ptype, pstart, psize, names = [], [], [], [], []
+        bfaces, ptype, pstart, psize, names, ppress = [], [], [], [], [], []
         b_rho, b_U, b_T = [], [], []
         for p in m.patches:
             if p.geometric_type == "empty" or p.size == 0:
@@ -107,6 +111,7 @@ class Case:
             if bc.rho_type not in RHO_BC_TO_DF:
                 raise ValueError(f"rho patch type {bc.rho_type!r} on {p.name!r} is not supported")
             ptype.append(PATCH_CODE[bc.rho_type])
+            ppress.append(float(bc.pressure))
             pstart.append(len(bfaces))
             psize.append(p.size)
             names.append(p.name)
@@ -117,20 +122,43 @@ class Case:
         faces = np.concatenate([np.arange(nIF), np.asarray(bfaces, dtype=np.int64)])
         i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
         f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
-        return dict(
+        out = dict(
             nC=m.n_cells, nIF=nIF, nB=len(bfaces), D=self.D,
             owner=i32(m.owner[faces]), neighbour=i32(m.neighbour),
             Sf=f64(g.Sf[faces]), Cf=f64(g.Cf[faces]), C=f64(g.C), V=f64(g.V),
             weights=f64(g.weights), delta_coeffs=f64(g.delta_coeffs),
             patch_names=names, patch_type=i32(ptype), patch_start=i32(pstart), patch_size=i32(psize),
+            patch_pressure=f64(ppress),
             bf_rho=f64(np.concatenate(b_rho) if b_rho else np.zeros(0)),
             bf_U=f64(np.concatenate(b_U) if b_U else np.zeros((0, 3))),
             bf_T=f64(np.concatenate(b_T) if b_T else np.zeros(0)),
             xis=f64(self.xis), wts=f64(self.wts), xi_max=float(self.xi_max),
             R=self.gas.R, omega=self.gas.omega, Tref=self.gas.Tref, muRef=self.gas.muRef,
             Pr=self.gas.Pr, K=int(self.gas.KInner), q1_compat=int(self.q1_compat),
-            rho0=f64(self.rho0), U0=f64(self.U0), T0=f64(self.T0),
+            rho0=f64(self.rho0), U0=f64(self.U0), T0=f64(self.T0), ls_own=None, ls_nei=None,
         )
+        out.update(self._gradient_vectors())
+        return out
+
+    def _gradient_vectors(self) -> dict:
+        """ls_own / ls_nei [nIF, 3] for the gradient scheme: None for leastSquares (the
+        library and the oracle form the least-squares vectors themselves); for Gauss linear,
+        grad_P = (1/V_P) sum_f S_f phi_f with phi_f = w phi_P + (1 - w) phi_N and boundary values
+        equal to the cell value, which by sum_f S_f = 0 over a closed cell is
+        sum_f S_f (1 - w_f)/V_P (phi_N - phi_P) at the owner and sum_f S_f w_f / V_N (phi_N - phi_P)
+        at the neighbour: the vectors S_f (1 - w_f) / V_P and -S_f w_f / V_N of the form
+        grad_P += own . (phi_N - phi_P), grad_N -= nei . (phi_N - phi_P)."""
+        if self.grad_scheme in ("leastSquares", "", None):
+            return {}
+        if self.grad_scheme.replace(" ", "") != "Gausslinear":
+            raise ValueError(f"gradient scheme {self.grad_scheme!r} is not supported "
+                             "(leastSquares, Gauss linear)")
+        m, g = self.mesh, self.geo
+        nIF = m.n_internal_faces
+        Sf, w = g.Sf[:nIF], g.weights
+        own, nei = m.owner[:nIF], m.neighbour
+        return dict(ls_own=np.ascontiguousarray(Sf * ((1.0 - w) / g.V[own])[:, None]),
+                    ls_nei=np.ascontiguousarray(-Sf * (w / g.V[nei])[:, None]))
 
 
 def mu_ref_from_kn(Kn: float, rho0: float = 1.0, R: float = 1.0, T0: float = 1.0, L: float = 1.0,
@@ -190,6 +218,42 @@ def cavity_case(N: int, rule: str, n_dv: int, vel_extent: float, Kn: float,
                 xi_max=float(np.max(np.abs(xis))), xi_min=float(np.min(xis)), gas=gas,
                 rho0=rho0, U0=U0, T0=T0c, bcs=bcs, controls=controls or Controls(),
                 meta=dict(Kn=Kn, lid_speed=lid_speed, rule=rule, n_dv=n_dv, seed_tag=seed_tag))
+
+
+def box_case(nx: int, ny: int, sides: dict, rule: str = "GH", n_dv: int = 8, vel_extent: float = 1.0,
+             Kn: float = 0.1, extent=(0.0, 1.0, 0.0, 1.0), rho0=1.0, U0=(0.0, 0.0, 0.0), T0=1.0,
+             controls: Controls | None = None, name: str | None = None) -> Case:
+    """A 2-D box (nondimensional, R = Tref = 1) with one boundary condition per side.
+
+    ``sides`` maps ``left``, ``right``, ``bottom``, ``top`` to ``(rho_type, rho, U, T)`` or
+    ``(rho_type, rho, U, T, pressure)``; rho, U, T may be scalars / 3-vectors or per-face
+    arrays.  ``rho0``, ``U0``, ``T0`` are the initial cell fields (constants or callables of
+    the cell centres).  ``symmetryPlane`` sides get that geometric type, walls ``wall``."""
+    gtypes = {k: ("symmetryPlane" if v[0] == "symmetryPlane" else "wall" if v[0] == "calculatedMaxwell" else "patch")
+              for k, v in sides.items()}
+    x0, x1, y0, y1 = extent
+    m = meshmod.box2d(nx, ny, x0, x1, y0, y1, types=gtypes)
+    geo = meshmod.geometry(m)
+    gas = Gas(R=1.0, omega=0.81, Tref=1.0, muRef=mu_ref_from_kn(Kn, omega=0.81), Pr=2.0 / 3.0, KInner=0)
+    scale = vel_extent * math.sqrt((2.0 if rule.startswith("GH") else 1.0))
+    xis, wts = quadrature.make_rule(rule, scale, n_dv)
+    bcs = []
+    for p in m.patches:
+        if p.geometric_type == "empty":
+            continue
+        spec = sides[p.name]
+        rt, r, u, t = spec[:4]
+        bcs.append(PatchBC(p.name, rt, np.broadcast_to(np.asarray(r, float), (p.size,)).copy(),
+                           np.broadcast_to(np.asarray(u, float), (p.size, 3)).copy(),
+                           np.broadcast_to(np.asarray(t, float), (p.size,)).copy(),
+                           float(spec[4]) if len(spec) > 4 else 0.0))
+    C = geo.C
+    f = lambda v, width: (np.asarray(v(C), float) if callable(v) else np.broadcast_to(np.asarray(v, float),
+                                                                                       (m.n_cells,) + width)).copy()
+    return Case(name=name or f"box{nx}x{ny}", mesh=m, geo=geo, D=2, xis=xis, wts=wts,
+                xi_max=float(np.max(np.abs(xis))), xi_min=float(np.min(xis)), gas=gas,
+                rho0=f(rho0, ()), U0=f(U0, (3,)), T0=f(T0, ()), bcs=bcs,
+                controls=controls or Controls(maxCo=STABLE_MAX_CO[2]), meta=dict(Kn=Kn, rule=rule, n_dv=n_dv))
 
 
 def config0() -> Case:

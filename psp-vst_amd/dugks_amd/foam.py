@@ -60,6 +60,8 @@ def load_library(path: str = LIB_PATH):
     L.dugks_foam_patch.argtypes = [vp, ct.c_int, ct.POINTER(ct.c_char_p), ct.POINTER(ct.c_char_p),
                                    ct.POINTER(ct.c_char_p), ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]
     L.dugks_foam_controls_get.argtypes = [vp, ct.POINTER(Controls)]
+    L.dugks_foam_grad_scheme.argtypes = [vp]
+    L.dugks_foam_grad_scheme.restype = ct.c_char_p
     L.dugks_foam_write.argtypes = [vp, ct.POINTER(Output)]
     L.dugks_foam_write_df.argtypes = [vp, ct.c_double, ct.c_int, ct.POINTER(ct.c_double), ct.c_int]
     _lib = L
@@ -126,6 +128,7 @@ def read_case(case_dir: str, time_name: str | None = None) -> casesmod.Case:
                               n_cells=nC)
         bcs, off = [], 0
         bf_rho, bf_U, bf_T = h.d("bf_rho"), h.d("bf_U", (-1, 3)), h.d("bf_T")
+        press = h.d("patch_pressure")
         for k in range(int(h.s("n_patches"))):
             name, gtype, rtype, start, size = h.patch(k)
             pm.patches.append(meshmod.Patch(name=name, geometric_type="empty" if gtype == "empty" else gtype,
@@ -133,7 +136,7 @@ def read_case(case_dir: str, time_name: str | None = None) -> casesmod.Case:
             if gtype == "empty" or size == 0:
                 continue
             bcs.append(casesmod.PatchBC(name, rtype, bf_rho[off:off + size].copy(), bf_U[off:off + size].copy(),
-                                        bf_T[off:off + size].copy()))
+                                        bf_T[off:off + size].copy(), float(press[len(bcs)])))
             off += size
         Sf, Cf = h.d("Sf", (-1, 3)), h.d("Cf", (-1, 3))
         geo = meshmod.Geometry(Sf=Sf, Cf=Cf, magSf=np.linalg.norm(Sf, axis=1), C=h.d("C", (-1, 3)), V=h.d("V"),
@@ -149,7 +152,8 @@ def read_case(case_dir: str, time_name: str | None = None) -> casesmod.Case:
                           bcs=bcs, controls=controls,
                           meta=dict(case_dir=case_dir, time=h.s("time"), end_time=ctl.end_time,
                                     converge_tol=ctl.converge_tol, converge_check_steps=ctl.converge_check_steps,
-                                    df_write_cells=h.i("df_write_cells").tolist()))
+                                    df_write_cells=h.i("df_write_cells").tolist()),
+                          grad_scheme=h.L.dugks_foam_grad_scheme(h.h).decode())
         assert c.n_cells == nC and pm.n_internal_faces == nIF
         return c
     finally:
@@ -234,6 +238,9 @@ def write_case(case: casesmod.Case, case_dir: str, end_time: float = 1.0e300, wr
         with open(os.path.join(case_dir, "constant", name), "w") as f:
             f.write(_header("scalarList", "constant", name))
             f.write(f"{len(vals)}\n(\n" + "\n".join(_num(v) for v in vals) + "\n)\n")
+    with open(os.path.join(case_dir, "system", "fvSchemes"), "w") as f:
+        f.write(_header("dictionary", "system", "fvSchemes"))
+        f.write(f"gradSchemes\n{{\n    default         {case.grad_scheme};\n}}\n")
     c = case.controls
     with open(os.path.join(case_dir, "system", "controlDict"), "w") as f:
         f.write(_header("dictionary", "system", "controlDict"))
@@ -259,8 +266,11 @@ def write_case(case: casesmod.Case, case_dir: str, end_time: float = 1.0e300, wr
                 b = bc_by_name[p.name]
                 vals = {"rho": np.broadcast_to(b.rho, (p.size,)), "U": np.broadcast_to(b.U, (p.size, 3)),
                         "T": np.broadcast_to(b.T, (p.size,))}[name]
-                ptype = b.rho_type if name == "rho" else "fixedValue"
-                f.write(f"    {p.name}\n    {{\n        type            {ptype};\n        value           "
+                ptype = b.rho_type if name == "rho" else ("symmetryPlane" if b.rho_type == "symmetryPlane"
+                                                          else "fixedValue")
+                extra = (f"        {b.rho_type}      {_num(b.pressure)};\n"
+                         if name == "rho" and b.rho_type in ("pressureIn", "pressureOut") else "")
+                f.write(f"    {p.name}\n    {{\n        type            {ptype};\n{extra}        value           "
                         + _values(vals, vtype) + ";\n    }\n")
             f.write("}\n")
     return case_dir

@@ -34,7 +34,7 @@ class Mesh(ct.Structure):
 
 class Patch(ct.Structure):
     _fields_ = [("type", ct.c_int), ("start", ct.c_int), ("size", ct.c_int),
-                ("rho", _D), ("U", _D), ("T", _D)]
+                ("rho", _D), ("U", _D), ("T", _D), ("pressure", ct.c_double)]
 
 
 class VSet(ct.Structure):
@@ -135,6 +135,8 @@ class FvDVM:
         self.nF = self.nIF + self.nB
         self.D = a["D"]
         ls_own = ls_nei = None
+        if ls_vectors is None and a.get("ls_own") is not None:   # the case's gradient scheme
+            ls_vectors = (a["ls_own"], a["ls_nei"])
         if ls_vectors is not None:
             ls_own = np.ascontiguousarray(ls_vectors[0], dtype=np.float64)
             ls_nei = np.ascontiguousarray(ls_vectors[1], dtype=np.float64)
@@ -152,7 +154,8 @@ class FvDVM:
             U = np.ascontiguousarray(a["bf_U"][s:s + n])
             T = np.ascontiguousarray(a["bf_T"][s:s + n])
             self._pv += [rho, U, T]
-            patches[i] = Patch(type=int(a["patch_type"][i]), start=s, size=n, rho=_p(rho), U=_p(U), T=_p(T))
+            patches[i] = Patch(type=int(a["patch_type"][i]), start=s, size=n, rho=_p(rho), U=_p(U), T=_p(T),
+                               pressure=float(a["patch_pressure"][i]))
         vset = VSet(n_per_dim=len(a["xis"]), xis=_p(a["xis"]), weights=_p(a["wts"]), xi_max=a["xi_max"],
                     xi_min=float(case.xi_min))
         gas = Gas(R=a["R"], omega=a["omega"], Tref=a["Tref"], muRef=a["muRef"], Pr=a["Pr"], KInner=a["K"])

@@ -339,6 +339,39 @@ def cavity(N: int, L: float = 1.0, dz: float = 0.1) -> PolyMesh:
                     patches=patches, n_cells=N * N)
 
 
+def box2d(nx: int, ny: int, x0: float = 0.0, x1: float = 1.0, y0: float = 0.0, y1: float = 1.0,
+          dz: float = 0.1, types=None) -> PolyMesh:
+    """Uniform nx x ny x 1 hex mesh of [x0,x1] x [y0,y1] x [0,dz] with the patches ``left``,
+    ``right``, ``bottom``, ``top`` (geometric types from ``types``, default ``patch``) and
+    ``frontAndBack`` (empty)."""
+    xs = np.linspace(x0, x1, nx + 1)
+    ys = np.linspace(y0, y1, ny + 1)
+    k_, j_, i_ = np.meshgrid(np.arange(2), np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
+    pts = np.stack([xs[i_.ravel()], ys[j_.ravel()], np.where(k_.ravel() == 0, 0.0, dz)], axis=1)
+    pid = lambda i, j, k: i + (nx + 1) * (j + (ny + 1) * k)
+    cells = []
+    for j in range(ny):
+        for i in range(nx):
+            cells.append([pid(i, j, 0), pid(i + 1, j, 0), pid(i + 1, j + 1, 0), pid(i, j + 1, 0),
+                          pid(i, j, 1), pid(i + 1, j, 1), pid(i + 1, j + 1, 1), pid(i, j + 1, 1)])
+    names = ["left", "right", "bottom", "top", "frontAndBack"]
+    types = dict(types or {})
+    gtypes = [types.get(n, "patch") for n in names[:4]] + ["empty"]
+    tol = 1e-12 * max(abs(x1 - x0), abs(y1 - y0))
+
+    def classify(q):
+        c = q.mean(axis=1)
+        out = np.full(len(q), -1)
+        out[np.abs(c[:, 0] - x0) < tol] = 0
+        out[np.abs(c[:, 0] - x1) < tol] = 1
+        out[np.abs(c[:, 1] - y0) < tol] = 2
+        out[np.abs(c[:, 1] - y1) < tol] = 3
+        out[(np.abs(c[:, 2]) < tol) | (np.abs(c[:, 2] - dz) < tol)] = 4
+        return out
+
+    return hex_mesh(pts, np.array(cells), names, gtypes, classify)
+
+
 # OpenFOAM hex model faces (vertex order 0-3 bottom, 4-7 top), outward for a right-handed hex
 _HEX_FACES = np.array([[0, 4, 7, 3], [1, 2, 6, 5], [0, 1, 5, 4], [3, 7, 6, 2], [0, 3, 2, 1], [4, 5, 6, 7]])
 

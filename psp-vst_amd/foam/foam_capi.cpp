@@ -60,8 +60,8 @@ int dugks_foam_solver_inputs(const dugks_foam_case *h, dugks_mesh *mesh, dugks_p
         mesh->V = c.V.data();
         mesh->weights = c.weights.data();
         mesh->delta_coeffs = c.delta_coeffs.data();
-        mesh->ls_own = nullptr;
-        mesh->ls_nei = nullptr;
+        mesh->ls_own = c.ls_own.empty() ? nullptr : c.ls_own.data();
+        mesh->ls_nei = c.ls_nei.empty() ? nullptr : c.ls_nei.data();
     }
     const int np = (int)c.solver_patch.size();
     if (n_patches) *n_patches = np;
@@ -74,6 +74,7 @@ int dugks_foam_solver_inputs(const dugks_foam_case *h, dugks_mesh *mesh, dugks_p
             patches[i].rho = c.bf_rho.data() + s;
             patches[i].U = c.bf_U.data() + 3 * (size_t)s;
             patches[i].T = c.bf_T.data() + s;
+            patches[i].pressure = c.patch_pressure[i];
         }
     }
     if (vset) {
@@ -144,6 +145,7 @@ int dugks_foam_array_d(const dugks_foam_case *h, const char *name, const double 
     else if (n == "bf_rho") v = &c.bf_rho;
     else if (n == "bf_U") v = &c.bf_U;
     else if (n == "bf_T") v = &c.bf_T;
+    else if (n == "patch_pressure") v = &c.patch_pressure;
     else return fail("dugks_foam_array_d: no double array named " + n);
     *data = v->data();
     *count = v->size();
@@ -192,6 +194,8 @@ int dugks_foam_scalar(const dugks_foam_case *h, const char *name, double *value)
     else return fail("dugks_foam_scalar: no scalar named " + n);
     return 0;
 }
+
+const char *dugks_foam_grad_scheme(const dugks_foam_case *h) { return h ? h->c.grad_scheme.c_str() : ""; }
 
 int dugks_foam_patch(const dugks_foam_case *h, int i, const char **name, const char **geom_type, const char **rho_type,
                      int *start, int *size)

@@ -170,6 +170,9 @@ bool read_vol_field(const Case &c, const std::string &name, int width, VolField 
         if (const Entry *val = pd->find("value"))
             if (!field_value(val->tokens, width, (size_t)pi.size, fld.bc[p].value, path + " patch " + pi.name, err))
                 return false;
+        for (const char *key : {"pressureIn", "pressureOut"})
+            if (const Entry *pe = pd->find(key))
+                if (!entry_scalar(*pe, fld.bc[p].pressure, err, key)) return false;
     }
     return true;
 }
@@ -188,6 +191,12 @@ int df_patch_code(const std::string &t)
 {
     if (t == "calculatedMaxwell") return 1;  // maxwellWall
     if (t == "fixedValue") return 2;         // mixed: incoming half frozen at the patch Maxwellian
+    if (t == "zeroGradient") return 3;
+    if (t == "farField") return 4;
+    if (t == "pressureIn") return 5;         // farField with the inlet pressure
+    if (t == "pressureOut") return 6;        // farField with the outlet pressure
+    if (t == "symmetryMod") return 7;        // DVMsymmetry
+    if (t == "symmetryPlane") return 8;      // geometric symmetry plane
     if (t == "empty") return 0;
     return -1;
 }
@@ -507,6 +516,36 @@ bool read_case(const std::string &dir, const std::string &tname, Case &c, std::s
         }
     }
     if (!compute_geometry(c, err)) return false;
+    // ---- gradient scheme (system/fvSchemes; absent: leastSquares)
+    {
+        FoamFile f;
+        std::string e2;
+        if (read_foam_dict(join(dir, "system/fvSchemes"), f, e2)) {
+            if (const Dict *gs = f.body.sub("gradSchemes"))
+                if (const Entry *d = gs->find("default")) {
+                    std::string sch;
+                    for (const Token &t : d->tokens) sch += (sch.empty() ? "" : " ") + t.text;
+                    c.grad_scheme = sch;
+                }
+        }
+        if (c.grad_scheme == "Gauss linear") {
+            // grad_P = (1/V_P) sum_f S_f (w phi_P + (1-w) phi_N) with boundary values equal to
+            // the cell value is, as sum_f S_f = 0 over a closed cell, the form
+            // grad_P += own.(phi_N - phi_P), grad_N -= nei.(phi_N - phi_P) with these vectors
+            c.ls_own.resize(3 * (size_t)c.nIF);
+            c.ls_nei.resize(3 * (size_t)c.nIF);
+            for (int fi = 0; fi < c.nIF; ++fi) {
+                const double w = c.weights[fi];
+                for (int k = 0; k < 3; ++k) {
+                    c.ls_own[3 * (size_t)fi + k] = c.Sf[3 * (size_t)fi + k] * (1.0 - w) / c.V[c.owner[fi]];
+                    c.ls_nei[3 * (size_t)fi + k] = -c.Sf[3 * (size_t)fi + k] * w / c.V[c.neighbour[fi]];
+                }
+            }
+        } else if (c.grad_scheme != "leastSquares") {
+            err = "gradSchemes default '" + c.grad_scheme + "' is not supported (leastSquares, Gauss linear)";
+            return false;
+        }
+    }
     // ---- DVMProperties (fvDVM.C:1928-1948) and the velocity set (fvDVM.C:62-123)
     {
         const std::string path = join(dir, "constant/DVMProperties");
@@ -571,19 +610,31 @@ bool read_case(const std::string &dir, const std::string &tname, Case &c, std::s
             return false;
         }
         const FieldBC &bu = c.U.bc[p], &bt = c.T.bc[p], &br = c.rho.bc[p];
-        if (bu.value.size() != 3 * (size_t)pi.size || bt.value.size() != (size_t)pi.size ||
-            br.value.size() != (size_t)pi.size) {
-            err = "patch " + pi.name + ": rho, U and T need patch values (fixedValue U/T, rho value)";
+        // values each type reads: walls and fixedValue patches rho, U, T; farField rho, T;
+        // pressureIn T and the pressure; pressureOut the pressure (rho and T follow from the
+        // cells); zeroGradient and symmetry patches none (missing values are taken as 1 / 0)
+        const bool need_rho = code <= 2 || code == 4, need_U = code <= 2, need_T = code <= 2 || code == 4 || code == 5;
+        auto size_ok = [&](const FieldBC &f, size_t w) { return f.value.size() == w * (size_t)pi.size; };
+        if ((need_rho && !size_ok(br, 1)) || (need_U && !size_ok(bu, 3)) || (need_T && !size_ok(bt, 1))) {
+            err = "patch " + pi.name + ": the " + rt + " boundary needs patch values of " +
+                  (code <= 2 ? "rho, U and T" : code == 4 ? "rho and T" : "T");
+            return false;
+        }
+        if ((code == 5 || code == 6) && !(br.pressure > 0)) {
+            err = "patch " + pi.name + ": " + rt + " needs a positive `" + rt + "` entry";
             return false;
         }
         c.solver_patch.push_back((int)p);
         c.patch_code.push_back(code);
         c.patch_start.push_back(c.nB);
         c.patch_size.push_back(pi.size);
-        for (int i = 0; i < pi.size; ++i) c.abi_face.push_back(pi.start + i);
-        c.bf_rho.insert(c.bf_rho.end(), br.value.begin(), br.value.end());
-        c.bf_U.insert(c.bf_U.end(), bu.value.begin(), bu.value.end());
-        c.bf_T.insert(c.bf_T.end(), bt.value.begin(), bt.value.end());
+        c.patch_pressure.push_back(br.pressure);
+        for (int i = 0; i < pi.size; ++i) {
+            c.abi_face.push_back(pi.start + i);
+            c.bf_rho.push_back(size_ok(br, 1) ? br.value[i] : 1.0);
+            for (int k = 0; k < 3; ++k) c.bf_U.push_back(size_ok(bu, 3) ? bu.value[3 * (size_t)i + k] : 0.0);
+            c.bf_T.push_back(size_ok(bt, 1) ? bt.value[i] : 1.0);
+        }
         c.nB += pi.size;
     }
     const size_t nabi = c.abi_face.size();

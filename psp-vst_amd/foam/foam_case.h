@@ -10,6 +10,7 @@
 //   system/controlDict       startFrom/startTime, endTime, deltaT, writeControl,
 //                            writeInterval, writePrecision, timePrecision, adjustTimeStep,
 //                            maxCo, maxDeltaT, convergeTol, convergeCheckSteps
+//   system/fvSchemes         gradSchemes default: leastSquares or Gauss linear
 //   <time>/{rho,U,T}         internalField and boundaryField (patch type and value)
 // Geometry follows OpenFOAM's primitiveMesh (face centres and area vectors by triangle
 // fans about the face's point average; cell centres and volumes by face pyramids about the
@@ -35,6 +36,7 @@ struct PatchInfo {
 struct FieldBC {
     std::string type;             // boundaryField type
     std::vector<double> value;    // patch values [size * width] (empty when the patch has none)
+    double pressure = 0.0;        // pressureIn / pressureOut entry of the patch (rho)
 };
 
 struct VolField {
@@ -75,6 +77,7 @@ struct Case {
     std::vector<double> abi_Sf, abi_Cf;
     std::vector<int> solver_patch;            // non-empty patch -> polyMesh patch index
     std::vector<int> patch_code, patch_start, patch_size;  // per non-empty patch (start within boundary faces)
+    std::vector<double> patch_pressure;       // per non-empty patch (pressureIn / pressureOut)
     std::vector<double> bf_rho, bf_U, bf_T;   // [nB], [nB][3], [nB]
     // DVMProperties, velocity set
     int nDV = 0;
@@ -82,6 +85,8 @@ struct Case {
     double R = 0.0, omega = 0.0, Tref = 0.0, muRef = 0.0, Pr = 0.0;
     int KInner = 0;
     std::string macro_flux = "no";
+    std::string grad_scheme = "leastSquares";   // system/fvSchemes gradSchemes default
+    std::vector<double> ls_own, ls_nei;         // Gauss linear: gradient vectors [nIF][3]
     std::vector<int> df_write_cells;
     std::vector<double> xis, wts;
     // fields of the time directory read

@@ -123,6 +123,32 @@ def test_written_fields_read_back(tmp_path):
     assert "index           42;" in txt and 'name            "0.125";' in txt
 
 
+def test_boundary_types_round_trip(tmp_path):
+    """Every rho boundary type the path carries survives write_case -> read_case with its
+    DF code and pressure (discreteVelocity.C:188-195)."""
+    from dugks_amd import cases
+    U = (0.1, 0.0, 0.0)
+    sides = dict(left=("pressureIn", 1.0, U, 1.1, 1.05), right=("pressureOut", 1.0, U, 1.0, 0.95),
+                 bottom=("symmetryPlane", 1.0, U, 1.0), top=("symmetryMod", 1.0, U, 1.0))
+    c = cases.box_case(5, 4, sides, rule="NC38", n_dv=7, vel_extent=4.0)
+    d = str(tmp_path / "box")
+    _foam().write_case(c, d)
+    r = _foam().read_case(d)
+    a, b = c.arrays(), r.arrays()
+    assert list(b["patch_type"]) == [5, 6, 8, 7]     # pressureIn, pressureOut, symmetryPlane, symmetryMod
+    assert list(b["patch_pressure"]) == [1.05, 0.95, 0.0, 0.0]
+    assert np.array_equal(a["bf_T"][:4], b["bf_T"][:4])
+    assert [p.geometric_type for p in r.mesh.patches] == ["patch", "patch", "symmetryPlane", "patch", "empty"]
+    zg = ("zeroGradient", 1.0, U, 1.0)
+    c2 = cases.box_case(4, 4, dict(left=("farField", 1.2, U, 0.9), right=zg, bottom=zg, top=zg), rule="NC38",
+                        n_dv=7, vel_extent=4.0)
+    d2 = str(tmp_path / "box2")
+    _foam().write_case(c2, d2)
+    b2 = _foam().read_case(d2).arrays()
+    assert list(b2["patch_type"]) == [4, 3, 3, 3]
+    assert np.all(b2["bf_rho"][:4] == 1.2) and np.all(b2["bf_T"][:4] == 0.9)
+
+
 def _copy_demo(tmp_path):
     d = tmp_path / "demo"
     shutil.copytree(DEMO, d)
@@ -135,8 +161,11 @@ def test_reader_error_paths(tmp_path):
         foam.read_case(str(tmp_path / "nothing"))
     d = _copy_demo(tmp_path)
     rho = (d / "0" / "rho").read_text()
-    (d / "0" / "rho").write_text(rho.replace("calculatedMaxwell", "zeroGradient", 1))
-    with pytest.raises(foam.FoamError, match="zeroGradient.*not supported"):
+    (d / "0" / "rho").write_text(rho.replace("calculatedMaxwell", "totalPressure", 1))
+    with pytest.raises(foam.FoamError, match="totalPressure.*not supported"):
+        foam.read_case(str(d))
+    (d / "0" / "rho").write_text(rho.replace("calculatedMaxwell", "pressureIn", 1))
+    with pytest.raises(foam.FoamError, match="positive `pressureIn`"):
         foam.read_case(str(d))
     (d / "0" / "rho").write_text(rho)
     ctl = (d / "system" / "controlDict").read_text()

@@ -358,3 +358,66 @@ def test_checkpoint_refuses_other_storage_reference(tmp_path):
     other = hip.FvDVM(c2, precision=hip.MIXED)
     with pytest.raises(hip.DugksError, match="reference"):
         other.checkpoint_load(path)
+
+
+def _boundary_cases():
+    from dugks_amd import cases
+    U = (0.15, 0.05, 0.0)
+    wall = ("calculatedMaxwell", 1.0, (0.0, 0.0, 0.0), 1.0)
+    zg = ("zeroGradient", 1.0, U, 1.0)
+    bumpy = dict(rho0=lambda C: 1.0 + 0.1 * np.sin(3 * C[:, 0]) * np.cos(2 * C[:, 1]),
+                 U0=lambda C: np.stack([0.1 + 0.05 * C[:, 1], -0.05 * C[:, 0], 0 * C[:, 0]], 1),
+                 T0=lambda C: 1.0 + 0.05 * C[:, 0] * C[:, 1], rule="NC38", n_dv=13, vel_extent=4.0)
+    return {
+        "farField_zeroGradient": cases.box_case(10, 8, dict(left=("farField", 1.0, (9.0, 0, 0), 1.0),
+                                                            right=("farField", 0.9, U, 1.1), bottom=zg, top=zg),
+                                                **bumpy),
+        "pressure": cases.box_case(12, 6, dict(left=("pressureIn", 1.0, U, 1.1, 1.05),
+                                               right=("pressureOut", 1.0, U, 1.0, 0.95), bottom=wall, top=wall),
+                                   extent=(0.0, 2.0, 0.0, 1.0), **bumpy),
+        "symmetryMod": cases.box_case(6, 8, dict(left=wall, right=("symmetryMod", 1.0, U, 1.0), bottom=wall,
+                                                 top=("calculatedMaxwell", 1.0, (0.0, 0.2, 0.0), 1.0)),
+                                      extent=(-1.0, 0.0, 0.0, 1.0), **bumpy),
+        "symmetryPlane": cases.box_case(6, 8, dict(left=wall, right=("symmetryPlane", 1.0, U, 1.0), bottom=wall,
+                                                   top=("calculatedMaxwell", 1.0, (0.0, 0.2, 0.0), 1.0)),
+                                        extent=(-1.0, 0.0, 0.0, 1.0), **bumpy),
+    }
+
+
+@pytest.mark.parametrize("kind", ["farField_zeroGradient", "pressure", "symmetryMod", "symmetryPlane"])
+def test_boundary_types_match_oracle(kind):
+    """zeroGradient, farField, pressureIn / pressureOut and both symmetry types on the staged
+    path (the fused schedules carry walls and fixedValue patches only and refuse these; with
+    fused = -1 the handle falls back to the staged path): FP64 within 1e-12 of the oracle
+    after 20 steps in cell and face fields and boundary rho, mixed storage within 1e-5."""
+    from dugks_amd import hip
+    c = _boundary_cases()[kind]
+    with pytest.raises(hip.DugksError, match="fused path carries"):
+        hip.FvDVM(c, fused=1)
+    o, g, ho, hg = _run_pair(c, 20, 0, -1)
+    assert g.path_info()["fused"] == 0
+    e = _errs(o, g)
+    fo, fg = o.face_macros(), g.face_macros()
+    ef = {k: field_rel_err(fg[k], fo[k]) for k in ("rho", "U", "T", "q")}
+    print("boundary", kind, e, ef, flush=True)
+    assert np.max(np.abs(hg - ho) / ho) < 1e-12
+    for k in ("rho", "U", "T", "q"):
+        assert e[k] < 1e-12 and ef[k] < 1e-12, k
+    assert field_rel_err(g.boundary_rho(), o.wall_rho()) < 1e-12
+    o2, g2, _, _ = _run_pair(c, 20, 1, -1)
+    e2 = _errs(o2, g2)
+    for k in ("rho", "U", "T"):
+        assert e2[k] < 1e-5, (k, e2[k])
+
+
+@pytest.mark.parametrize("fused", [0, 2, 3])
+def test_gauss_linear_gradient_matches_oracle(fused):
+    """Gauss linear gradients (demo/ball's scheme) reach every path as vectors of the
+    least-squares form: FP64 within 1e-12 of the oracle after 30 steps on a 20x20 cavity."""
+    c = _cases().cavity_case(20, "NC38", 10, 4.0, 0.1)
+    c.grad_scheme = "Gauss linear"
+    o, g, ho, hg = _run_pair(c, 30, 0, fused)
+    e = _errs(o, g)
+    print("gauss linear", fused, e)
+    for k, v in e.items():
+        assert v < 1e-12, (k, v)

@@ -142,6 +142,8 @@ def test_case_arrays_skip_empty_patches_and_map_bc_types():
     assert list(a["patch_type"]) == [1, 1] and list(a["patch_start"]) == [0, 40]
     assert np.allclose(a["bf_U"][:40, 0], 0.16 * math.sqrt(2.0))
     c.bcs[0].rho_type = "farField"
+    assert list(c.arrays()["patch_type"]) == [4, 1]
+    c.bcs[0].rho_type = "totalPressure"
     with pytest.raises(ValueError, match="not supported"):
         c.arrays()
 
