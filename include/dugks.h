@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define DUGKS_ABI_VERSION 2
+#define DUGKS_ABI_VERSION 3
 
 /* storage precision of the distributions */
 enum dugks_precision {
@@ -88,13 +88,21 @@ typedef struct dugks_patch {
     double pressure;           /* pressureIn / pressureOut patches: the patch pressure */
 } dugks_patch;
 
-/* constant/Xis, constant/weights and the fvDVMparas entries (fvDVM.C:1928-1933). */
+/* constant/Xis, constant/weights and the fvDVMparas entries (fvDVM.C:1928-1933), or the
+ * velocity mesh of the VMESH build (fvDVM.C:129-253). */
 typedef struct dugks_vset {
-    int n_per_dim;             /* nDV */
+    int n_per_dim;             /* nDV: tensor-product set of xis x xis (x xis), ix fastest */
     const double *xis;         /* [nDV] */
     const double *weights;     /* [nDV] full quadrature measures */
-    double xi_max;             /* xiMax (used by the Courant number only) */
+    double xi_max;             /* xiMax (used by the Courant number only; the VMESH build sets it
+                                  to the largest |xi| of the list) */
     double xi_min;             /* xiMin (unused by the step, kept for the record) */
+    int n_list;                /* > 0: the velocity set is list_xi / list_w instead (the cell
+                                  centres and volumes of constant/vMesh); components of list_xi
+                                  beyond n_solution_d are dropped as fvDVM.C:213, 238 do.  Runs
+                                  on the staged path; symmetry patches need a tensor set. */
+    const double *list_xi;     /* [n_list][3] */
+    const double *list_w;      /* [n_list] */
 } dugks_vset;
 
 /* gasProperties (fvDVM.C:1943-1948). */
@@ -115,6 +123,17 @@ typedef struct dugks_options {
                                   3: fused, recompute schedule required;
                                   0: staged path with materialised face distributions;
                                   -1: fused when available, else staged */
+    int macro_flux;            /* updateMacroVol (fvDVM.C:1521-1682): 0 macroFlux no: cell
+                                  rho, rhoU, rhoE are moments of g~, h~;
+                                  1 macroFlux yes: the previous conserved variables minus the
+                                    face-macro fluxes, as the default build computes them
+                                    (F:1620-1679: no face stress, rho_f|U_f|^2 in the energy
+                                    flux, boundary momentum dotted with the internal-face Sf at
+                                    the patch-local index);
+                                  2 macroFlux yes as the VMESH build computes it: minus the
+                                    fluxes of the velocity moments of the relaxed face DFs
+                                    (F:1562-1618).
+                                  1 and 2 run on the staged path. */
 } dugks_options;
 
 typedef struct dugks_handle dugks_handle;

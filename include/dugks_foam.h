@@ -34,13 +34,19 @@ typedef struct dugks_foam_case dugks_foam_case;
 /* Reads the case in case_dir.  time_name NULL or "": the time directory named by
  * controlDict's startFrom (startTime, firstTime, latestTime). */
 int dugks_foam_read(const char *case_dir, const char *time_name, dugks_foam_case **out);
+/* As dugks_foam_read for dugksFoam built with -DVMESH (src/fvDVM/Make/options:13): the
+ * velocity set is the cell centres and volumes of the polyMesh in constant/vMesh, components
+ * beyond the mesh's nSolutionD set to 0, and xiMax the largest |xi| (fvDVM.C:129-253);
+ * constant/Xis and constant/weights are not read.  macroFlux yes then selects the
+ * velocity-moment flux update (dugks_options.macro_flux 2, fvDVM.C:1562-1618). */
+int dugks_foam_read_vmesh(const char *case_dir, const char *time_name, dugks_foam_case **out);
 void dugks_foam_free(dugks_foam_case *c);
 const char *dugks_foam_last_error(void);
 
 /* The case as dugks_create takes it.  Patches: the non-empty patches in polyMesh order
  * (at most patch_cap are written; *n_patches receives the count).  Pointers inside the
  * structs stay valid until dugks_foam_free.  Boundary faces of the DUGKS C-ABI are the faces
- * of these patches in that order. */
+ * of these patches in that order.  dugks_options.macro_flux is the scalar "macro_flux". */
 int dugks_foam_solver_inputs(const dugks_foam_case *c, dugks_mesh *mesh, dugks_patch *patches, int patch_cap,
                              int *n_patches, dugks_vset *vset, dugks_gas *gas);
 /* cell fields of the time directory read: [n_cells], [n_cells][3], [n_cells] */
@@ -60,12 +66,12 @@ int dugks_foam_controls_get(const dugks_foam_case *c, dugks_foam_controls *ctl);
  * double arrays: "points" [nP][3], "Sf", "Cf" [nF][3] (polyMesh faces), "C" [nC][3], "V",
  *   "weights", "delta_coeffs" [nIF], "xis", "xi_weights" [nDV], "rho", "T" [nC], "U" [nC][3],
  *   "bf_rho", "bf_T" [nB], "bf_U" [nB][3] (C-ABI boundary faces), "patch_pressure" [non-empty
- *   patches] (pressureIn / pressureOut);
+ *   patches] (pressureIn / pressureOut), "xi_list" [n][3], "xi_list_weights" [n] (VMESH);
  * int arrays: "face_offsets" [nF+1], "face_points", "owner" [nF], "neighbour" [nIF],
  *   "abi_faces" [nIF + nB] (C-ABI face -> polyMesh face), "df_write_cells";
  * scalars: "n_cells", "n_faces", "n_internal_faces", "n_boundary_faces", "n_solution_d",
  *   "n_patches" (all polyMesh patches), "nDV", "xiMax", "xiMin", "R", "omega", "Tref",
- *   "muRef", "Pr", "KInner", "time". */
+ *   "muRef", "Pr", "KInner", "time", "macro_flux" (dugks_options.macro_flux), "vmesh". */
 int dugks_foam_array_d(const dugks_foam_case *c, const char *name, const double **data, size_t *count);
 int dugks_foam_array_i(const dugks_foam_case *c, const char *name, const int **data, size_t *count);
 int dugks_foam_scalar(const dugks_foam_case *c, const char *name, double *value);

@@ -189,8 +189,13 @@ static void compute_ls_vectors(orc_state *s)
 /* ------------------------------------------------------------------------- create */
 orc_state *orc_create(const orc_case *cs)
 {
-    if (!cs || cs->nC <= 0 || cs->nIF < 0 || cs->nB < 0 || cs->D < 1 || cs->D > 3 || cs->n1d <= 0) {
+    if (!cs || cs->nC <= 0 || cs->nIF < 0 || cs->nB < 0 || cs->D < 1 || cs->D > 3 ||
+        (cs->n_list > 0 ? !cs->list_xi || !cs->list_w : cs->n1d <= 0)) {
         set_err("orc_create: invalid sizes");
+        return NULL;
+    }
+    if (cs->macro_flux < 0 || cs->macro_flux > 2) {
+        set_err("orc_create: macro_flux must be 0, 1 or 2");
         return NULL;
     }
     orc_state *s = calloc(1, sizeof *s);
@@ -211,8 +216,8 @@ orc_state *orc_create(const orc_case *cs)
     s->bf_rho = dup(cs->bf_rho, sizeof(double) * nB);
     s->bf_U = dup(cs->bf_U, sizeof(double) * 3 * nB);
     s->bf_T = dup(cs->bf_T, sizeof(double) * nB);
-    s->xis = dup(cs->xis, sizeof(double) * cs->n1d);
-    s->wts = dup(cs->wts, sizeof(double) * cs->n1d);
+    s->xis = cs->n_list > 0 ? NULL : dup(cs->xis, sizeof(double) * cs->n1d);
+    s->wts = cs->n_list > 0 ? NULL : dup(cs->wts, sizeof(double) * cs->n1d);
 
     s->bf_type = calloc((size_t)nB + 1, sizeof(int));
     s->bf_local = calloc((size_t)nB + 1, sizeof(int));
@@ -236,8 +241,8 @@ orc_state *orc_create(const orc_case *cs)
             orc_destroy(s);
             return NULL;
         }
-        if (cs->q1_compat && s->psize[p] > nIF) {
-            set_err("orc_create: patch larger than the internal-face count (Q1 read)");
+        if ((cs->q1_compat || cs->macro_flux == 1) && s->psize[p] > nIF) {
+            set_err("orc_create: patch larger than the internal-face count (Q1 read, macro-flux Sf read)");
             orc_destroy(s);
             return NULL;
         }
@@ -273,14 +278,27 @@ orc_state *orc_create(const orc_case *cs)
             return NULL;
         }
 
-    /* tensor-product velocity set, ix fastest (F:293-373) */
+    /* tensor-product velocity set, ix fastest (F:293-373), or the explicit list of the
+     * VMESH build: xi = cell centre with the components beyond D set to 0, weight = cell
+     * volume (F:170-249) */
     const int n = cs->n1d, D = cs->D;
-    int Nv = (D == 3) ? n * n * n : (D == 2 ? n * n : n);
+    int Nv = cs->n_list > 0 ? cs->n_list : ((D == 3) ? n * n * n : (D == 2 ? n * n : n));
     s->Nv = Nv;
     s->xi = calloc((size_t)Nv * 3, sizeof(double));
     s->w = calloc((size_t)Nv, sizeof(double));
     int i = 0;
-    if (D == 3) {
+    if (cs->n_list > 0) {
+        if (any_sym) {
+            set_err("orc_create: symmetry patches need a tensor-product velocity set "
+                    "(the VMESH build sets no mirror map)");
+            orc_destroy(s);
+            return NULL;
+        }
+        for (int v = 0; v < Nv; ++v) {
+            s->w[v] = cs->list_w[v];
+            for (int k = 0; k < D; ++k) s->xi[3 * v + k] = cs->list_xi[3 * v + k];
+        }
+    } else if (D == 3) {
         for (int iz = 0; iz < n; ++iz)
             for (int iy = 0; iy < n; ++iy)
                 for (int ix = 0; ix < n; ++ix, ++i) {
@@ -302,7 +320,7 @@ orc_state *orc_create(const orc_case *cs)
 
     /* mirror velocities of a symmetric 1-D set (x[n-1-i] = -x[i], equal weights) */
     s->mirror = calloc(3 * (size_t)Nv, sizeof(int));
-    if (any_sym) {
+    if (any_sym && cs->n_list <= 0) {
         for (int k = 0; k < n; ++k)
             if (fabs(s->xis[n - 1 - k] + s->xis[k]) > 1e-12 * fabs(s->xis[0]) || s->wts[n - 1 - k] != s->wts[k]) {
                 set_err("orc_create: symmetry patches need a velocity set symmetric about 0");
@@ -310,7 +328,7 @@ orc_state *orc_create(const orc_case *cs)
                 return NULL;
             }
     }
-    for (int v = 0; v < Nv; ++v) {
+    for (int v = 0; v < Nv && cs->n_list <= 0; ++v) {
         const int ix = v % n, iy = (v / n) % n, iz = v / (n * n);
         s->mirror[v] = (n - 1 - ix) + n * (iy + n * iz);
         s->mirror[Nv + v] = D >= 2 ? ix + n * ((n - 1 - iy) + n * iz) : v;
@@ -754,22 +772,105 @@ static void update_gh_tilde_vol(orc_state *s, double dt)
     }
 }
 
-/* updateMacroVol (F:1521-1554, 1682-1743), macroFlux no */
+/* Conserved variables rho, rhoU, rhoE [nC][5] after the face fluxes of updateMacroVol's
+ * macroFlux branch (F:1556-1679), starting from the previous step's cell macros (F:1523-1525).
+ *  form 1 (default build, F:1620-1679): fluxes of the face macros.  The face stress is zero
+ *    (stressSurf_ is reset and never accumulated, F:1187-1192, 1304-1306), the energy flux
+ *    carries rho_f (|U_f|^2 + (K+3)/2 R T_f) U_f + q_f (no 1/2 on |U_f|^2, F:1630), and the
+ *    boundary momentum flux is dotted with Sf[pFacei] of the internal-face field, i.e. the
+ *    area vector of internal face number <patch-local index> (F:1562, 1667-1668).
+ *  form 2 (VMESH build, F:1562-1618): fluxes of the moments 1, xi, (|xi|^2 g + h)/2 of the
+ *    relaxed face DFs, summed into separate flux fields and added at the end (F:1615-1617). */
+static void macro_flux_conserved(const orc_state *s, double dt, double *cons)
+{
+    const orc_case *cs = &s->cs;
+    const int nC = cs->nC, nIF = cs->nIF, nF = s->nF, Nv = s->Nv;
+    const double cv = (cs->K + 3) / 2.0 * cs->R;
+    for (int c = 0; c < nC; ++c) {
+        const double rho = s->rho[c], *U = s->U + 3 * c;
+        cons[5 * c] = rho;
+        for (int k = 0; k < 3; ++k) cons[5 * c + 1 + k] = rho * U[k];
+        cons[5 * c + 4] = rho * (0.5 * magsqr3(U) + cv * s->T[c]);
+    }
+    if (cs->macro_flux == 1) {
+        for (int f = 0; f < nF; ++f) {
+            const int own = s->owner[f];
+            const double *Sf = s->Sf + 3 * f, *U = s->US + 3 * f, *q = s->qS + 3 * f;
+            const double rho = s->rhoS[f];
+            /* momentum: (rho U U + 0) . S with S the internal-face field at the patch-local index */
+            const double *Sm = f < nIF ? Sf : s->Sf + 3 * s->bf_local[f - nIF];
+            const double mass = rho * dot3(U, Sf);
+            const double un = dot3(U, Sm);
+            const double rhoE = rho * (magsqr3(U) + cv * s->TS[f]);
+            const double energy = rhoE * dot3(U, Sf) + dot3(q, Sf);
+            double F[5] = {mass, rho * U[0] * un, rho * U[1] * un, rho * U[2] * un, energy};
+            for (int k = 0; k < 5; ++k) cons[5 * own + k] -= F[k] * dt / s->V[own];
+            if (f < nIF) {
+                const int nei = s->neighbour[f];
+                for (int k = 0; k < 5; ++k) cons[5 * nei + k] += F[k] * dt / s->V[nei];
+            }
+        }
+        return;
+    }
+    double *flux = calloc(5 * (size_t)nC, sizeof(double));
+    double *F = malloc(sizeof(double) * 5 * (size_t)nF);
+#pragma omp parallel for schedule(static)
+    for (int f = 0; f < nF; ++f) {
+        double a[5] = {0, 0, 0, 0, 0};
+        for (int v = 0; v < Nv; ++v) {
+            const double *xi = s->xi + 3 * v;
+            const double g = s->gS[(size_t)v * nF + f], h = s->hS[(size_t)v * nF + f];
+            const double xs = dot3(xi, s->Sf + 3 * f);
+            const double wg = s->w[v] * g;
+            a[0] += xs * wg;
+            for (int k = 0; k < 3; ++k) a[1 + k] += xs * wg * xi[k];
+            a[4] += xs * 0.5 * s->w[v] * (g * magsqr3(xi) + h);
+        }
+        for (int k = 0; k < 5; ++k) F[5 * (size_t)f + k] = a[k];
+    }
+    for (int f = 0; f < nF; ++f) {
+        const int own = s->owner[f];
+        for (int k = 0; k < 5; ++k) flux[5 * own + k] -= F[5 * (size_t)f + k] * dt / s->V[own];
+        if (f < nIF) {
+            const int nei = s->neighbour[f];
+            for (int k = 0; k < 5; ++k) flux[5 * nei + k] += F[5 * (size_t)f + k] * dt / s->V[nei];
+        }
+    }
+    for (size_t k = 0; k < 5 * (size_t)nC; ++k) cons[k] += flux[k];
+    free(F);
+    free(flux);
+}
+
+/* updateMacroVol (F:1521-1743): rho, rhoU, rhoE from the moments of g~, h~ (macroFlux no,
+ * F:1528-1554) or from the face fluxes (macroFlux yes, macro_flux_conserved); U, T from them
+ * (F:1682-1684); tau; q from the moments of g~, h~ about U with the 2 tau / (2 tau + dt Pr)
+ * correction (F:1700-1743). */
 static void update_macro_vol(orc_state *s, double dt)
 {
     const orc_case *cs = &s->cs;
     const int nC = cs->nC, Nv = s->Nv;
     const double R = cs->R, Pr = cs->Pr;
+    double *cons = NULL;
+    if (cs->macro_flux) {
+        cons = malloc(sizeof(double) * 5 * (size_t)nC);
+        macro_flux_conserved(s, dt, cons);
+    }
 #pragma omp parallel for schedule(static)
     for (int c = 0; c < nC; ++c) {
         double rho = 0.0, rhoU[3] = {0, 0, 0}, rhoE = 0.0;
-        for (int v = 0; v < Nv; ++v) {
-            const double *xi = s->xi + 3 * v;
-            double g = s->gT[(size_t)v * nC + c], h = s->hT[(size_t)v * nC + c];
-            rho += s->w[v] * g;
-            double wg = s->w[v] * g;
-            rhoU[0] += wg * xi[0]; rhoU[1] += wg * xi[1]; rhoU[2] += wg * xi[2];
-            rhoE += 0.5 * s->w[v] * (magsqr3(xi) * g + h);
+        if (cons) {
+            rho = cons[5 * c];
+            for (int k = 0; k < 3; ++k) rhoU[k] = cons[5 * c + 1 + k];
+            rhoE = cons[5 * c + 4];
+        } else {
+            for (int v = 0; v < Nv; ++v) {
+                const double *xi = s->xi + 3 * v;
+                double g = s->gT[(size_t)v * nC + c], h = s->hT[(size_t)v * nC + c];
+                rho += s->w[v] * g;
+                double wg = s->w[v] * g;
+                rhoU[0] += wg * xi[0]; rhoU[1] += wg * xi[1]; rhoU[2] += wg * xi[2];
+                rhoE += 0.5 * s->w[v] * (magsqr3(xi) * g + h);
+            }
         }
         double U[3] = {rhoU[0] / rho, rhoU[1] / rho, rhoU[2] / rho};
         double T = (rhoE - 0.5 * rho * magsqr3(U)) / ((cs->K + 3) / 2.0 * R * rho);
@@ -791,6 +892,7 @@ static void update_macro_vol(orc_state *s, double dt)
             s->q[3 * c + k] = corr * q[k];
         }
     }
+    free(cons);
 }
 
 /* fvDVM::evolution (F:2128-2174).  GHbarPvolComm only exchanges processor halos and

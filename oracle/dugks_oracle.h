@@ -49,6 +49,16 @@ typedef struct orc_case {
     int q1_compat;           /* replicate the Tsurf internal-field read of fvDVM.C:1413 */
     const double *patch_pressure; /* [n_patches] pressureIn / pressureOut values (may be NULL) */
     const double *ls_own, *ls_nei; /* [nIF][3] gradient vectors (NULL: least squares) */
+    /* n_list > 0: the velocity set is this list instead of the tensor product of xis
+     * (the VMESH build: cell centres and volumes of constant/vMesh, fvDVM.C:129-253);
+     * components beyond D are dropped as there.  Symmetry patches need the tensor set. */
+    int n_list;
+    const double *list_xi;   /* [n_list][3] */
+    const double *list_w;    /* [n_list] */
+    /* updateMacroVol (fvDVM.C:1521-1682): 0 macroFlux no (moments of g~, h~); 1 macroFlux yes
+     * in the default build (fluxes of the face macros, F:1620-1679); 2 macroFlux yes in the
+     * VMESH build (fluxes of the velocity moments of the relaxed face DFs, F:1562-1618) */
+    int macro_flux;
 } orc_case;
 
 typedef struct orc_state orc_state;

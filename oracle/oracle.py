@@ -30,6 +30,7 @@ class OrcCase(ct.Structure):
         ("R", ct.c_double), ("omega", ct.c_double), ("Tref", ct.c_double), ("muRef", ct.c_double),
         ("Pr", ct.c_double), ("K", ct.c_int), ("q1_compat", ct.c_int), ("patch_pressure", _D),
         ("ls_own", _D), ("ls_nei", _D),
+        ("n_list", ct.c_int), ("list_xi", _D), ("list_w", _D), ("macro_flux", ct.c_int),
     ]
 
 
@@ -90,7 +91,9 @@ class Oracle:
             R=a["R"], omega=a["omega"], Tref=a["Tref"], muRef=a["muRef"], Pr=a["Pr"],
             K=a["K"], q1_compat=a["q1_compat"], patch_pressure=_p(a["patch_pressure"]),
             ls_own=_p(a["ls_own"]) if a.get("ls_own") is not None else None,
-            ls_nei=_p(a["ls_nei"]) if a.get("ls_nei") is not None else None)
+            ls_nei=_p(a["ls_nei"]) if a.get("ls_nei") is not None else None,
+            n_list=a["n_list"], list_xi=_p(a["list_xi"]) if a["n_list"] else None,
+            list_w=_p(a["list_w"]) if a["n_list"] else None, macro_flux=a["macro_flux"])
         self._h = L.orc_create(ct.byref(cs))
         if not self._h:
             raise RuntimeError(L.orc_last_error().decode())

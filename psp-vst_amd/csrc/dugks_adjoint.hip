@@ -756,7 +756,7 @@ int run_adjoint(const Geo &g, DevState &d, hipStream_t st, const AdjointIO &io, 
         Staged<S>::face_moments(g, gS, hS, fm, dt, d.cfl_part, st);
         Staged<S>::face_relax(g, fm, gS, hS, dt, st);
         Staged<S>::update(g, gB, hB, gS, hS, gt, ht, dt, st);
-        Staged<S>::cell_moments(g, gt, ht, cm, dt, st);
+        Staged<S>::cell_moments(g, gt, ht, cm, fm, dt, st);
     };
     auto copy_state = [&](S *g2, S *h2, const Macros &m2, const S *g1, const S *h1, const Macros &m1) {
         return hipMemcpyAsync(g2, g1, ncell * sizeof(S), hipMemcpyDeviceToDevice, st) == hipSuccess &&

@@ -174,7 +174,8 @@ int staged_step(dugks_handle *h)
     PHASE(kPhFaceMom, Staged<S>::face_moments(g, gS, hS, d.face, d.dt, d.cfl_part, st));
     PHASE(kPhFaceRelax, Staged<S>::face_relax(g, d.face, gS, hS, d.dt, st));
     PHASE(kPhUpdate, Staged<S>::update(g, gB, hB, gS, hS, gT, hT, d.dt, st));
-    PHASE(kPhCellMom, Staged<S>::cell_moments(g, gT, hT, d.cell, d.dt, st));
+    if (g.macro_flux == 2) PHASE(kPhCellMom, Staged<S>::face_flux(g, gS, hS, st));
+    PHASE(kPhCellMom, Staged<S>::cell_moments(g, gT, hT, d.cell, d.face, d.dt, st));
     if (h->has_pressure) launch_pressure_bc(g, d.cell, d.rho_b, d.bf_T, st);  // fvDVM.C:2172
     HIPCHK(hipGetLastError());
     return 0;
@@ -247,10 +248,12 @@ int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patch
     *out = nullptr;
     if (!mesh || !vset || !gas) return fail("dugks_create: mesh, vset and gas are required");
     const int nC = mesh->n_cells, nIF = mesh->n_internal_faces, nB = mesh->n_boundary_faces;
-    const int D = mesh->n_solution_d, n1 = vset->n_per_dim;
+    const bool vlist = vset->n_list > 0;  // velocity mesh of the VMESH build
+    const int D = mesh->n_solution_d, n1 = vlist ? 0 : vset->n_per_dim;
     if (nC <= 0 || nIF < 0 || nB < 0) return fail("dugks_create: invalid mesh sizes");
     if (D < 1 || D > 3) return fail("dugks_create: n_solution_d must be 1, 2 or 3");
-    if (n1 <= 0 || !vset->xis || !vset->weights) return fail("dugks_create: empty velocity set");
+    if (vlist ? (!vset->list_xi || !vset->list_w) : (n1 <= 0 || !vset->xis || !vset->weights))
+        return fail("dugks_create: empty velocity set");
     if (!mesh->owner || (nIF > 0 && !mesh->neighbour) || !mesh->Sf || !mesh->Cf || !mesh->C || !mesh->V ||
         (nIF > 0 && (!mesh->weights || !mesh->delta_coeffs)))
         return fail("dugks_create: missing mesh arrays");
@@ -264,10 +267,18 @@ int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patch
     if (opt) o = *opt;
     if (o.precision != DUGKS_FP64 && o.precision != DUGKS_MIXED) return fail("dugks_create: unknown precision");
     if (o.fused < -1 || o.fused > 3) return fail("dugks_create: options.fused must be -1, 0, 1, 2 or 3");
+    if (o.macro_flux < 0 || o.macro_flux > 2) return fail("dugks_create: options.macro_flux must be 0, 1 or 2");
+    // velocity lists and the macroFlux update run on the staged path
+    if (vlist || o.macro_flux != 0) {
+        if (o.fused >= 1)
+            return fail("dugks_create: the fused path needs a tensor-product velocity set and macro_flux = 0; "
+                        "use fused = 0 or -1 for this case");
+        o.fused = 0;
+    }
 
     const int nF = nIF + nB;
-    long long nv_ll = 1;
-    for (int k = 0; k < D; ++k) nv_ll *= n1;
+    long long nv_ll = vlist ? vset->n_list : 1;
+    for (int k = 0; k < D && !vlist; ++k) nv_ll *= n1;
     if (nv_ll > 65535) return fail("dugks_create: more than 65535 discrete velocities");
     const int Nv = (int)nv_ll;
 
@@ -311,6 +322,9 @@ int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patch
         if (o.q1_compat && pp.size > nIF)
             return fail("dugks_create: q1_compat reads the internal-face temperature at patch-local indices; patch " +
                         std::to_string(p) + " is larger than the internal-face count");
+        if (o.macro_flux == 1 && pp.size > nIF)
+            return fail("dugks_create: macro_flux 1 reads internal-face area vectors at patch-local indices; patch " +
+                        std::to_string(p) + " is larger than the internal-face count");
         for (int i = 0; i < pp.size; ++i) {
             const int b = pp.start + i;
             if (bf_type[b] != -1) return fail("dugks_create: boundary face " + std::to_string(b) + " in two patches");
@@ -324,6 +338,9 @@ int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patch
     }
     for (int b = 0; b < nB; ++b)
         if (bf_type[b] == -1) return fail("dugks_create: boundary face " + std::to_string(b) + " is in no patch");
+    if (any_sym && vlist)
+        return fail("dugks_create: symmetry patches need a tensor-product velocity set (the velocity mesh of "
+                    "the VMESH build has no mirror map)");
     if (any_sym)
         for (int k = 0; k < n1; ++k)
             if (std::fabs(vset->xis[n1 - 1 - k] + vset->xis[k]) > 1e-12 * std::fabs(vset->xis[0]) ||
@@ -394,13 +411,23 @@ int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patch
     prep.owner_h = own_p;
     prep.neighbour_h = nei_p;
 
-    // tensor-product velocity set, ix fastest (fvDVM.C:293-373)
+    // tensor-product velocity set, ix fastest (fvDVM.C:293-373), or the velocity mesh's cell
+    // centres (components beyond D dropped) and volumes (fvDVM.C:170-249)
     std::vector<d3> xi(Nv);
     std::vector<double> w(Nv);
     {
         const double *X = vset->xis, *W = vset->weights;
         int i = 0;
-        if (D == 3) {
+        if (vlist) {
+            for (int v = 0; v < Nv; ++v) {
+                const double *L = vset->list_xi + 3 * (size_t)v;
+                xi[v] = d3{L[0], D >= 2 ? L[1] : 0.0, D == 3 ? L[2] : 0.0};
+                w[v] = vset->list_w[v];
+                if (!(w[v] > 0) || !std::isfinite(L[0] + L[1] + L[2]))
+                    return fail("dugks_create: velocity " + std::to_string(v) + " of the list has a non-positive "
+                                "weight or a non-finite component");
+            }
+        } else if (D == 3) {
             for (int iz = 0; iz < n1; ++iz)
                 for (int iy = 0; iy < n1; ++iy)
                     for (int ix = 0; ix < n1; ++ix, ++i) {
@@ -454,7 +481,8 @@ int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patch
     {
         // velocity mirrored about each axis (ix fastest: v = ix + n (iy + n iz))
         std::vector<int> mir(3 * (size_t)Nv);
-        for (int v = 0; v < Nv; ++v) {
+        for (int v = 0; v < Nv && vlist; ++v) mir[v] = mir[Nv + v] = mir[2 * (size_t)Nv + v] = v;
+        for (int v = 0; v < Nv && !vlist; ++v) {
             const int ix = v % n1, iy = (v / n1) % n1, iz = v / (n1 * n1);
             mir[v] = (n1 - 1 - ix) + n1 * (iy + n1 * iz);
             mir[Nv + v] = D >= 2 ? ix + n1 * ((n1 - 1 - iy) + n1 * iz) : v;
@@ -518,6 +546,9 @@ int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patch
     g.in_by_rho = d.in_by_rho; g.delta_coeffs = d.dc; g.weights = d.weights; g.xi = d.xi; g.w = d.w;
     g.ref_g = d.ref_g; g.ref_h = d.ref_h;
     g.bf_p = d.bf_p; g.bf_axis = d.bf_axis; g.bf_sf0 = d.bf_sf0; g.mirror = d.mirror;
+    g.macro_flux = o.macro_flux;
+    if (o.macro_flux == 2 && dalloc(h, &d.face_flux, 5 * (size_t)nF)) return bail(1);
+    g.face_flux = d.face_flux;
 
     // distributions
     const size_t ncell = (size_t)Nv * nC, nface = (size_t)Nv * nF;
@@ -970,6 +1001,7 @@ int dugks_adjoint(dugks_handle *h, int n_steps, const double *dts, int checkpoin
     if (checkpoint_every < 1) return fail("dugks_adjoint: checkpoint_every must be >= 1");
     if (h->has_far_field || h->has_symmetry)
         return fail("dugks_adjoint: the reverse kernels carry diffuse-wall and fixedValue patches only");
+    if (h->g.macro_flux != 0) return fail("dugks_adjoint: the reverse kernels carry macro_flux = 0 only");
     {
         std::vector<int> bt(h->g.nB);
         if (h->g.nB > 0)

@@ -84,6 +84,10 @@ struct Geo {
     // (rho = 1, U = 0, T = Tref; h = (K+3-D) R Tref g): value = stored + ref.  FP32 storage
     // then rounds the deviation, not the full value.  Zero for FP64 storage.
     const double *ref_g, *ref_h;  // [Nv]
+    // updateMacroVol: 0 moments of g~, h~; 1 face-macro fluxes; 2 velocity-moment fluxes of the
+    // relaxed face DFs, held per face in face_flux [nF][5] (rho, rhoU, rhoE)
+    int macro_flux;
+    double *face_flux;
 };
 
 struct Macros {
@@ -104,6 +108,7 @@ struct DevState {
            *weights = nullptr, *ref_g = nullptr, *ref_h = nullptr, *bf_p = nullptr;
     int *bf_axis = nullptr, *mirror = nullptr;
     d3 *bf_sf0 = nullptr;
+    double *face_flux = nullptr;  // [nF][5] (macro_flux 2)
     // distributions (storage type chosen at run time; void* + element size)
     void *gT = nullptr, *hT = nullptr, *gB = nullptr, *hB = nullptr, *gS = nullptr, *hS = nullptr;
     // macros
@@ -161,7 +166,11 @@ struct Staged {
     static void face_relax(const Geo &g, const Macros &fm, S *gS, S *hS, const double *dt, hipStream_t st);
     static void update(const Geo &g, const S *gB, const S *hB, const S *gS, const S *hS, S *gT, S *hT,
                        const double *dt, hipStream_t st);
-    static void cell_moments(const Geo &g, const S *gT, const S *hT, Macros cm, const double *dt, hipStream_t st);
+    // fluxes of the moments 1, xi, (|xi|^2 g + h)/2 of the relaxed face DFs (macro_flux 2)
+    static void face_flux(const Geo &g, const S *gS, const S *hS, hipStream_t st);
+    // cell macros; macro_flux 1 reads the face macros fm, 2 the face fluxes of face_flux
+    static void cell_moments(const Geo &g, const S *gT, const S *hT, Macros cm, const Macros &fm, const double *dt,
+                             hipStream_t st);
 };
 
 }  // namespace dugks

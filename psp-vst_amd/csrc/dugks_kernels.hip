@@ -394,26 +394,109 @@ __global__ void __launch_bounds__(kBlock)
     hT[kc] = (S)ah;
 }
 
-// updateMacroVol (fvDVM.C:1521-1554, 1682-1743)
+// Fluxes of the velocity moments 1, xi, (|xi|^2 g + h)/2 of the relaxed face DFs through
+// each face: flux[f] = sum_v w_v (xi_v . S_f) m(v) (updateMacroVol, macroFlux yes in the VMESH
+// build, fvDVM.C:1562-1612).  One thread per face, velocities in sequence: coalesced reads of
+// the [v][face] arrays.
 template <class S>
 __global__ void __launch_bounds__(kBlock)
-    k_cell_moments(Geo g, const S *__restrict__ gT, const S *__restrict__ hT, Macros cm, const double *dtp)
+    k_face_flux(Geo g, const S *__restrict__ gS, const S *__restrict__ hS, double *flux)
+{
+    const int f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= g.nF) return;
+    const d3 Sf = g.Sf[f];
+    double m0 = 0.0, mx = 0.0, my = 0.0, mz = 0.0, me = 0.0;
+    for (int v = 0; v < g.Nv; ++v) {
+        const size_t k = (size_t)v * g.nF + f;
+        const double gv = (double)gS[k] + g.ref_g[v], hv = (double)hS[k] + g.ref_h[v];
+        const d3 xi = g.xi[v];
+        const double xs = sigma_dot(xi, Sf);
+        const double wg = g.w[v] * gv;
+        m0 += xs * wg;
+        mx += xs * wg * xi.x;
+        my += xs * wg * xi.y;
+        mz += xs * wg * xi.z;
+        me += xs * 0.5 * g.w[v] * (gv * magsqr(xi) + hv);
+    }
+    double *o = flux + 5 * (size_t)f;
+    o[0] = m0;
+    o[1] = mx;
+    o[2] = my;
+    o[3] = mz;
+    o[4] = me;
+}
+
+// updateMacroVol (fvDVM.C:1521-1743).  rho, rhoU, rhoE: moments of g~, h~ (macroFlux no,
+// F:1528-1554), or the previous conserved variables minus the face fluxes (macroFlux yes):
+//   macro_flux 1 (F:1620-1679): fluxes of the face macros with zero face stress (stressSurf_ is
+//     never accumulated), rho_f (|U_f|^2 + (K+3)/2 R T_f) U_f + q_f as the energy flux, and the
+//     boundary momentum flux dotted with the internal-face area vector at the patch-local
+//     index (Sf there is the internal field), subtracted face by face in ascending order;
+//   macro_flux 2 (F:1562-1618): the velocity-moment fluxes of k_face_flux, summed apart and
+//     added at the end.
+// Then U, T (F:1682-1684), tau, and q from the moments of g~, h~ about U with the
+// 2 tau / (2 tau + dt Pr) correction (F:1700-1743).
+template <class S>
+__global__ void __launch_bounds__(kBlock)
+    k_cell_moments(Geo g, const S *__restrict__ gT, const S *__restrict__ hT, Macros cm, Macros fm,
+                   const double *dtp)
 {
     const int c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= g.nC) return;
     const double dt = *dtp;
     double rho = 0.0, rx = 0.0, ry = 0.0, rz = 0.0, rhoE = 0.0;
-    for (int v = 0; v < g.Nv; ++v) {
-        const size_t k = (size_t)v * g.nC + c;
-        const double gv = (double)gT[k] + g.ref_g[v], hv = (double)hT[k] + g.ref_h[v];
-        const d3 xi = g.xi[v];
-        const double w = g.w[v];
-        rho += w * gv;
-        const double wg = w * gv;
-        rx += wg * xi.x;
-        ry += wg * xi.y;
-        rz += wg * xi.z;
-        rhoE += 0.5 * w * (magsqr(xi) * gv + hv);
+    if (g.macro_flux == 0) {
+        for (int v = 0; v < g.Nv; ++v) {
+            const size_t k = (size_t)v * g.nC + c;
+            const double gv = (double)gT[k] + g.ref_g[v], hv = (double)hT[k] + g.ref_h[v];
+            const d3 xi = g.xi[v];
+            const double w = g.w[v];
+            rho += w * gv;
+            const double wg = w * gv;
+            rx += wg * xi.x;
+            ry += wg * xi.y;
+            rz += wg * xi.z;
+            rhoE += 0.5 * w * (magsqr(xi) * gv + hv);
+        }
+    } else {
+        const double cv = (g.K + 3) / 2.0 * g.R;
+        const double V = g.V[c];
+        rho = cm.rho[c];
+        const d3 U0 = cm.U[c];
+        rx = rho * U0.x;
+        ry = rho * U0.y;
+        rz = rho * U0.z;
+        rhoE = rho * (0.5 * magsqr(U0) + cv * cm.T[c]);
+        double s0 = 0.0, sx = 0.0, sy = 0.0, sz = 0.0, se = 0.0;  // macro_flux 2: summed apart
+        for (int k = g.cf_off[c]; k < g.cf_off[c + 1]; ++k) {
+            const int f = g.cf_face[k];
+            const double sgn = (f < g.nIF && g.owner[f] != c) ? 1.0 : -1.0;
+            if (g.macro_flux == 1) {
+                const d3 Sf = g.Sf[f];
+                const d3 Sm = f < g.nIF ? Sf : g.Sf[g.bf_local[f - g.nIF]];
+                const double rf = fm.rho[f];
+                const d3 Uf = fm.U[f], qf = fm.q[f];
+                const double un = dot(Uf, Sm);
+                const double eU = rf * (magsqr(Uf) + cv * fm.T[f]);
+                rho += sgn * (rf * dot(Uf, Sf) * dt / V);
+                rx += sgn * (rf * Uf.x * un * dt / V);
+                ry += sgn * (rf * Uf.y * un * dt / V);
+                rz += sgn * (rf * Uf.z * un * dt / V);
+                rhoE += sgn * ((eU * dot(Uf, Sf) + dot(qf, Sf)) * dt / V);
+            } else {
+                const double *F = g.face_flux + 5 * (size_t)f;
+                s0 += sgn * (F[0] * dt / V);
+                sx += sgn * (F[1] * dt / V);
+                sy += sgn * (F[2] * dt / V);
+                sz += sgn * (F[3] * dt / V);
+                se += sgn * (F[4] * dt / V);
+            }
+        }
+        rho += s0;
+        rx += sx;
+        ry += sy;
+        rz += sz;
+        rhoE += se;
     }
     const d3 U{rx / rho, ry / rho, rz / rho};
     const double T = (rhoE - 0.5 * rho * magsqr(U)) / ((g.K + 3) / 2.0 * g.R * rho);
@@ -617,9 +700,15 @@ void Staged<S>::update(const Geo &g, const S *gB, const S *hB, const S *gS, cons
     hipLaunchKernelGGL(k_update<S>, grid2(g.nC, g.Nv), dim3(kBlock), 0, st, g, gB, hB, gS, hS, gT, hT, dt);
 }
 template <class S>
-void Staged<S>::cell_moments(const Geo &g, const S *gT, const S *hT, Macros cm, const double *dt, hipStream_t st)
+void Staged<S>::face_flux(const Geo &g, const S *gS, const S *hS, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_cell_moments<S>, grid1(g.nC), dim3(kBlock), 0, st, g, gT, hT, cm, dt);
+    hipLaunchKernelGGL(k_face_flux<S>, grid1(g.nF), dim3(kBlock), 0, st, g, gS, hS, g.face_flux);
+}
+template <class S>
+void Staged<S>::cell_moments(const Geo &g, const S *gT, const S *hT, Macros cm, const Macros &fm, const double *dt,
+                             hipStream_t st)
+{
+    hipLaunchKernelGGL(k_cell_moments<S>, grid1(g.nC), dim3(kBlock), 0, st, g, gT, hT, cm, fm, dt);
 }
 
 template void launch_init_df<float>(const Geo &, const Macros &, float *, float *, hipStream_t);

@@ -86,6 +86,12 @@ class Case:
     meta: dict = field(default_factory=dict)
     # gradient scheme of system/fvSchemes: "leastSquares" (dugksFoam's demos) or "Gauss linear"
     grad_scheme: str = "leastSquares"
+    # velocity list of the VMESH build (cell centres [n, 3] and volumes [n] of constant/vMesh,
+    # src/fvDVM/fvDVM/fvDVM.C:129-253); None: the tensor product of xis / wts
+    vlist: tuple | None = None
+    # updateMacroVol (fvDVM.C:1521-1682): 0 macroFlux no, 1 macroFlux yes (face-macro fluxes),
+    # 2 macroFlux yes in the VMESH build (velocity-moment fluxes of the face DFs)
+    macro_flux: int = 0
 
     @property
     def n_cells(self) -> int:
@@ -93,6 +99,8 @@ class Case:
 
     @property
     def nv(self) -> int:
+        if self.vlist is not None:
+            return len(self.vlist[1])
         return len(self.xis) ** self.D
 
     def arrays(self) -> dict:
@@ -136,6 +144,10 @@ class Case:
             R=self.gas.R, omega=self.gas.omega, Tref=self.gas.Tref, muRef=self.gas.muRef,
             Pr=self.gas.Pr, K=int(self.gas.KInner), q1_compat=int(self.q1_compat),
             rho0=f64(self.rho0), U0=f64(self.U0), T0=f64(self.T0), ls_own=None, ls_nei=None,
+            n_list=0 if self.vlist is None else len(self.vlist[1]),
+            list_xi=None if self.vlist is None else f64(np.reshape(self.vlist[0], (-1, 3))),
+            list_w=None if self.vlist is None else f64(self.vlist[1]),
+            macro_flux=int(self.macro_flux),
         )
         out.update(self._gradient_vectors())
         return out
@@ -222,17 +234,18 @@ def cavity_case(N: int, rule: str, n_dv: int, vel_extent: float, Kn: float,
 
 def box_case(nx: int, ny: int, sides: dict, rule: str = "GH", n_dv: int = 8, vel_extent: float = 1.0,
              Kn: float = 0.1, extent=(0.0, 1.0, 0.0, 1.0), rho0=1.0, U0=(0.0, 0.0, 0.0), T0=1.0,
-             controls: Controls | None = None, name: str | None = None) -> Case:
+             controls: Controls | None = None, name: str | None = None, grading: float = 1.0) -> Case:
     """A 2-D box (nondimensional, R = Tref = 1) with one boundary condition per side.
 
     ``sides`` maps ``left``, ``right``, ``bottom``, ``top`` to ``(rho_type, rho, U, T)`` or
     ``(rho_type, rho, U, T, pressure)``; rho, U, T may be scalars / 3-vectors or per-face
     arrays.  ``rho0``, ``U0``, ``T0`` are the initial cell fields (constants or callables of
-    the cell centres).  ``symmetryPlane`` sides get that geometric type, walls ``wall``."""
+    the cell centres).  ``symmetryPlane`` sides get that geometric type, walls ``wall``.
+    ``grading`` > 1 grows the cell widths geometrically along x and y (mesh.box2d)."""
     gtypes = {k: ("symmetryPlane" if v[0] == "symmetryPlane" else "wall" if v[0] == "calculatedMaxwell" else "patch")
               for k, v in sides.items()}
     x0, x1, y0, y1 = extent
-    m = meshmod.box2d(nx, ny, x0, x1, y0, y1, types=gtypes)
+    m = meshmod.box2d(nx, ny, x0, x1, y0, y1, types=gtypes, grading=grading)
     geo = meshmod.geometry(m)
     gas = Gas(R=1.0, omega=0.81, Tref=1.0, muRef=mu_ref_from_kn(Kn, omega=0.81), Pr=2.0 / 3.0, KInner=0)
     scale = vel_extent * math.sqrt((2.0 if rule.startswith("GH") else 1.0))

@@ -51,6 +51,7 @@ def load_library(path: str = LIB_PATH):
     L = ct.CDLL(path)
     vp = ct.c_void_p
     L.dugks_foam_read.argtypes = [ct.c_char_p, ct.c_char_p, ct.POINTER(vp)]
+    L.dugks_foam_read_vmesh.argtypes = [ct.c_char_p, ct.c_char_p, ct.POINTER(vp)]
     L.dugks_foam_free.argtypes = [vp]
     L.dugks_foam_free.restype = None
     L.dugks_foam_last_error.restype = ct.c_char_p
@@ -73,10 +74,11 @@ class FoamError(RuntimeError):
 
 
 class _Handle:
-    def __init__(self, case_dir: str, time_name: str | None):
+    def __init__(self, case_dir: str, time_name: str | None, vmesh: bool = False):
         self.L = load_library()
         self.h = ct.c_void_p()
-        if self.L.dugks_foam_read(case_dir.encode(), (time_name or "").encode(), ct.byref(self.h)) != 0:
+        read = self.L.dugks_foam_read_vmesh if vmesh else self.L.dugks_foam_read
+        if read(case_dir.encode(), (time_name or "").encode(), ct.byref(self.h)) != 0:
             raise FoamError(self.L.dugks_foam_last_error().decode())
 
     def close(self):
@@ -117,9 +119,11 @@ class _Handle:
         return c
 
 
-def read_case(case_dir: str, time_name: str | None = None) -> casesmod.Case:
-    """The case in case_dir as a Case (nondimensional or SI values as the files hold them)."""
-    h = _Handle(case_dir, time_name)
+def read_case(case_dir: str, time_name: str | None = None, vmesh: bool = False) -> casesmod.Case:
+    """The case in case_dir as a Case (nondimensional or SI values as the files hold them).
+    vmesh: read the velocity set from constant/vMesh as dugksFoam built with -DVMESH does
+    (src/fvDVM/fvDVM/fvDVM.C:129-253)."""
+    h = _Handle(case_dir, time_name, vmesh)
     try:
         nC = int(h.s("n_cells"))
         nIF = int(h.s("n_internal_faces"))
@@ -153,7 +157,9 @@ def read_case(case_dir: str, time_name: str | None = None) -> casesmod.Case:
                           meta=dict(case_dir=case_dir, time=h.s("time"), end_time=ctl.end_time,
                                     converge_tol=ctl.converge_tol, converge_check_steps=ctl.converge_check_steps,
                                     df_write_cells=h.i("df_write_cells").tolist()),
-                          grad_scheme=h.L.dugks_foam_grad_scheme(h.h).decode())
+                          grad_scheme=h.L.dugks_foam_grad_scheme(h.h).decode(),
+                          vlist=(h.d("xi_list", (-1, 3)), h.d("xi_list_weights")) if vmesh else None,
+                          macro_flux=int(h.s("macro_flux")))
         assert c.n_cells == nC and pm.n_internal_faces == nIF
         return c
     finally:
@@ -185,24 +191,14 @@ def _values(vals: np.ndarray, vtype: str) -> str:
     return f"nonuniform List<{vtype}> \n{len(vals)}\n(\n{body}\n)\n"
 
 
-def write_case(case: casesmod.Case, case_dir: str, end_time: float = 1.0e300, write_interval: int = 1000000,
-               delta_t: float | None = None, compress_mesh: bool = False, converge_tol: float = -1.0,
-               converge_check_steps: int = 1000000, write_precision: int = 17, df_write_cells=()) -> str:
-    """Writes `case` as an OpenFOAM case directory that dugksFoam (and read_case) takes:
-    polyMesh, DVMProperties (values as the case holds them, with dimension sets), Xis,
-    weights, controlDict (the case's time controls; writeControl timeStep) and 0/{rho,U,T}.
-    The default convergeTol of -1 never stops the loop on convergence."""
-    m = case.mesh
-    pmdir = os.path.join(case_dir, "constant", "polyMesh")
+def _write_poly_mesh(m: meshmod.PolyMesh, pmdir: str, location: str, compress_mesh: bool) -> None:
     os.makedirs(pmdir, exist_ok=True)
-    os.makedirs(os.path.join(case_dir, "system"), exist_ok=True)
-    os.makedirs(os.path.join(case_dir, "0"), exist_ok=True)
     with _open(os.path.join(pmdir, "points"), compress_mesh) as f:
-        f.write(_header("vectorField", "constant/polyMesh", "points"))
+        f.write(_header("vectorField", location, "points"))
         f.write(f"{len(m.points)}\n(\n" + "\n".join("(" + " ".join(_num(x) for x in p) + ")" for p in m.points)
                 + "\n)\n")
     with _open(os.path.join(pmdir, "faces"), compress_mesh) as f:
-        f.write(_header("faceList", "constant/polyMesh", "faces"))
+        f.write(_header("faceList", location, "faces"))
         lines = []
         for i in range(m.n_faces):
             fp = m.face(i)
@@ -210,22 +206,40 @@ def write_case(case: casesmod.Case, case_dir: str, end_time: float = 1.0e300, wr
         f.write(f"{m.n_faces}\n(\n" + "\n".join(lines) + "\n)\n")
     for name, arr in (("owner", m.owner), ("neighbour", m.neighbour)):
         with _open(os.path.join(pmdir, name), compress_mesh) as f:
-            f.write(_header("labelList", "constant/polyMesh", name))
+            f.write(_header("labelList", location, name))
             f.write(f"{len(arr)}\n(\n" + "\n".join(str(int(x)) for x in arr) + "\n)\n")
     with open(os.path.join(pmdir, "boundary"), "w") as f:
-        f.write(_header("polyBoundaryMesh", "constant/polyMesh", "boundary"))
+        f.write(_header("polyBoundaryMesh", location, "boundary"))
         f.write(f"{len(m.patches)}\n(\n")
         for p in m.patches:
             f.write(f"    {p.name}\n    {{\n        type            {p.geometric_type};\n"
                     f"        nFaces          {p.size};\n        startFace       {p.start};\n    }}\n")
         f.write(")\n")
+
+
+def write_case(case: casesmod.Case, case_dir: str, end_time: float = 1.0e300, write_interval: int = 1000000,
+               delta_t: float | None = None, compress_mesh: bool = False, converge_tol: float = -1.0,
+               converge_check_steps: int = 1000000, write_precision: int = 17, df_write_cells=(),
+               vmesh: meshmod.PolyMesh | None = None) -> str:
+    """Writes `case` as an OpenFOAM case directory that dugksFoam (and read_case) takes:
+    polyMesh, DVMProperties (values as the case holds them, with dimension sets; macroFlux
+    yes when case.macro_flux is non-zero), Xis, weights, controlDict (the case's time
+    controls; writeControl timeStep) and 0/{rho,U,T}; with `vmesh`, also the velocity mesh
+    constant/vMesh of the VMESH build.  The default convergeTol of -1 never stops the loop
+    on convergence."""
+    m = case.mesh
+    os.makedirs(os.path.join(case_dir, "system"), exist_ok=True)
+    os.makedirs(os.path.join(case_dir, "0"), exist_ok=True)
+    _write_poly_mesh(m, os.path.join(case_dir, "constant", "polyMesh"), "constant/polyMesh", compress_mesh)
+    if vmesh is not None:
+        _write_poly_mesh(vmesh, os.path.join(case_dir, "constant", "vMesh"), "constant/vMesh", compress_mesh)
     g = case.gas
     with open(os.path.join(case_dir, "constant", "DVMProperties"), "w") as f:
         f.write(_header("dictionary", "constant", "DVMProperties"))
         f.write("fvDVMparas\n{\n"
                 f"    xiMax       xiMax [0 1 -1 0 0 0 0] {_num(case.xi_max)};\n"
                 f"    xiMin       xiMin [0 1 -1 0 0 0 0] {_num(case.xi_min)};\n"
-                f"    nDV         {len(case.xis)};\n    macroFlux   no;\n}}\n\n"
+                f"    nDV         {len(case.xis)};\n    macroFlux   {'yes' if case.macro_flux else 'no'};\n}}\n\n"
                 "gasProperties\n{\n"
                 f"    R           R [0 2 -2 -1 0 0 0] {_num(g.R)};\n    omega       {_num(g.omega)};\n"
                 f"    Tref        Tref [0 0 0 1 0 0 0] {_num(g.Tref)};\n"

@@ -39,7 +39,7 @@ class Patch(ct.Structure):
 
 class VSet(ct.Structure):
     _fields_ = [("n_per_dim", ct.c_int), ("xis", _D), ("weights", _D), ("xi_max", ct.c_double),
-                ("xi_min", ct.c_double)]
+                ("xi_min", ct.c_double), ("n_list", ct.c_int), ("list_xi", _D), ("list_w", _D)]
 
 
 class Gas(ct.Structure):
@@ -48,7 +48,8 @@ class Gas(ct.Structure):
 
 
 class Options(ct.Structure):
-    _fields_ = [("precision", ct.c_int), ("q1_compat", ct.c_int), ("device", ct.c_int), ("fused", ct.c_int)]
+    _fields_ = [("precision", ct.c_int), ("q1_compat", ct.c_int), ("device", ct.c_int), ("fused", ct.c_int),
+                ("macro_flux", ct.c_int)]
 
 
 EXPORTS = [
@@ -157,10 +158,12 @@ class FvDVM:
             patches[i] = Patch(type=int(a["patch_type"][i]), start=s, size=n, rho=_p(rho), U=_p(U), T=_p(T),
                                pressure=float(a["patch_pressure"][i]))
         vset = VSet(n_per_dim=len(a["xis"]), xis=_p(a["xis"]), weights=_p(a["wts"]), xi_max=a["xi_max"],
-                    xi_min=float(case.xi_min))
+                    xi_min=float(case.xi_min), n_list=a["n_list"], list_xi=_p(a["list_xi"]),
+                    list_w=_p(a["list_w"]))
         gas = Gas(R=a["R"], omega=a["omega"], Tref=a["Tref"], muRef=a["muRef"], Pr=a["Pr"], KInner=a["K"])
         q1 = a["q1_compat"] if q1_compat is None else int(q1_compat)
-        opt = Options(precision=int(precision), q1_compat=q1, device=int(device), fused=int(fused))
+        opt = Options(precision=int(precision), q1_compat=q1, device=int(device), fused=int(fused),
+                      macro_flux=int(a["macro_flux"]))
         h = ct.c_void_p()
         if L.dugks_create(ct.byref(mesh), patches, npatch, ct.byref(vset), ct.byref(gas), ct.byref(opt),
                           ct.byref(h)) != 0:

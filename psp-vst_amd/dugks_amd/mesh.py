@@ -340,12 +340,20 @@ def cavity(N: int, L: float = 1.0, dz: float = 0.1) -> PolyMesh:
 
 
 def box2d(nx: int, ny: int, x0: float = 0.0, x1: float = 1.0, y0: float = 0.0, y1: float = 1.0,
-          dz: float = 0.1, types=None) -> PolyMesh:
-    """Uniform nx x ny x 1 hex mesh of [x0,x1] x [y0,y1] x [0,dz] with the patches ``left``,
+          dz: float = 0.1, types=None, grading: float = 1.0) -> PolyMesh:
+    """nx x ny x 1 hex mesh of [x0,x1] x [y0,y1] x [0,dz] with the patches ``left``,
     ``right``, ``bottom``, ``top`` (geometric types from ``types``, default ``patch``) and
-    ``frontAndBack`` (empty)."""
-    xs = np.linspace(x0, x1, nx + 1)
-    ys = np.linspace(y0, y1, ny + 1)
+    ``frontAndBack`` (empty).  ``grading``: ratio of the last to the first cell width in x
+    and in y (geometric progression; 1 = uniform)."""
+    def edges(a, b, n):
+        if grading == 1.0 or n < 2:
+            return np.linspace(a, b, n + 1)
+        r = grading ** (1.0 / (n - 1))
+        w = r ** np.arange(n)
+        return a + (b - a) * np.concatenate([[0.0], np.cumsum(w) / np.sum(w)])
+    xs = edges(x0, x1, nx)
+    ys = edges(y0, y1, ny)
+    xs[-1], ys[-1] = x1, y1
     k_, j_, i_ = np.meshgrid(np.arange(2), np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
     pts = np.stack([xs[i_.ravel()], ys[j_.ravel()], np.where(k_.ravel() == 0, 0.0, dz)], axis=1)
     pid = lambda i, j, k: i + (nx + 1) * (j + (ny + 1) * k)
@@ -370,6 +378,34 @@ def box2d(nx: int, ny: int, x0: float = 0.0, x1: float = 1.0, y0: float = 0.0, y
         return out
 
     return hex_mesh(pts, np.array(cells), names, gtypes, classify)
+
+
+def velocity_mesh(ex, ey, ez, jitter: float = 0.0, seed: int = 7) -> PolyMesh:
+    """Hex mesh of velocity space with node coordinates ex x ey x ez (one patch ``boundary``),
+    as dugksFoam's VMESH build reads from constant/vMesh: its cell centres are the discrete
+    velocities and its cell volumes the weights (src/fvDVM/fvDVM/fvDVM.C:170-249).  ``jitter``
+    moves every interior node by up to that fraction of the local spacing (seeded), so the
+    cells are general hexahedra with non-planar faces and the velocities no tensor product; the
+    outer box and its volume stay as they are."""
+    ex, ey, ez = (np.asarray(e, dtype=np.float64) for e in (ex, ey, ez))
+    nx, ny, nz = len(ex) - 1, len(ey) - 1, len(ez) - 1
+    k_, j_, i_ = np.meshgrid(np.arange(nz + 1), np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
+    i_, j_, k_ = i_.ravel(), j_.ravel(), k_.ravel()
+    pts = np.stack([ex[i_], ey[j_], ez[k_]], axis=1)
+    if jitter > 0:
+        rng = np.random.default_rng(seed)
+        for ax, (idx, e, n) in enumerate(((i_, ex, nx), (j_, ey, ny), (k_, ez, nz))):
+            inner = (idx > 0) & (idx < n)
+            h = np.minimum(np.diff(e)[np.clip(idx - 1, 0, n - 1)], np.diff(e)[np.clip(idx, 0, n - 1)])
+            pts[inner, ax] += jitter * h[inner] * rng.uniform(-1.0, 1.0, int(inner.sum()))
+    pid = lambda i, j, k: i + (nx + 1) * (j + (ny + 1) * k)
+    cells = []
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                cells.append([pid(i, j, k), pid(i + 1, j, k), pid(i + 1, j + 1, k), pid(i, j + 1, k),
+                              pid(i, j, k + 1), pid(i + 1, j, k + 1), pid(i + 1, j + 1, k + 1), pid(i, j + 1, k + 1)])
+    return hex_mesh(pts, np.array(cells), ["boundary"], ["patch"], lambda q: np.zeros(len(q), dtype=np.int64))
 
 
 # OpenFOAM hex model faces (vertex order 0-3 bottom, 4-7 top), outward for a right-handed hex

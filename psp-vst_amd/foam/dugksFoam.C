@@ -2,7 +2,7 @@
 // facade (fvDVM.H) on one GPU.
 //
 //   dugksFoam [-case DIR] [-precision fp64|mixed] [-device N] [-fused N] [-maxSteps N]
-//             [-hostLoop]
+//             [-hostLoop] [-vmesh]
 //
 // Per iteration, as dugksFoam.C:63-108: Courant number (CourantNo.H), setDeltaTvar.H
 // (dt = min(min(maxCo/(Co + SMALL), 1 + 0.1 maxCo/(Co + SMALL), 1.2) dt, maxDeltaT) when
@@ -21,7 +21,9 @@
 // reference's per-step host loop instead (getCoNum, setDeltaT, evolution).  -maxSteps N
 // stops after N steps (a test aid); the last state is written when the loop ends.  The
 // reference's -dvParallel velocity decomposition across processes is not part of this
-// one-GPU build and is refused.
+// one-GPU build and is refused.  -vmesh reads the velocity set from constant/vMesh, what the
+// reference does when built with -DVMESH (src/fvDVM/Make/options:13; fvDVM.C:129-253); the
+// macroFlux entry of DVMProperties then selects that build's flux update.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -47,7 +49,7 @@ struct Args {
 int usage(const char *msg)
 {
     std::fprintf(stderr, "dugksFoam: %s\nusage: dugksFoam [-case DIR] [-precision fp64|mixed] [-device N] "
-                         "[-fused N] [-maxSteps N] [-hostLoop]\n", msg);
+                         "[-fused N] [-maxSteps N] [-hostLoop] [-vmesh]\n", msg);
     return 1;
 }
 
@@ -85,6 +87,8 @@ bool parse(int argc, char **argv, Args &a, std::string &err)
             a.max_steps = std::atol(v.c_str());
         } else if (k == "-hostLoop") {
             a.host_loop = true;
+        } else if (k == "-vmesh") {
+            a.opt.vmesh = true;
         } else if (k == "-dvParallel") {
             err = "-dvParallel (velocity decomposition over processes) is not part of this one-GPU build";
             return false;

@@ -24,20 +24,32 @@ extern "C" {
 
 const char *dugks_foam_last_error(void) { return g_err.c_str(); }
 
-int dugks_foam_read(const char *case_dir, const char *time_name, dugks_foam_case **out)
+namespace {
+int read_any(const char *case_dir, const char *time_name, dugks_foam_case **out, bool vmesh)
 {
     if (!out) return fail("dugks_foam_read: out is NULL");
     *out = nullptr;
     if (!case_dir) return fail("dugks_foam_read: case_dir is NULL");
     auto *h = new dugks_foam_case();
     std::string err;
-    if (!dugks_foam::read_case(case_dir, time_name ? time_name : "", h->c, err)) {
+    if (!dugks_foam::read_case(case_dir, time_name ? time_name : "", h->c, err, vmesh)) {
         delete h;
         return fail(err);
     }
     for (const auto &b : h->c.rho.bc) h->rho_types.push_back(b.type);
     *out = h;
     return 0;
+}
+}  // namespace
+
+int dugks_foam_read(const char *case_dir, const char *time_name, dugks_foam_case **out)
+{
+    return read_any(case_dir, time_name, out, false);
+}
+
+int dugks_foam_read_vmesh(const char *case_dir, const char *time_name, dugks_foam_case **out)
+{
+    return read_any(case_dir, time_name, out, true);
 }
 
 void dugks_foam_free(dugks_foam_case *c) { delete c; }
@@ -78,11 +90,14 @@ int dugks_foam_solver_inputs(const dugks_foam_case *h, dugks_mesh *mesh, dugks_p
         }
     }
     if (vset) {
-        vset->n_per_dim = c.nDV;
-        vset->xis = c.xis.data();
-        vset->weights = c.wts.data();
+        vset->n_per_dim = c.vmesh ? 0 : c.nDV;
+        vset->xis = c.vmesh ? nullptr : c.xis.data();
+        vset->weights = c.vmesh ? nullptr : c.wts.data();
         vset->xi_max = c.xi_max;
         vset->xi_min = c.xi_min;
+        vset->n_list = c.vmesh ? (int)c.vlist_w.size() : 0;
+        vset->list_xi = c.vmesh ? c.vlist_xi.data() : nullptr;
+        vset->list_w = c.vmesh ? c.vlist_w.data() : nullptr;
     }
     if (gas) {
         gas->R = c.R;
@@ -146,6 +161,8 @@ int dugks_foam_array_d(const dugks_foam_case *h, const char *name, const double 
     else if (n == "bf_U") v = &c.bf_U;
     else if (n == "bf_T") v = &c.bf_T;
     else if (n == "patch_pressure") v = &c.patch_pressure;
+    else if (n == "xi_list") v = &c.vlist_xi;
+    else if (n == "xi_list_weights") v = &c.vlist_w;
     else return fail("dugks_foam_array_d: no double array named " + n);
     *data = v->data();
     *count = v->size();
@@ -191,6 +208,8 @@ int dugks_foam_scalar(const dugks_foam_case *h, const char *name, double *value)
     else if (n == "Pr") *value = c.Pr;
     else if (n == "KInner") *value = c.KInner;
     else if (n == "time") *value = c.time;
+    else if (n == "macro_flux") *value = c.macro_flux_form;
+    else if (n == "vmesh") *value = c.vmesh ? 1 : 0;
     else return fail("dugks_foam_scalar: no scalar named " + n);
     return 0;
 }

@@ -350,76 +350,9 @@ bool compute_geometry(Case &c, std::string &err)
     return true;
 }
 
-bool read_case(const std::string &dir, const std::string &tname, Case &c, std::string &err)
+// points, faces, owner, neighbour and boundary of the polyMesh directory pm into c
+bool read_poly_mesh(const std::string &pm, Case &c, std::string &err)
 {
-    c = Case();
-    c.dir = dir;
-    // ---- controlDict (readTimeControlsExplicit.H:39-52, createTime.H)
-    {
-        const std::string path = join(dir, "system/controlDict");
-        FoamFile f;
-        if (!read_foam_dict(path, f, err)) return false;
-        const Dict &d = f.body;
-        Controls &k = c.ctl;
-        const Entry *e;
-        if ((e = d.find("startFrom")) && !entry_word(*e, k.start_from, err, "startFrom")) return false;
-        if ((e = d.find("startTime")) && !entry_scalar(*e, k.start_time, err, "startTime")) return false;
-        if (!(e = need(d, "endTime", path, err)) || !entry_scalar(*e, k.end_time, err, "endTime")) return false;
-        if (!(e = need(d, "deltaT", path, err)) || !entry_scalar(*e, k.delta_t, err, "deltaT")) return false;
-        if ((e = d.find("writeControl")) && !entry_word(*e, k.write_control, err, "writeControl")) return false;
-        if ((e = d.find("writeInterval")) && !entry_scalar(*e, k.write_interval, err, "writeInterval")) return false;
-        if ((e = d.find("timeFormat")) && !entry_word(*e, k.time_format, err, "timeFormat")) return false;
-        double v;
-        if ((e = d.find("writePrecision"))) {
-            if (!entry_scalar(*e, v, err, "writePrecision")) return false;
-            k.write_precision = (int)v;
-        }
-        if ((e = d.find("timePrecision"))) {
-            if (!entry_scalar(*e, v, err, "timePrecision")) return false;
-            k.time_precision = (int)v;
-        }
-        if ((e = d.find("adjustTimeStep")) && !entry_bool(*e, k.adjust_time_step, err, "adjustTimeStep")) return false;
-        if ((e = d.find("maxCo")) && !entry_scalar(*e, k.max_co, err, "maxCo")) return false;
-        if ((e = d.find("maxDeltaT")) && !entry_scalar(*e, k.max_delta_t, err, "maxDeltaT")) return false;
-        if ((e = d.find("convergeTol")) && !entry_scalar(*e, k.converge_tol, err, "convergeTol")) return false;
-        if ((e = d.find("convergeCheckSteps"))) {
-            if (!entry_scalar(*e, v, err, "convergeCheckSteps")) return false;
-            k.converge_check_steps = (int)v;
-        }
-        if (k.converge_check_steps < 1) {
-            err = path + ": convergeCheckSteps must be >= 1";
-            return false;
-        }
-        if (!(k.delta_t > 0)) {
-            err = path + ": deltaT must be positive";
-            return false;
-        }
-    }
-    // ---- the time directory to start from
-    const auto times = time_dirs(dir);
-    if (!tname.empty()) {
-        c.time_name = tname;
-        if (!is_number_name(tname, c.time)) {
-            err = "time name '" + tname + "' is not a number";
-            return false;
-        }
-    } else if (times.empty()) {
-        err = dir + ": no time directory";
-        return false;
-    } else if (c.ctl.start_from == "latestTime") {
-        c.time = times.back().first;
-        c.time_name = times.back().second;
-    } else if (c.ctl.start_from == "firstTime") {
-        c.time = times.front().first;
-        c.time_name = times.front().second;
-    } else {
-        c.time = c.ctl.start_time;
-        c.time_name = time_name(c.ctl, c.ctl.start_time);
-        for (const auto &t : times)
-            if (std::fabs(t.first - c.ctl.start_time) <= 1e-12 * std::max(1.0, std::fabs(t.first))) c.time_name = t.second;
-    }
-    // ---- polyMesh
-    const std::string pm = join(dir, "constant/polyMesh");
     {
         FoamFile f;
         std::unique_ptr<Lexer> lx;
@@ -515,6 +448,79 @@ bool read_case(const std::string &dir, const std::string &tname, Case &c, std::s
             c.patches.push_back(pi);
         }
     }
+    return true;
+}
+
+bool read_case(const std::string &dir, const std::string &tname, Case &c, std::string &err, bool vmesh)
+{
+    c = Case();
+    c.dir = dir;
+    // ---- controlDict (readTimeControlsExplicit.H:39-52, createTime.H)
+    {
+        const std::string path = join(dir, "system/controlDict");
+        FoamFile f;
+        if (!read_foam_dict(path, f, err)) return false;
+        const Dict &d = f.body;
+        Controls &k = c.ctl;
+        const Entry *e;
+        if ((e = d.find("startFrom")) && !entry_word(*e, k.start_from, err, "startFrom")) return false;
+        if ((e = d.find("startTime")) && !entry_scalar(*e, k.start_time, err, "startTime")) return false;
+        if (!(e = need(d, "endTime", path, err)) || !entry_scalar(*e, k.end_time, err, "endTime")) return false;
+        if (!(e = need(d, "deltaT", path, err)) || !entry_scalar(*e, k.delta_t, err, "deltaT")) return false;
+        if ((e = d.find("writeControl")) && !entry_word(*e, k.write_control, err, "writeControl")) return false;
+        if ((e = d.find("writeInterval")) && !entry_scalar(*e, k.write_interval, err, "writeInterval")) return false;
+        if ((e = d.find("timeFormat")) && !entry_word(*e, k.time_format, err, "timeFormat")) return false;
+        double v;
+        if ((e = d.find("writePrecision"))) {
+            if (!entry_scalar(*e, v, err, "writePrecision")) return false;
+            k.write_precision = (int)v;
+        }
+        if ((e = d.find("timePrecision"))) {
+            if (!entry_scalar(*e, v, err, "timePrecision")) return false;
+            k.time_precision = (int)v;
+        }
+        if ((e = d.find("adjustTimeStep")) && !entry_bool(*e, k.adjust_time_step, err, "adjustTimeStep")) return false;
+        if ((e = d.find("maxCo")) && !entry_scalar(*e, k.max_co, err, "maxCo")) return false;
+        if ((e = d.find("maxDeltaT")) && !entry_scalar(*e, k.max_delta_t, err, "maxDeltaT")) return false;
+        if ((e = d.find("convergeTol")) && !entry_scalar(*e, k.converge_tol, err, "convergeTol")) return false;
+        if ((e = d.find("convergeCheckSteps"))) {
+            if (!entry_scalar(*e, v, err, "convergeCheckSteps")) return false;
+            k.converge_check_steps = (int)v;
+        }
+        if (k.converge_check_steps < 1) {
+            err = path + ": convergeCheckSteps must be >= 1";
+            return false;
+        }
+        if (!(k.delta_t > 0)) {
+            err = path + ": deltaT must be positive";
+            return false;
+        }
+    }
+    // ---- the time directory to start from
+    const auto times = time_dirs(dir);
+    if (!tname.empty()) {
+        c.time_name = tname;
+        if (!is_number_name(tname, c.time)) {
+            err = "time name '" + tname + "' is not a number";
+            return false;
+        }
+    } else if (times.empty()) {
+        err = dir + ": no time directory";
+        return false;
+    } else if (c.ctl.start_from == "latestTime") {
+        c.time = times.back().first;
+        c.time_name = times.back().second;
+    } else if (c.ctl.start_from == "firstTime") {
+        c.time = times.front().first;
+        c.time_name = times.front().second;
+    } else {
+        c.time = c.ctl.start_time;
+        c.time_name = time_name(c.ctl, c.ctl.start_time);
+        for (const auto &t : times)
+            if (std::fabs(t.first - c.ctl.start_time) <= 1e-12 * std::max(1.0, std::fabs(t.first))) c.time_name = t.second;
+    }
+    // ---- polyMesh
+    if (!read_poly_mesh(join(dir, "constant/polyMesh"), c, err)) return false;
     if (!compute_geometry(c, err)) return false;
     // ---- gradient scheme (system/fvSchemes; absent: leastSquares)
     {
@@ -581,17 +587,44 @@ bool read_case(const std::string &dir, const std::string &tname, Case &c, std::s
             }
             for (double x : cl) c.df_write_cells.push_back((int)x);
         }
-        if (c.macro_flux != "no") {
-            err = path + ": macroFlux " + c.macro_flux + " is not supported (the cell update takes fvDVM.C:1521-1554)";
+        // updateMacroVol takes the moments of g~ only for macroFlux "no" (fvDVM.C:1528); any
+        // other word selects the flux update, whose form the VMESH build changes (F:1562)
+        c.macro_flux_form = c.macro_flux == "no" ? 0 : (vmesh ? 2 : 1);
+    }
+    c.vmesh = vmesh;
+    if (vmesh) {
+        // initialiseDV of the VMESH build (fvDVM.C:129-253): the velocity set is the cell
+        // centres and volumes of constant/vMesh, components beyond nSolutionD set to 0, and
+        // xiMax the largest |xi|; constant/Xis and constant/weights are not read
+        Case vm;
+        if (!read_poly_mesh(join(dir, "constant/vMesh"), vm, err) || !compute_geometry(vm, err)) {
+            err = "constant/vMesh: " + err;
             return false;
         }
-    }
-    if (!read_scalar_list_file(join(dir, "constant/Xis"), c.xis, err) ||
-        !read_scalar_list_file(join(dir, "constant/weights"), c.wts, err))
-        return false;
-    if ((int)c.xis.size() != c.nDV || (int)c.wts.size() != c.nDV) {
-        err = "constant/Xis and constant/weights must hold nDV = " + std::to_string(c.nDV) + " values";
-        return false;
+        c.vlist_xi.assign(3 * (size_t)vm.nC, 0.0);
+        c.vlist_w = vm.V;
+        c.xi_max = 0.0;
+        for (int i = 0; i < vm.nC; ++i) {
+            double m2 = 0.0;
+            for (int k = 0; k < c.n_solution_d; ++k) {
+                const double x = vm.C[3 * (size_t)i + k];
+                c.vlist_xi[3 * (size_t)i + k] = x;
+                m2 += x * x;
+            }
+            c.xi_max = std::max(c.xi_max, std::sqrt(m2));
+            if (!(vm.V[i] > 0)) {
+                err = "constant/vMesh: cell " + std::to_string(i) + " has a non-positive volume";
+                return false;
+            }
+        }
+    } else {
+        if (!read_scalar_list_file(join(dir, "constant/Xis"), c.xis, err) ||
+            !read_scalar_list_file(join(dir, "constant/weights"), c.wts, err))
+            return false;
+        if ((int)c.xis.size() != c.nDV || (int)c.wts.size() != c.nDV) {
+            err = "constant/Xis and constant/weights must hold nDV = " + std::to_string(c.nDV) + " values";
+            return false;
+        }
     }
     // ---- initial fields (createFields.H)
     if (!read_vol_field(c, "rho", 1, c.rho, err) || !read_vol_field(c, "U", 3, c.U, err) ||

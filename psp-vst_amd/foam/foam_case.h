@@ -6,7 +6,7 @@
 //   constant/polyMesh/{points,faces,owner,neighbour,boundary}  (ASCII, plain or .gz)
 //   constant/DVMProperties   fvDVMparas {nDV, xiMax, xiMin, macroFlux}, gasProperties
 //                            {R, omega, Tref, muRef, Pr, KInner}, DFwriteCellList
-//   constant/Xis, constant/weights
+//   constant/Xis, constant/weights, or (VMESH build) constant/vMesh's cell centres and volumes
 //   system/controlDict       startFrom/startTime, endTime, deltaT, writeControl,
 //                            writeInterval, writePrecision, timePrecision, adjustTimeStep,
 //                            maxCo, maxDeltaT, convergeTol, convergeCheckSteps
@@ -85,6 +85,9 @@ struct Case {
     double R = 0.0, omega = 0.0, Tref = 0.0, muRef = 0.0, Pr = 0.0;
     int KInner = 0;
     std::string macro_flux = "no";
+    int macro_flux_form = 0;                    // dugks_options.macro_flux: 0 no, 1 yes, 2 yes (VMESH)
+    bool vmesh = false;                         // velocity set from constant/vMesh (VMESH build)
+    std::vector<double> vlist_xi, vlist_w;      // VMESH velocity list [n][3], [n]
     std::string grad_scheme = "leastSquares";   // system/fvSchemes gradSchemes default
     std::vector<double> ls_own, ls_nei;         // Gauss linear: gradient vectors [nIF][3]
     std::vector<int> df_write_cells;
@@ -95,8 +98,12 @@ struct Case {
 };
 
 // Reads a case; `time_name` empty: the time named by controlDict's startFrom (startTime,
-// firstTime or latestTime).
-bool read_case(const std::string &dir, const std::string &time_name, Case &c, std::string &err);
+// firstTime or latestTime).  vmesh: the velocity set is the cell centres and volumes of
+// constant/vMesh, as dugksFoam built with -DVMESH reads it (fvDVM.C:129-253).
+bool read_case(const std::string &dir, const std::string &time_name, Case &c, std::string &err, bool vmesh = false);
+
+// points, faces, owner, neighbour and boundary of a polyMesh directory (no geometry)
+bool read_poly_mesh(const std::string &pm, Case &c, std::string &err);
 
 // Geometry of a polyMesh already in `c` (points, faces, owner, neighbour, patches).
 bool compute_geometry(Case &c, std::string &err);

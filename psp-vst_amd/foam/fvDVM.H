@@ -56,6 +56,7 @@ struct fvDVMOptions {
     int device = 0;
     int fused = -1;              // dugks_options.fused
     int q1_compat = 1;
+    bool vmesh = false;          // velocity set from constant/vMesh, as dugksFoam built with -DVMESH
 };
 
 class fvDVM {
@@ -66,7 +67,9 @@ public:
     // the solver: fvDVM::fvDVM (fvDVM.C:1901-2117).  time_name empty: controlDict startFrom.
     explicit fvDVM(const std::string &case_dir, const Options &opt = Options(), const std::string &time_name = "")
     {
-        check_foam(dugks_foam_read(case_dir.c_str(), time_name.c_str(), &case_), "reading the case");
+        check_foam(opt.vmesh ? dugks_foam_read_vmesh(case_dir.c_str(), time_name.c_str(), &case_)
+                             : dugks_foam_read(case_dir.c_str(), time_name.c_str(), &case_),
+                   "reading the case");
         try {
             int np = 0;
             check_foam(dugks_foam_solver_inputs(case_, nullptr, nullptr, 0, &np, nullptr, nullptr), "case inputs");
@@ -88,6 +91,9 @@ public:
             o.q1_compat = opt.q1_compat;
             o.device = opt.device;
             o.fused = opt.fused;
+            double mf = 0.0;
+            check_foam(dugks_foam_scalar(case_, "macro_flux", &mf), "macroFlux");
+            o.macro_flux = (int)mf;
             check(dugks_create(&mesh, patches_.data(), np, &vset, &gas, &o, &h_), "dugks_create");
             const double *rho, *U, *T;
             check_foam(dugks_foam_initial_fields(case_, &rho, &U, &T), "initial fields");

@@ -35,7 +35,7 @@ def test_library_exports_every_declared_symbol():
 def test_abi_version_and_errors_without_gpu():
     from dugks_amd import hip
     L = hip.load_library()
-    assert L.dugks_abi_version() == 2
+    assert L.dugks_abi_version() == 3
     # invalid arguments are rejected before any device call
     h = ct.c_void_p()
     assert L.dugks_create(None, None, 0, None, None, None, ct.byref(h)) != 0
