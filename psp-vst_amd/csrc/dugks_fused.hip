@@ -18,10 +18,14 @@
 //     stage 3  g~ <- 4/3 f-bar-plus - 1/3 g~ - dt/V sum(+-flux) in ascending face order
 //              (fvDVM.C:1462-1519), in place; FP64 raw moments of the new g~, h~ per cell
 //
-// Each thread owns fixed cells / faces for the whole work-group.  The velocity set is a
-// tensor product, so the Maxwellian factor A exp(-(xi_x - U_x)^2 / 2RT) of an entity is
-// formed once per x-stripe and kept in registers; a chunk then needs one exp for the y/z
-// part per entity.  Work-group ids are mapped so that the velocity groups of a block run on
+// Each thread owns fixed cells / faces for the whole work-group.  The passes wait on memory
+// latency more than on arithmetic, so the loads of a stage are issued together before their
+// values are used, the chunk index is wave-uniform (per-velocity tables are scalar loads),
+// and exponentials use a short exp (exp_cw).  Pass A keeps the tensor-set factorisation of
+// the cells' Maxwellians: A exp(-(xi_x - U_x)^2 / 2RT) formed once per x-stripe and kept in
+// registers, one exp per chunk for the y/z part.  Pass B forms each face and cell
+// velocity's Maxwellian factor on its own, which frees the registers the cached factors
+// held for the batched loads.  Work-group ids are mapped so that the velocity groups of a block run on
 // the same accelerator die and share its L2.  Reconstruction arithmetic runs in the storage
 // type S (FP32 for the mixed path); equilibria, relaxation, update and moments in FP64.
 // Entries with |xi.S| < VSMALL carry no flux (sigma f = 0 in double precision); their face
@@ -128,6 +132,48 @@ __device__ inline EqCoef make_coef(const Geo &g, double rho, const d3 &U, double
     return k;
 }
 
+// exp(x) for Maxwellian factors (x <= 0, above the underflow limit -708): Cody and Waite's
+// argument reduction (Software Manual for the Elementary Functions, 1980) x = n ln2 + r,
+// |r| <= ln2/2, with ln2 split into a high part of 32 significant bits (n ln2_hi is exact)
+// and a low part, then exp(r) by its Taylor series to degree 13 in Horner form (truncation
+// below 5e-18 relative) and the scaling by 2^n.  19 instructions, against about 45 for the
+// general-purpose exp; a few units in the last place from exp.
+__device__ inline double exp_cw(double x)
+{
+    const double n = __builtin_rint(x * 1.4426950408889634074);
+    double r = fma(-n, 6.93147180369123816490e-01, x);
+    r = fma(-n, 1.90821492927058770002e-10, r);
+    double p = 1.0 / 6227020800.0;          // 1/13!
+    p = fma(p, r, 1.0 / 479001600.0);       // 1/12!
+    p = fma(p, r, 1.0 / 39916800.0);
+    p = fma(p, r, 1.0 / 3628800.0);
+    p = fma(p, r, 1.0 / 362880.0);
+    p = fma(p, r, 1.0 / 40320.0);
+    p = fma(p, r, 1.0 / 5040.0);
+    p = fma(p, r, 1.0 / 720.0);
+    p = fma(p, r, 1.0 / 120.0);
+    p = fma(p, r, 1.0 / 24.0);
+    p = fma(p, r, 1.0 / 6.0);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return ldexp(p, (int)n);
+}
+
+// Shakhov equilibrium (fvDVM.C:673-679) of record k at velocity xi, Maxwellian factor
+// A exp(-|xi - U|^2 / 2RT) formed here with exp_cw
+__device__ inline void shakhov_eq(const Geo &g, const EqCoef &k, const d3 &xi, double &gEq, double &hEq)
+{
+    const double cx = xi.x - k.ux, cy = xi.y - k.uy, cz = xi.z - k.uz;
+    const double s = (cx * cx + cy * cy + cz * cz) * k.invRT;
+    const double gM = k.A * exp_cw(-0.5 * s);
+    const double cq = cx * k.qx + cy * k.qy + cz * k.qz;
+    const double omp = 1.0 - g.Pr;
+    const double kd = g.K + 3.0 - g.D;
+    gEq = (1.0 + omp * cq * (s - g.D - 2.0)) * gM;
+    hEq = (kd + omp * cq * ((s - g.D) * kd - 2 * g.K)) * gM * k.RTh;
+}
+
 // Shakhov equilibrium (fvDVM.C:673-679) given gM = A exp(-|xi - U|^2 / 2RT):
 // c = xi - U, s = |c|^2/(R T), cq = c.q/(5 rho (R T)^2)
 __device__ inline void shakhov_poly(const Geo &g, const EqCoef &k, const d3 &xi, double gM, double &gEq, double &hEq)
@@ -145,7 +191,7 @@ __device__ inline void shakhov_poly(const Geo &g, const EqCoef &k, const d3 &xi,
 __device__ inline double row_factor(const EqCoef &k, const d3 &xr)
 {
     const double dy = xr.y - k.uy, dz = xr.z - k.uz;
-    return exp(-((dy * dy + dz * dz) * k.invRT) / 2.0);
+    return exp_cw(-0.5 * ((dy * dy + dz * dz) * k.invRT));
 }
 
 // A exp(-(xi_x - U_x)^2 / 2RT) for the abscissae ix0 .. ix0+VC-1 of a stripe; xi[ix] is the
@@ -156,10 +202,11 @@ __device__ inline void stripe_factors(const Geo &g, const EqCoef &k, int ix0, in
 #pragma unroll
     for (int vv = 0; vv < VC; ++vv) {
         const int ix = min(ix0 + vv, n1 - 1);
-        const double dx = g.xi[ix].x - k.ux;
-        ex[vv * es] = k.A * exp(-((dx * dx) * k.invRT) / 2.0);
+        const double dx = cload(g.xi, ix).x - k.ux;
+        ex[vv * es] = k.A * exp_cw(-0.5 * ((dx * dx) * k.invRT));
     }
 }
+
 
 // raw moments of one (value, velocity) pair: m[k] += w * g * phi_k(xi), h likewise
 template <int D>
@@ -304,13 +351,16 @@ __device__ inline void macros_from_moments(const Geo &g, const double *m, double
 // Velocity chunk ch of the (x-stripe, row) order: stripe = ch / nrow, row = ch % nrow.
 struct Chunk {
     int ix0, nvv, vrow;  // first abscissa of the stripe, valid velocities, v of (ix = 0, row)
-    bool new_stripe;     // stripe differs from the previous chunk's: x-factors must be formed
+    bool new_stripe;     // stripe differs from the previous chunk's
 };
 
+// The chunk index is the same for every lane of a wave; readfirstlane tells the compiler,
+// so the velocity indices derived from it stay in scalar registers and the per-velocity
+// tables are read with scalar loads.
 __device__ inline Chunk chunk_of(const FArgs &a, int ch, int vc, int &stripe)
 {
-    const int st = ch / a.nrow;
-    const int row = ch - st * a.nrow;
+    const int st = __builtin_amdgcn_readfirstlane(ch / a.nrow);
+    const int row = __builtin_amdgcn_readfirstlane(ch - st * a.nrow);
     Chunk c;
     c.ix0 = st * vc;
     c.nvv = min(vc, a.n1 - c.ix0);
@@ -350,12 +400,37 @@ __device__ inline void cell_bar(const Geo &g, const FArgs &a, const EqCoef &k, i
     for (int vv = 0; vv < VC; ++vv) {
         const int v = ck.vrow + ck.ix0 + min(vv, ck.nvv - 1);
         double gEq, hEq;
-        shakhov_poly(g, k, g.xi[v], cex[vv * cs] * eyz, gEq, hEq);
+        shakhov_poly(g, k, cload(g.xi, v), cex[vv * cs] * eyz, gEq, hEq);
         // with FP32 storage, stored values and gb, hb are deviations from the per-velocity
         // reference; FP64 storage keeps values (its reference is zero)
         if (sizeof(S) == 4) {
-            gEq -= g.ref_g[v];
-            hEq -= g.ref_h[v];
+            gEq -= cload(g.ref_g, v);
+            hEq -= cload(g.ref_h, v);
+        }
+        gb[vv] = (1.0 - af) * (double)gt[vv] + af * gEq;
+        hb[vv] = (1.0 - af) * (double)ht[vv] + af * hEq;
+    }
+}
+
+
+// f-bar-plus of cell c (equilibrium record k) for the chunk's velocities (FP64), each
+// velocity's Maxwellian factor formed on its own (shakhov_eq), so no factors are held in
+// registers across chunks; LOAD: g~, h~ are loaded here, else gt, ht hold them
+template <class S, int VC, bool LOAD = true>
+__device__ inline void cell_bar_eq(const Geo &g, const EqCoef &k, int c, const S *__restrict__ gT,
+                                   const S *__restrict__ hT, double dt, const Chunk &ck, S *gt, S *ht, double *gb,
+                                   double *hb)
+{
+    if (LOAD) load_cell<S, VC>(g, c, gT, hT, ck, gt, ht);
+    const double af = 1.5 * dt / (2.0 * k.tr + dt);
+#pragma unroll
+    for (int vv = 0; vv < VC; ++vv) {
+        const int v = ck.vrow + ck.ix0 + min(vv, ck.nvv - 1);
+        double gEq, hEq;
+        shakhov_eq(g, k, cload(g.xi, v), gEq, hEq);
+        if (sizeof(S) == 4) {
+            gEq -= cload(g.ref_g, v);
+            hEq -= cload(g.ref_h, v);
         }
         gb[vv] = (1.0 - af) * (double)gt[vv] + af * gEq;
         hb[vv] = (1.0 - af) * (double)ht[vv] + af * hEq;
@@ -373,7 +448,7 @@ __device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &
                                  const S (&pg)[CPT][VC], const S (&ph)[CPT][VC], const EqCoef (&kc)[CPT],
                                  double (&cex)[CPT][VC], double *cexL, P2<S> *fb)
 {
-    const d3 xr = g.xi[ck.vrow];
+    const d3 xr = cload(g.xi, ck.vrow);
 #pragma unroll
     for (int s = 0; s < CPT; ++s) {
         const int l = threadIdx.x + s * NT;
@@ -488,7 +563,7 @@ __global__ void __launch_bounds__(kNTA)
     int blkid, vg;
     if (!wg_map(a, blkid, vg)) return;
     extern __shared__ __align__(16) unsigned char smem[];
-    const BlockInfo bi = a.blk[blkid];
+    const BlockInfo bi = cload(a.blk, blkid);
     const int ch0 = vg * a.cpg, ch1 = min(a.nchunk, ch0 + a.cpg);
     P2<S> *fb = (P2<S> *)smem;        // [VC][max_nl]
     P2<S> *gr = fb + VC * a.max_nl;    // [VC][max_nc][D]
@@ -576,7 +651,7 @@ __global__ void __launch_bounds__(kNTA)
         __syncthreads();
         stage_grad<S, D, VC, kNTA>(a, bi, ck.nvv, gk, fb, gr);
         __syncthreads();
-        const d3 xr = g.xi[ck.vrow];
+        const d3 xr = cload(g.xi, ck.vrow);
 #pragma unroll
         for (int s = 0; s < kSlotsA; ++s) {
             if (fid[s] < 0) continue;
@@ -587,8 +662,8 @@ __global__ void __launch_bounds__(kNTA)
 #pragma unroll 1
             for (int vv = vb; vv < ve; ++vv) {
                 const int v = ck.vrow + ck.ix0 + vv;
-                const d3 xi = g.xi[v];
-                const double w = g.w[v];
+                const d3 xi = cload(g.xi, v);
+                const double w = cload(g.w, v);
                 const double xs = sigma_dot(xi, fS[s]);
                 const P2<S> *pf = fb + vv * a.max_nl, *pq = gr + vv * a.max_nc * D;
                 double gf, hf;
@@ -795,26 +870,26 @@ __global__ void __launch_bounds__(kBlk) k_slot_sf(Geo g, FArgs a, int n_slots)
 template <class S>
 __device__ inline double face_value(S stored, const double *ref, int v)
 {
-    if (sizeof(S) == 4) return (double)stored + ref[v];
+    if (sizeof(S) == 4) return (double)stored + cload(ref, v);
     return (double)stored;
 }
 
-// flux sigma * f of velocity v through face slot own_slot (face f, area vector Sf): the
-// stored face value relaxed towards the face equilibrium (fvDVM.C:1362-1460), or the
-// incoming half of a wall / inlet face; gM is the Maxwellian factor A exp(-|xi - U|^2 / 2RT)
-// of the face record k.  Tangent velocities of internal faces carry no flux.
+// flux sigma * f of velocity v through a face slot (face f, area vector Sf, equilibrium
+// record k) whose stored value is fv: the stored face value relaxed towards the face
+// equilibrium (fvDVM.C:1362-1460), or the incoming half of a wall / inlet face.  Tangent
+// velocities of internal faces carry no flux.  The caller loads fv for every slot and velocity, incoming entries
+// included (they are not read), so that the loads of a slot are in flight together instead
+// of one behind the other under this branch.
 template <class S>
-__device__ inline P2<double> slot_flux(const Geo &g, const FArgs &a, const P2<S> *__restrict__ F, int own_slot,
-                                       int f, int v, const d3 &Sf, const EqCoef &k, double gM, const double *rho_b)
+__device__ inline P2<double> slot_flux(const Geo &g, const FArgs &a, const P2<S> fv, int f, int v, const d3 &Sf,
+                                       const EqCoef &k, const double *rho_b)
 {
-    const d3 xi = g.xi[v];
+    const d3 xi = cload(g.xi, v);
     const double xs = sigma_dot(xi, Sf);
     const int b = f - g.nIF;
     if (b < 0 ? (xs >= kVSmall || xs < -kVSmall) : xs > 0) {
-        const size_t kf = (size_t)v * a.n_slots + own_slot;
         double gEq, hEq;
-        shakhov_poly(g, k, xi, gM, gEq, hEq);
-        const P2<S> fv = F[kf];
+        shakhov_eq(g, k, xi, gEq, hEq);
         return P2<double>{xs * ((1.0 - k.tr) * face_value<S>(fv.g, g.ref_g, v) + k.tr * gEq),
                           xs * ((1.0 - k.tr) * face_value<S>(fv.h, g.ref_h, v) + k.tr * hEq)};
     }
@@ -822,7 +897,7 @@ __device__ inline P2<double> slot_flux(const Geo &g, const FArgs &a, const P2<S>
         const BndCoef kb = a.bc[b];
         const double scale = g.bf_type[b] == kMaxwellWall ? rho_b[b] : 1.0;
         const double cx = xi.x - kb.ux, cy = xi.y - kb.uy, cz = xi.z - kb.uz;
-        const double gv = scale * (kb.A * exp(-((cx * cx + cy * cy + cz * cz) * kb.invRT) / 2.0));
+        const double gv = scale * (kb.A * exp_cw(-0.5 * ((cx * cx + cy * cy + cz * cz) * kb.invRT)));
         return P2<double>{xs * gv, xs * (gv * kb.hfac)};
     }
     return P2<double>{0.0, 0.0};
@@ -838,7 +913,7 @@ __global__ void __launch_bounds__(kNTB)
     int blkid, vg;
     if (!wg_map(a, blkid, vg)) return;
     extern __shared__ __align__(16) unsigned char smem[];
-    const BlockInfo bi = a.blk[blkid];
+    const BlockInfo bi = cload(a.blk, blkid);
     const int ch0 = vg * a.cpg, ch1 = min(a.nchunk, ch0 + a.cpg);
     P2<double> *fl = (P2<double> *)smem;            // [VC][max_nfs] fluxes sigma * f_f
     const double dt = *dtp;
@@ -867,7 +942,6 @@ __global__ void __launch_bounds__(kNTB)
     }
 
     constexpr int FW = FPT - 1;  // whole faces per thread in stage 2
-    double cex[VC], fex[FW][VC];
     // 2-D FP64: the next chunk's g~, h~ of the cell are loaded before the chunk's last
     // barrier (7.52 -> 7.42 ms on the 256^2 cavity; with FP32 storage it measured slower, and
     // 3-D has no registers to spare).  The update below writes only this chunk's velocities
@@ -881,7 +955,7 @@ __global__ void __launch_bounds__(kNTB)
     }
     for (int ch = ch0; ch < ch1; ++ch) {
         const Chunk ck = chunk_of(a, ch, VC, stripe);
-        const d3 xr = g.xi[ck.vrow];
+        const d3 xr = cload(g.xi, ck.vrow);
         // stage 1: f-bar-plus of the thread's cell
         S gt[VC], ht[VC];
         double gb[VC], hb[VC];
@@ -894,12 +968,13 @@ __global__ void __launch_bounds__(kNTB)
         }
         if (has_cell) {
             const EqCoef k = a.cc[cell];
-            cell_bar<S, VC, !kAhead>(g, a, k, cell, gT, hT, dt, ck, xr, cex, gt, ht, gb, hb);
+            cell_bar_eq<S, VC, !kAhead>(g, k, cell, gT, hT, dt, ck, gt, ht, gb, hb);
         }
         // stage 2: face values -> flux sigma * f_f in LDS.  Slots j < FW * kNTB: whole faces,
-        // FW per thread, x-factors kept for the stripe; the (slot, velocity) pairs of the
-        // slots beyond are dealt over all threads, so no thread takes one whole face more
-        // than the others while they wait at the barrier.
+        // FW per thread; the (slot, velocity) pairs of the slots beyond are dealt over all
+        // threads, so no thread takes one whole face more than the others while they wait at
+        // the barrier.  Each slot's loads (stored values of the chunk's velocities, records)
+        // are issued together before any is used.
         // Per-slot copies are loads that do not wait on the face label: the face records
         // always, the area vectors except in 3-D with FP64 storage (there they cost the
         // registers of the second wave per SIMD).  Pass A wrote the upwind value into the
@@ -909,16 +984,17 @@ __global__ void __launch_bounds__(kNTB)
             const int j = threadIdx.x + s * kNTB;
             if (j >= bi.nfs) break;
             const int own_slot = bi.fs_off + j;
+            P2<S> fv[VC];
+#pragma unroll
+            for (int vv = 0; vv < VC; ++vv)
+                fv[vv] = F[(size_t)(ck.vrow + ck.ix0 + min(vv, ck.nvv - 1)) * a.n_slots + own_slot];
             const int f = a.fs_face[own_slot];
             const d3 Sf = (D == 2 || sizeof(S) == 4) ? a.fs_sf[own_slot] : g.Sf[f];
             const EqCoef k = a.fs_coef[own_slot];
-            if (ck.new_stripe) stripe_factors<VC>(g, k, ck.ix0, a.n1, fex[s]);
-            const double eyz = row_factor(k, xr);
 #pragma unroll
             for (int vv = 0; vv < VC; ++vv) {
                 if (vv >= ck.nvv) break;
-                fl[vv * a.max_nfs + j] = slot_flux<S>(g, a, F, own_slot, f, ck.vrow + ck.ix0 + vv, Sf, k,
-                                                      fex[s][vv] * eyz, rho_b);
+                fl[vv * a.max_nfs + j] = slot_flux<S>(g, a, fv[vv], f, ck.vrow + ck.ix0 + vv, Sf, k, rho_b);
             }
         }
         {
@@ -928,14 +1004,11 @@ __global__ void __launch_bounds__(kNTB)
                 const int vv = i - q * ck.nvv;
                 const int j = FW * kNTB + q;
                 const int own_slot = bi.fs_off + j;
+                const P2<S> fv = F[(size_t)(ck.vrow + ck.ix0 + vv) * a.n_slots + own_slot];
                 const int f = a.fs_face[own_slot];
                 const d3 Sf = (D == 2 || sizeof(S) == 4) ? a.fs_sf[own_slot] : g.Sf[f];
                 const EqCoef k = a.fs_coef[own_slot];
-                // the x-factor of this velocity alone, formed as stripe_factors forms it
-                const double dx = g.xi[ck.ix0 + vv].x - k.ux;
-                const double ex = k.A * exp(-((dx * dx) * k.invRT) / 2.0);
-                fl[vv * a.max_nfs + j] = slot_flux<S>(g, a, F, own_slot, f, ck.vrow + ck.ix0 + vv, Sf, k,
-                                                      ex * row_factor(k, xr), rho_b);
+                fl[vv * a.max_nfs + j] = slot_flux<S>(g, a, fv, f, ck.vrow + ck.ix0 + vv, Sf, k, rho_b);
             }
         }
         __syncthreads();
@@ -967,8 +1040,8 @@ __global__ void __launch_bounds__(kNTB)
                 const size_t kk = (size_t)v * g.nC + cell;
                 gT[kk] = sg;
                 hT[kk] = sh;
-                const double w = g.w[v];
-                row_add(rs, w * (double)sg, w * (double)sh, g.xi[v].x);  // deviations
+                const double w = cload(g.w, v);
+                row_add(rs, w * (double)sg, w * (double)sh, cload(g.xi, v).x);  // deviations
             }
             fold_row<D>(mom, rs, xr.y, xr.z);
             if (kAhead && ch + 1 < ch1) {
@@ -1001,7 +1074,7 @@ __global__ void __launch_bounds__(kBlk) k_cell_macros(Geo g, FArgs a, Macros cm,
         for (int k = 0; k < NM; ++k) m[k] += p[k];
     }
 #pragma unroll
-    for (int k = 0; k < NM; ++k) m[k] += a.cref[k];  // the storage reference's share
+    for (int k = 0; k < NM; ++k) m[k] += cload(a.cref, k);  // the storage reference's share
     double rho, T, tau;
     d3 U, q;
     macros_from_moments<D>(g, m, *dtp, rho, U, T, tau, q);
@@ -1044,11 +1117,11 @@ __global__ void __launch_bounds__(kBlk) k_incoming(Geo g, double *bin, BndCoef *
 #pragma unroll
     for (int j = 0; j < NM; ++j) m[j] = 0.0;
     for (int v = 0; v < g.Nv; ++v) {
-        const d3 xi = g.xi[v];
+        const d3 xi = cload(g.xi, v);
         if (sigma_dot(xi, Sf) > 0) continue;
         const double cx = xi.x - k.ux, cy = xi.y - k.uy, cz = xi.z - k.uz;
         const double gv = k.A * exp(-((cx * cx + cy * cy + cz * cz) * k.invRT) / 2.0);
-        add_moments<D>(m, g.w[v] * gv, g.w[v] * (gv * k.hfac), xi);
+        add_moments<D>(m, cload(g.w, v) * gv, cload(g.w, v) * (gv * k.hfac), xi);
     }
     for (int j = 0; j < NM; ++j) bin[(size_t)b * NM + j] = m[j];
     // outgoing half of the storage reference (the passes accumulate deviations from it),
@@ -1057,11 +1130,11 @@ __global__ void __launch_bounds__(kBlk) k_incoming(Geo g, double *bin, BndCoef *
 #pragma unroll
     for (int j = 0; j < NMA; ++j) r[j] = 0.0;
     for (int v = 0; v < g.Nv; ++v) {
-        const d3 xi = g.xi[v];
+        const d3 xi = cload(g.xi, v);
         const double xs = sigma_dot(xi, Sf);
         if (!(xs > 0)) continue;
-        const double wg = g.w[v] * g.ref_g[v];
-        add_moments<D>(r, wg, g.w[v] * g.ref_h[v], xi);
+        const double wg = cload(g.w, v) * cload(g.ref_g, v);
+        add_moments<D>(r, wg, cload(g.w, v) * cload(g.ref_h, v), xi);
         if (wall) r[NMA - 1] += wg * xs;
     }
     for (int j = 0; j < NMA; ++j) bref[(size_t)b * NMA + j] = r[j];
@@ -1076,7 +1149,7 @@ __global__ void k_ref_moments(Geo g, double *cref)
     double r[NMA];
 #pragma unroll
     for (int j = 0; j < NMA; ++j) r[j] = 0.0;
-    for (int v = 0; v < g.Nv; ++v) add_moments<D>(r, g.w[v] * g.ref_g[v], g.w[v] * g.ref_h[v], g.xi[v]);
+    for (int v = 0; v < g.Nv; ++v) add_moments<D>(r, cload(g.w, v) * cload(g.ref_g, v), cload(g.w, v) * cload(g.ref_h, v), cload(g.xi, v));
     for (int j = 0; j < NMA; ++j) cref[j] = r[j];
 }
 

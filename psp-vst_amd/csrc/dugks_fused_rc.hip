@@ -152,7 +152,7 @@ __device__ inline void stripe_factors(const Geo &g, const EqCoef &k, int ix0, in
 #pragma unroll
     for (int vv = 0; vv < VC; ++vv) {
         const int ix = min(ix0 + vv, n1 - 1);
-        const double dx = g.xi[ix].x - k.ux;
+        const double dx = cload(g.xi, ix).x - k.ux;
         ex[vv] = k.A * exp(-((dx * dx) * k.invRT) / 2.0);
     }
 }
@@ -345,7 +345,7 @@ __device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &
                                  const S (&pg)[CPT][VC], const S (&ph)[CPT][VC], double (&cex)[CPT][VC],
                                  P2<S> *fb, P2<S> *ft)
 {
-    const d3 xr = g.xi[ck.vrow];
+    const d3 xr = cload(g.xi, ck.vrow);
 #pragma unroll
     for (int s = 0; s < CPT; ++s) {
         const int l = threadIdx.x + s * NT;
@@ -372,12 +372,12 @@ __device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &
             if (vv >= ck.nvv) break;
             const int v = ck.vrow + ck.ix0 + vv;
             double gEq, hEq;
-            shakhov_poly(g, k, g.xi[v], cex[s][vv] * eyz, gEq, hEq);
+            shakhov_poly(g, k, cload(g.xi, v), cex[s][vv] * eyz, gEq, hEq);
             // LDS holds f-bar-plus as deviations from the storage reference, like g~ itself
             // (FP32 storage only; FP64 storage keeps values, its reference being zero)
             if (sizeof(S) == 4) {
-                gEq -= g.ref_g[v];
-                hEq -= g.ref_h[v];
+                gEq -= cload(g.ref_g, v);
+                hEq -= cload(g.ref_h, v);
             }
             fb[vv * a.max_nl + l] = P2<S>{(S)((1.0 - af) * (double)gt[vv] + af * gEq),
                                           (S)((1.0 - af) * (double)ht[vv] + af * hEq)};
@@ -473,7 +473,7 @@ __global__ void __launch_bounds__(kNTA, kWavesA)
     int blkid, vg;
     if (!wg_map(a, blkid, vg)) return;
     extern __shared__ __align__(16) unsigned char smem[];
-    const BlockInfo bi = a.blk[blkid];
+    const BlockInfo bi = cload(a.blk, blkid);
     const int ch0 = vg * a.cpg, ch1 = min(a.nchunk, ch0 + a.cpg);
     // LDS: nbuf_a buffers of {f-bar-plus [VC][max_nl], gradients [VC][max_ng][D]}.  With two,
     // chunk c uses buffer c % 2: the next chunk's stage 1 writes the other buffer while slow
@@ -532,7 +532,7 @@ __global__ void __launch_bounds__(kNTA, kWavesA)
         __syncthreads();
         stage_grad<S, D, VC, kNTA>(a, bi, ck.nvv, fb, gr);
         __syncthreads();
-        const d3 xr = g.xi[ck.vrow];
+        const d3 xr = cload(g.xi, ck.vrow);
 #pragma unroll
         for (int s = 0; s < kSlotsA; ++s) {
             if (fid[s] < 0) continue;
@@ -541,8 +541,8 @@ __global__ void __launch_bounds__(kNTA, kWavesA)
             for (int vv = 0; vv < VC; ++vv) {
                 if (vv >= ck.nvv) break;
                 const int v = ck.vrow + ck.ix0 + vv;
-                const d3 xi = g.xi[v];
-                const double w = g.w[v];
+                const d3 xi = cload(g.xi, v);
+                const double w = cload(g.w, v);
                 const double xs = sigma_dot(xi, fS[s]);
                 const P2<S> *pf = fb + vv * a.max_nl, *pq = gr + vv * a.max_ng * D;
                 if (ftyp[s] != 0 && !(xs > 0)) continue;  // incoming half added from constants
@@ -637,7 +637,7 @@ __global__ void __launch_bounds__(kBlk)
 // pass-B face slot geometry (one work-group per block)
 __global__ void __launch_bounds__(kBlk) k_fb_geo(Geo g, FArgs a)
 {
-    const BlockInfo bi = a.blk[blockIdx.x];
+    const BlockInfo bi = cload(a.blk, blockIdx.x);
     for (int j = threadIdx.x; j < bi.nfb; j += kBlk) {
         const int f = a.fb_face[bi.fb_off + j];
         const int lo = a.fb_own[bi.fb_off + j], ln = a.fb_nei[bi.fb_off + j];
@@ -683,7 +683,7 @@ __device__ inline P2<double> face_flux(const Geo &g, const FArgs &a, const P2<S>
                                        int f, int lo, int ln, const FaceGeo &fgeo, const EqCoef &k, double gM,
                                        double hx, const double *rho_b)
 {
-    const d3 xi = g.xi[v];
+    const d3 xi = cload(g.xi, v);
     const double xs = sigma_dot(xi, fgeo.Sf);
     const int b = f - g.nIF;
     if (b >= 0 && !(xs > 0)) {  // incoming half of a wall or inlet face
@@ -715,8 +715,8 @@ __device__ inline P2<double> face_flux(const Geo &g, const FArgs &a, const P2<S>
         hv = 0.5 * ((double)rn.h + (double)ro.h);
     }
     if (sizeof(S) == 4) {  // reconstructed deviation -> value
-        gv += g.ref_g[v];
-        hv += g.ref_h[v];
+        gv += cload(g.ref_g, v);
+        hv += cload(g.ref_h, v);
     }
     double gEq, hEq;
     shakhov_poly(g, k, xi, gM, gEq, hEq);
@@ -736,7 +736,7 @@ __global__ void __launch_bounds__(kNTB, kWavesB)
     int blkid, vg;
     if (!wg_map(a, blkid, vg)) return;
     extern __shared__ __align__(16) unsigned char smem[];
-    const BlockInfo bi = a.blk[blkid];
+    const BlockInfo bi = cload(a.blk, blkid);
     const int ch0 = vg * a.cpg, ch1 = min(a.nchunk, ch0 + a.cpg);
     P2<double> *fl = (P2<double> *)smem;            // [VC][max_nfb] fluxes sigma * f_f
     P2<S> *fb = (P2<S> *)(fl + VC * a.max_nfb);   // [VC][max_nl]
@@ -797,7 +797,7 @@ __global__ void __launch_bounds__(kNTB, kWavesB)
         // FW per thread, x-factors kept for the stripe; the (face, velocity) pairs of the
         // faces beyond are dealt over all threads, so no thread takes a third whole face
         // while the others wait at the barrier
-        const d3 xr = g.xi[ck.vrow];
+        const d3 xr = cload(g.xi, ck.vrow);
 #pragma unroll
         for (int s = 0; s < FW; ++s) {
             const int j = threadIdx.x + s * kNTB;
@@ -827,7 +827,7 @@ __global__ void __launch_bounds__(kNTB, kWavesB)
                 const int f = a.fb_face[bi.fb_off + j];
                 const int lo = a.fb_own[bi.fb_off + j], ln = a.fb_nei[bi.fb_off + j];
                 // the x-factor of this velocity alone, formed as stripe_factors forms it
-                const double dx = g.xi[ck.ix0 + vv].x - k.ux;
+                const double dx = cload(g.xi, ck.ix0 + vv).x - k.ux;
                 const double ex = k.A * exp(-((dx * dx) * k.invRT) / 2.0);
                 fl[vv * a.max_nfb + j] = face_flux<S, D>(g, a, fb + vv * a.max_nl, gr + vv * a.max_ng * D,
                                                         ck.vrow + ck.ix0 + vv, f, lo, ln, fgeo, k,
@@ -866,8 +866,8 @@ __global__ void __launch_bounds__(kNTB, kWavesB)
                 const size_t kk = (size_t)v * g.nC + cell;
                 gN[kk] = sg;
                 hN[kk] = sh;
-                const double w = g.w[v];
-                add_moments<D>(mom, w * (double)sg, w * (double)sh, g.xi[v]);  // deviations
+                const double w = cload(g.w, v);
+                add_moments<D>(mom, w * (double)sg, w * (double)sh, cload(g.xi, v));  // deviations
             }
         }
         if (kAhead && ch + 1 < ch1) {
@@ -899,7 +899,7 @@ __global__ void __launch_bounds__(kBlk) k_cell_macros(Geo g, FArgs a, Macros cm,
         for (int k = 0; k < NM; ++k) m[k] += p[k];
     }
 #pragma unroll
-    for (int k = 0; k < NM; ++k) m[k] += a.cref[k];  // the storage reference's share
+    for (int k = 0; k < NM; ++k) m[k] += cload(a.cref, k);  // the storage reference's share
     double rho, T, tau;
     d3 U, q;
     macros_from_moments<D>(g, m, *dtp, rho, U, T, tau, q);
@@ -942,11 +942,11 @@ __global__ void __launch_bounds__(kBlk) k_incoming(Geo g, double *bin, BndCoef *
 #pragma unroll
     for (int j = 0; j < NM; ++j) m[j] = 0.0;
     for (int v = 0; v < g.Nv; ++v) {
-        const d3 xi = g.xi[v];
+        const d3 xi = cload(g.xi, v);
         if (sigma_dot(xi, Sf) > 0) continue;
         const double cx = xi.x - k.ux, cy = xi.y - k.uy, cz = xi.z - k.uz;
         const double gv = k.A * exp(-((cx * cx + cy * cy + cz * cz) * k.invRT) / 2.0);
-        add_moments<D>(m, g.w[v] * gv, g.w[v] * (gv * k.hfac), xi);
+        add_moments<D>(m, cload(g.w, v) * gv, cload(g.w, v) * (gv * k.hfac), xi);
     }
     for (int j = 0; j < NM; ++j) bin[(size_t)b * NM + j] = m[j];
     // outgoing half of the storage reference (the passes accumulate deviations from it),
@@ -955,11 +955,11 @@ __global__ void __launch_bounds__(kBlk) k_incoming(Geo g, double *bin, BndCoef *
 #pragma unroll
     for (int j = 0; j < NMA; ++j) r[j] = 0.0;
     for (int v = 0; v < g.Nv; ++v) {
-        const d3 xi = g.xi[v];
+        const d3 xi = cload(g.xi, v);
         const double xs = sigma_dot(xi, Sf);
         if (!(xs > 0)) continue;
-        const double wg = g.w[v] * g.ref_g[v];
-        add_moments<D>(r, wg, g.w[v] * g.ref_h[v], xi);
+        const double wg = cload(g.w, v) * cload(g.ref_g, v);
+        add_moments<D>(r, wg, cload(g.w, v) * cload(g.ref_h, v), xi);
         if (wall) r[NMA - 1] += wg * xs;
     }
     for (int j = 0; j < NMA; ++j) bref[(size_t)b * NMA + j] = r[j];
@@ -974,7 +974,7 @@ __global__ void k_ref_moments(Geo g, double *cref)
     double r[NMA];
 #pragma unroll
     for (int j = 0; j < NMA; ++j) r[j] = 0.0;
-    for (int v = 0; v < g.Nv; ++v) add_moments<D>(r, g.w[v] * g.ref_g[v], g.w[v] * g.ref_h[v], g.xi[v]);
+    for (int v = 0; v < g.Nv; ++v) add_moments<D>(r, cload(g.w, v) * cload(g.ref_g, v), cload(g.w, v) * cload(g.ref_h, v), cload(g.xi, v));
     for (int j = 0; j < NMA; ++j) cref[j] = r[j];
 }
 

@@ -34,6 +34,22 @@ __device__ inline double sigma_dot(const d3 &xi, const d3 &Sf)
 }
 __host__ __device__ inline double magsqr(const d3 &a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
 
+// p[i] read through the constant address space.  For tables no kernel writes while it runs
+// (velocity set, storage reference, block records): a wave-uniform index then becomes a
+// scalar load (one per wave, held in SGPRs) instead of a vector load per lane, which the
+// compiler cannot otherwise prove safe in kernels that also store to global memory.
+template <class T>
+__device__ inline T cload(const T *p, int i)
+{
+    static_assert(sizeof(T) % 4 == 0, "cload moves 4-byte words");
+    const __attribute__((address_space(4))) unsigned *q = (const __attribute__((address_space(4))) unsigned *)(p + i);
+    T r;
+    unsigned *d = (unsigned *)&r;
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / 4); ++k) d[k] = q[k];
+    return r;
+}
+
 // DF boundary types selected by the rho patch type (discreteVelocity.C:188-195); the
 // numbers are the dugks_patch_type codes of include/dugks.h
 enum PatchType : int {
