@@ -941,7 +941,11 @@ __global__ void __launch_bounds__(kNTB)
         }
     }
 
-    constexpr int FW = FPT - 1;  // whole faces per thread in stage 2
+    // whole faces per thread in stage 2: every slot the plan allows (FPT per thread), so that
+    // a thread issues each of its slots' loads together; the dealt (slot, velocity) items
+    // below take only what a larger block would leave (sphere pass B 101.9 -> 97.3 ms with
+    // FP32 storage against FPT - 1 whole faces plus about three dealt rounds)
+    constexpr int FW = FPT;
     // 2-D FP64: the next chunk's g~, h~ of the cell are loaded before the chunk's last
     // barrier (7.52 -> 7.42 ms on the 256^2 cavity; with FP32 storage it measured slower, and
     // 3-D has no registers to spare).  The update below writes only this chunk's velocities
@@ -971,10 +975,9 @@ __global__ void __launch_bounds__(kNTB)
             cell_bar_eq<S, VC, !kAhead>(g, k, cell, gT, hT, dt, ck, gt, ht, gb, hb);
         }
         // stage 2: face values -> flux sigma * f_f in LDS.  Slots j < FW * kNTB: whole faces,
-        // FW per thread; the (slot, velocity) pairs of the slots beyond are dealt over all
-        // threads, so no thread takes one whole face more than the others while they wait at
-        // the barrier.  Each slot's loads (stored values of the chunk's velocities, records)
-        // are issued together before any is used.
+        // FW per thread; the (slot, velocity) pairs of any slots beyond are dealt over all
+        // threads.  Each slot's loads (stored values of the chunk's velocities, records) are
+        // issued together before any is used.
         // Per-slot copies are loads that do not wait on the face label: the face records
         // always, the area vectors except in 3-D with FP64 storage (there they cost the
         // registers of the second wave per SIMD).  Pass A wrote the upwind value into the
