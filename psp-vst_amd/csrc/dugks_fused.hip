@@ -866,32 +866,42 @@ __global__ void __launch_bounds__(kBlk) k_slot_sf(Geo g, FArgs a, int n_slots)
 }
 
 // ------------------------------------------------------------------------- pass B
-// a stored face value as a value: FP32 slots hold deviations from the reference
+// what pass B reads of one velocity: xi and (FP32 storage) the storage reference.  Loaded
+// outside the branches of slot_flux, where a wave-uniform velocity makes them scalar loads.
 template <class S>
-__device__ inline double face_value(S stored, const double *ref, int v)
+struct VelK {
+    d3 xi;
+    double rg, rh;
+};
+template <class S>
+__device__ inline VelK<S> vel_k(const Geo &g, int v)
 {
-    if (sizeof(S) == 4) return (double)stored + cload(ref, v);
-    return (double)stored;
+    VelK<S> q;
+    q.xi = cload(g.xi, v);
+    q.rg = sizeof(S) == 4 ? cload(g.ref_g, v) : 0.0;
+    q.rh = sizeof(S) == 4 ? cload(g.ref_h, v) : 0.0;
+    return q;
 }
 
 // flux sigma * f of velocity v through a face slot (face f, area vector Sf, equilibrium
 // record k) whose stored value is fv: the stored face value relaxed towards the face
 // equilibrium (fvDVM.C:1362-1460), or the incoming half of a wall / inlet face.  Tangent
-// velocities of internal faces carry no flux.  The caller loads fv for every slot and velocity, incoming entries
-// included (they are not read), so that the loads of a slot are in flight together instead
-// of one behind the other under this branch.
+// velocities of internal faces carry no flux.  The caller loads fv for every slot and
+// velocity, incoming entries included (they are not read), and the velocity's constants vk,
+// so that the loads of a slot are in flight together instead of one behind the other under
+// this branch.  FP32 slots hold deviations from the storage reference.
 template <class S>
-__device__ inline P2<double> slot_flux(const Geo &g, const FArgs &a, const P2<S> fv, int f, int v, const d3 &Sf,
-                                       const EqCoef &k, const double *rho_b)
+__device__ inline P2<double> slot_flux(const Geo &g, const FArgs &a, const P2<S> fv, int f, const VelK<S> &vk,
+                                       const d3 &Sf, const EqCoef &k, const double *rho_b)
 {
-    const d3 xi = cload(g.xi, v);
+    const d3 xi = vk.xi;
     const double xs = sigma_dot(xi, Sf);
     const int b = f - g.nIF;
     if (b < 0 ? (xs >= kVSmall || xs < -kVSmall) : xs > 0) {
         double gEq, hEq;
         shakhov_eq(g, k, xi, gEq, hEq);
-        return P2<double>{xs * ((1.0 - k.tr) * face_value<S>(fv.g, g.ref_g, v) + k.tr * gEq),
-                          xs * ((1.0 - k.tr) * face_value<S>(fv.h, g.ref_h, v) + k.tr * hEq)};
+        return P2<double>{xs * ((1.0 - k.tr) * ((double)fv.g + vk.rg) + k.tr * gEq),
+                          xs * ((1.0 - k.tr) * ((double)fv.h + vk.rh) + k.tr * hEq)};
     }
     if (b >= 0) {  // incoming half of a wall or inlet face
         const BndCoef kb = a.bc[b];
@@ -997,7 +1007,8 @@ __global__ void __launch_bounds__(kNTB)
 #pragma unroll
             for (int vv = 0; vv < VC; ++vv) {
                 if (vv >= ck.nvv) break;
-                fl[vv * a.max_nfs + j] = slot_flux<S>(g, a, fv[vv], f, ck.vrow + ck.ix0 + vv, Sf, k, rho_b);
+                const VelK<S> vk = vel_k<S>(g, ck.vrow + ck.ix0 + vv);
+                fl[vv * a.max_nfs + j] = slot_flux<S>(g, a, fv[vv], f, vk, Sf, k, rho_b);
             }
         }
         {
@@ -1011,7 +1022,7 @@ __global__ void __launch_bounds__(kNTB)
                 const int f = a.fs_face[own_slot];
                 const d3 Sf = (D == 2 || sizeof(S) == 4) ? a.fs_sf[own_slot] : g.Sf[f];
                 const EqCoef k = a.fs_coef[own_slot];
-                fl[vv * a.max_nfs + j] = slot_flux<S>(g, a, fv, f, ck.vrow + ck.ix0 + vv, Sf, k, rho_b);
+                fl[vv * a.max_nfs + j] = slot_flux<S>(g, a, fv, f, vel_k<S>(g, ck.vrow + ck.ix0 + vv), Sf, k, rho_b);
             }
         }
         __syncthreads();
