@@ -79,7 +79,7 @@ struct FusedPlan {
     int n1 = 0, nrow = 0, nstripe = 0, nchunk = 0, cpg = 0;
     int vc_b = 0, nchunk_b = 0, cpg_b = 0, nvg_b = 0;  // the same for pass B (own chunk size)
     int b8 = 0;                        // blocks per XCD in the work-group mapping
-    int wg_target = 4096;              // work-groups per pass the velocity grouping aims at
+    int wg_target = 6144;              // work-groups per pass the velocity grouping aims at
     int max_nl = 0, max_nfs = 0, max_nc = 0;
     int nm = 0, nma = 0;               // raw moments per cell / per face (+ outgoing wall flux)
     size_t lds_a = 0, lds_b = 0;       // dynamic LDS bytes of the two passes
