@@ -567,9 +567,9 @@ __global__ void __launch_bounds__(kNTA)
     const int ch0 = vg * a.cpg, ch1 = min(a.nchunk, ch0 + a.cpg);
     P2<S> *fb = (P2<S> *)smem;        // [VC][max_nl]
     P2<S> *gr = fb + VC * a.max_nl;    // [VC][max_nc][D]
-    // FP64 3-D: the local cells' x-factors in LDS, [VC][max_nl] (registers are short there;
-    // the 512-thread work-group is alone on its CU, so the LDS is free)
-    constexpr bool kCexLds = D == 3 && sizeof(S) == 8;
+    // 3-D: the local cells' x-factors in LDS, [VC][max_nl] (registers are short there; the
+    // 512-thread work-group is alone on its CU, so the LDS is free)
+    constexpr bool kCexLds = D == 3;
     double *cexL = (double *)(gr + VC * a.max_nc * D);
     const double dt = *dtp;
     const double hx = 0.5 * dt;
@@ -1168,7 +1168,7 @@ __global__ void k_ref_moments(Geo g, double *cref)
 }
 
 // pass-A LDS: f-bar-plus and gradient tiles (what the budget limits), plus the x-factors
-// of FP64 3-D
+// in 3-D
 template <class S>
 size_t lds_tiles_a(const FusedPlan &p, int D, int VC)
 {
@@ -1177,17 +1177,25 @@ size_t lds_tiles_a(const FusedPlan &p, int D, int VC)
 template <class S>
 size_t lds_pass_a(const FusedPlan &p, int D, int VC)
 {
-    const size_t cex = (D == 3 && sizeof(S) == 8) ? (size_t)VC * p.max_nl * sizeof(double) : 0;
+    const size_t cex = (D == 3) ? (size_t)VC * p.max_nl * sizeof(double) : 0;
     return lds_tiles_a<S>(p, D, VC) + cex;
 }
 size_t lds_pass_b(const FusedPlan &p, int VC) { return (size_t)VC * 2 * p.max_nfs * sizeof(double); }
 inline int vc_for_b(int vc) { return std::min(vc, kFusedVcB); }
 
 constexpr size_t kLdsBudget = 80 * 1024;
+// In 3-D the 512-thread pass-A work-group is alone on its CU (2 waves per SIMD), so its
+// chunks may use most of the CU's LDS: 8 or 6 velocities per chunk (pass B keeps 4) cut the
+// per-chunk barriers and loads (211k-cell sphere pass A, FP32 storage 99.3 -> 93.8 ms with
+// 8; FP64 148.0 -> 145.0 ms with 6, the largest whose tiles fit)
+constexpr size_t kLdsBudgetA3 = 150 * 1024;
 
 template <class S>
 int choose_vc(const FusedPlan &p, int D)
 {
+    if (D == 3)
+        for (int vc : {8, 6})
+            if (lds_pass_a<S>(p, D, vc) <= kLdsBudgetA3 && lds_pass_b(p, vc_for_b(vc)) <= kLdsBudget) return vc;
     for (int vc : {4, 2, 1})
         if (lds_tiles_a<S>(p, D, vc) <= kLdsBudget && lds_pass_b(p, vc_for_b(vc)) <= kLdsBudget) return vc;
     return 0;
@@ -1223,6 +1231,9 @@ bool set_lds_attrs(int D, int vc, const FusedPlan &p)
     if (D == DD && vc_for_b(vc) == VV) ok &= set((const void *)k_pass_b<S, DD, VV>, lds_pass_b(p, VV));
     DUGKS_SET(2, 1) DUGKS_SET(2, 2) DUGKS_SET(2, 4) DUGKS_SET(3, 1) DUGKS_SET(3, 2) DUGKS_SET(3, 4)
 #undef DUGKS_SET
+    // 3-D pass A alone takes 6 or 8 velocities per chunk (pass B keeps 4)
+    if (D == 3 && vc == 6) ok &= set((const void *)k_pass_a<S, 3, 6>, lds_pass_a<S>(p, 3, 6));
+    if (D == 3 && vc == 8) ok &= set((const void *)k_pass_a<S, 3, 8>, lds_pass_a<S>(p, 3, 8));
     return ok;
 }
 
@@ -1390,7 +1401,7 @@ void Fused<S>::face_pass(const Geo &g, FusedPlan &p, const S *gT, const S *hT, c
 {
 #define DUGKS_A(DD, VV) \
     if (g.D == DD && p.vc == VV) return launch_a<S, DD, VV>(g, p, gT, hT, dt, st);
-    DUGKS_A(2, 1) DUGKS_A(2, 2) DUGKS_A(2, 4) DUGKS_A(3, 1) DUGKS_A(3, 2) DUGKS_A(3, 4)
+    DUGKS_A(2, 1) DUGKS_A(2, 2) DUGKS_A(2, 4) DUGKS_A(3, 1) DUGKS_A(3, 2) DUGKS_A(3, 4) DUGKS_A(3, 6) DUGKS_A(3, 8)
 #undef DUGKS_A
 }
 
