@@ -492,10 +492,18 @@ __device__ inline int2 grad_range(const FArgs &a, const BlockInfo &bi)
     return int2{k0, a.gs_beg[bi.gsb_off + l + 1] - k0};
 }
 
+// Entries per (cell, velocity) of the gradient tile: the gradient's D components and, with
+// FP32 storage, f* = f-bar-plus - (dt/2) xi . grad f-bar-plus, so that a face value is
+// f* + grad . (x_f - x_cell) with an arm that does not depend on the velocity.
+template <class S, int D>
+constexpr int grad_entries() { return sizeof(S) == 4 ? D + 1 : D; }
+
 template <class S, int D, int VC, int NT>
-__device__ inline void stage_grad(const FArgs &a, const BlockInfo &bi, int nvv, int2 gk, const P2<S> *fb,
-                                  P2<S> *gr)
+__device__ inline void stage_grad(const Geo &g, const FArgs &a, const BlockInfo &bi, const Chunk &ck, double hdt,
+                                  int2 gk, const P2<S> *fb, P2<S> *gr)
 {
+    constexpr int GE = grad_entries<S, D>();
+    const int nvv = ck.nvv;
     const V3<S> *ls = (const V3<S> *)a.gs_ls;
     const int l = threadIdx.x % (NT / 2);
     const int v0 = threadIdx.x / (NT / 2);
@@ -529,12 +537,35 @@ __device__ inline void stage_grad(const FArgs &a, const BlockInfo &bi, int nvv, 
                     }
                 }
             }
-            P2<S> *o = gr + (vv * a.max_nc + l) * D;
+            P2<S> *o = gr + (vv * a.max_nc + l) * GE;
             o[0] = P2<S>{gx, hx};
             o[1] = P2<S>{gy, hy};
             if (D == 3) o[2] = P2<S>{gz, hz};
+            if (GE > D) {
+                const d3 xi = cload(g.xi, ck.vrow + ck.ix0 + vv);
+                const S sx = (S)(xi.x * hdt), sy = (S)(xi.y * hdt), sz = (S)(xi.z * hdt);
+                S dg = sx * gx + sy * gy, dh = sx * hx + sy * hy;
+                if (D == 3) {
+                    dg += sz * gz;
+                    dh += sz * hz;
+                }
+                o[D] = P2<S>{c.g - dg, c.h - dh};
+            }
         }
     }
+}
+
+// g and h of local cell l at the face with arm a = x_f - x_cell: f* + grad f . a, from the
+// gradient tile with f* entries (FP32 storage)
+template <class S, int D>
+__device__ inline P2<S> recon_s(const P2<S> *gr, int l, const V3<S> &ar)
+{
+    const P2<S> *gl = gr + l * (D + 1);
+    const P2<S> f = gl[D];
+    if (D == 3)
+        return P2<S>{f.g + (gl[0].g * ar.x + gl[1].g * ar.y + gl[2].g * ar.z),
+                     f.h + (gl[0].h * ar.x + gl[1].h * ar.y + gl[2].h * ar.z)};
+    return P2<S>{f.g + (gl[0].g * ar.x + gl[1].g * ar.y), f.h + (gl[0].h * ar.x + gl[1].h * ar.y)};
 }
 
 // g and h at arm (ax, ay, az) from local cell l: f + grad f . arm
@@ -566,11 +597,12 @@ __global__ void __launch_bounds__(kNTA)
     const BlockInfo bi = cload(a.blk, blkid);
     const int ch0 = vg * a.cpg, ch1 = min(a.nchunk, ch0 + a.cpg);
     P2<S> *fb = (P2<S> *)smem;        // [VC][max_nl]
-    P2<S> *gr = fb + VC * a.max_nl;    // [VC][max_nc][D]
+    constexpr int GE = grad_entries<S, D>();
+    P2<S> *gr = fb + VC * a.max_nl;    // [VC][max_nc][GE]
     // 3-D: the local cells' x-factors in LDS, [VC][max_nl] (registers are short there; the
     // 512-thread work-group is alone on its CU, so the LDS is free)
     constexpr bool kCexLds = D == 3;
-    double *cexL = (double *)(gr + VC * a.max_nc * D);
+    double *cexL = (double *)(gr + VC * a.max_nc * GE);
     const double dt = *dtp;
     const double hx = 0.5 * dt;
 
@@ -588,6 +620,7 @@ __global__ void __launch_bounds__(kNTA)
     // shared face (-1: none)
     int fid[kSlotsA], fmeta[kSlotsA], flo[kSlotsA], fln[kSlotsA], ftyp[kSlotsA];
     d3 fS[kSlotsA], armO[kSlotsA], armN[kSlotsA];
+    V3<S> aO[kSlotsA], aN[kSlotsA];  // the arms in the storage type (FP32 storage: recon_s)
 #pragma unroll
     for (int s = 0; s < kSlotsA; ++s) {
 #pragma unroll
@@ -614,6 +647,8 @@ __global__ void __launch_bounds__(kNTA)
                 armN[s] = armO[s];
             }
         }
+        aO[s] = V3<S>{(S)armO[s].x, (S)armO[s].y, (S)armO[s].z};
+        aN[s] = V3<S>{(S)armN[s].x, (S)armN[s].y, (S)armN[s].z};
     }
 
     double cex[CPT][VC];
@@ -649,7 +684,7 @@ __global__ void __launch_bounds__(kNTA)
         stage_bar<S, VC, kNTA, CPT, D == 3, kAhead, kHoldK, kCexLds>(g, a, bi, gT, hT, dt, ck, lc, gt, ht, kc, cex,
                                                                      cexL, fb);
         __syncthreads();
-        stage_grad<S, D, VC, kNTA>(a, bi, ck.nvv, gk, fb, gr);
+        stage_grad<S, D, VC, kNTA>(g, a, bi, ck, hx, gk, fb, gr);
         __syncthreads();
         const d3 xr = cload(g.xi, ck.vrow);
 #pragma unroll
@@ -665,7 +700,7 @@ __global__ void __launch_bounds__(kNTA)
                 const d3 xi = cload(g.xi, v);
                 const double w = cload(g.w, v);
                 const double xs = sigma_dot(xi, fS[s]);
-                const P2<S> *pf = fb + vv * a.max_nl, *pq = gr + vv * a.max_nc * D;
+                const P2<S> *pf = fb + vv * a.max_nl, *pq = gr + vv * a.max_nc * GE;
                 double gf, hf;
                 bool store = true;
                 // FP32 storage: the upwind side (the owner for xs >= VSMALL and outgoing boundary
@@ -683,27 +718,23 @@ __global__ void __launch_bounds__(kNTA)
                     if (!tangent) {
                         const bool own = ftyp[s] != 0 || xs >= kVSmall;
                         if (!(own ? own_in : nei_in)) continue;
-                        const double ax = (own ? armO[s].x : armN[s].x) - xi.x * hx;
-                        const double ay = (own ? armO[s].y : armN[s].y) - xi.y * hx;
-                        const double az = (own ? armO[s].z : armN[s].z) - xi.z * hx;
-                        const P2<S> r = recon_l<S, D>(pf, pq, own ? flo[s] : fln[s], ax, ay, az);
+                        const V3<S> ar = own ? aO[s] : aN[s];
+                        const P2<S> r = recon_s<S, D>(pq, own ? flo[s] : fln[s], ar);
                         gf = (double)r.g;
                         hf = (double)r.h;
                     } else {
                         store = false;
-                        const double ox = armO[s].x - xi.x * hx, oy = armO[s].y - xi.y * hx, oz = armO[s].z - xi.z * hx;
-                        const double nx = armN[s].x - xi.x * hx, ny = armN[s].y - xi.y * hx, nz = armN[s].z - xi.z * hx;
                         if (own_in && nei_in) {
-                            const P2<S> rn = recon_l<S, D>(pf, pq, fln[s], nx, ny, nz);
-                            const P2<S> ro = recon_l<S, D>(pf, pq, flo[s], ox, oy, oz);
+                            const P2<S> rn = recon_s<S, D>(pq, fln[s], aN[s]);
+                            const P2<S> ro = recon_s<S, D>(pq, flo[s], aO[s]);
                             gf = 0.5 * ((double)rn.g + (double)ro.g);
                             hf = 0.5 * ((double)rn.h + (double)ro.h);
                         } else if (own_in) {
-                            const P2<S> r = recon_l<S, D>(pf, pq, flo[s], ox, oy, oz);
+                            const P2<S> r = recon_s<S, D>(pq, flo[s], aO[s]);
                             gf = 0.5 * (double)r.g;
                             hf = 0.5 * (double)r.h;
                         } else {
-                            const P2<S> r = recon_l<S, D>(pf, pq, fln[s], nx, ny, nz);
+                            const P2<S> r = recon_s<S, D>(pq, fln[s], aN[s]);
                             gf = 0.5 * (double)r.g;
                             hf = 0.5 * (double)r.h;
                         }
@@ -1072,6 +1103,196 @@ __global__ void __launch_bounds__(kNTB)
     }
 }
 
+// ------------------------------------------------------------------ pass B, 3-D
+// 512 threads per work-group: two threads per block cell, each taking half of the chunk's
+// velocities in stages 1 and 3, and two face slots per thread in stage 2.  What does not
+// change over the velocities is loaded once per kernel and held in registers: the cell's
+// equilibrium record and face table, each slot's face record and area vector.  The next
+// chunk's stored face values and g~, h~ are loaded while the current chunk is computed.  The
+// cell's raw moments are summed per half and the halves added in a fixed order at the end.
+constexpr int kNTB3 = 2 * kFusedThreadsB;
+constexpr int kSlotsB3 = 2;
+
+template <class S, int VC>
+__global__ void __launch_bounds__(kNTB3)
+    k_pass_b3(Geo g, FArgs a, S *gT, S *hT, const P2<S> *__restrict__ F, const double *rho_b, const double *dtp)
+{
+    constexpr int NM = 17;
+    constexpr int HV = VC / 2;
+    static_assert(VC % 2 == 0, "the two threads of a cell split the chunk's velocities");
+    int blkid, vg;
+    if (!wg_map(a, blkid, vg)) return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const BlockInfo bi = cload(a.blk, blkid);
+    const int ch0 = vg * a.cpg, ch1 = min(a.nchunk, ch0 + a.cpg);
+    // two flux tiles [VC][max_nfs] (sigma * f_f), one per chunk parity: stage 2 of the next
+    // chunk writes the other tile, so one barrier per chunk suffices
+    P2<double> *fl2 = (P2<double> *)smem;
+    const double dt = *dtp;
+
+    const int my = threadIdx.x % kNTB;
+    const int half = threadIdx.x / kNTB;  // wave-uniform
+    const bool has_cell = my < bi.nc;
+    const int cell = bi.c0 + my;
+    double mom[NM];
+#pragma unroll
+    for (int k = 0; k < NM; ++k) mom[k] = 0.0;
+    double dtV = 0.0, af = 0.0;
+    int nf = 0;
+    // the cell's faces as block slot indices (< 2^15) with the owner flag in bit 15, two per word
+    unsigned fpk[kNb / 2] = {};
+    EqCoef kc{};
+    if (has_cell) {
+        dtV = dt / g.V[cell];
+        const int k0 = a.fbc_beg[bi.fbc_off + my];
+        nf = a.fbc_beg[bi.fbc_off + my + 1] - k0;
+#pragma unroll
+        for (int j = 0; j < kNb; ++j) {
+            if (j < nf) {
+                const unsigned e = (unsigned)a.fbc_idx[k0 + j] | (a.fbc_sgn[k0 + j] > 0 ? 0x8000u : 0u);
+                fpk[j / 2] |= e << (16 * (j % 2));
+            }
+        }
+        kc = a.cc[cell];
+        af = 1.5 * dt / (2.0 * kc.tr + dt);
+    }
+    int sfc[kSlotsB3];
+    d3 sS[kSlotsB3];
+    EqCoef sk[kSlotsB3];
+#pragma unroll
+    for (int s = 0; s < kSlotsB3; ++s) {
+        const int j = threadIdx.x + s * kNTB3;
+        sfc[s] = -1;
+        sS[s] = d3{0, 0, 0};
+        sk[s] = EqCoef{};
+        if (j < bi.nfs) {
+            const int own_slot = bi.fs_off + j;
+            sfc[s] = a.fs_face[own_slot];
+            sS[s] = a.fs_sf[own_slot];
+            sk[s] = a.fs_coef[own_slot];
+        }
+    }
+
+    // next chunk's stored face values of the thread's slots and g~, h~ of its cell
+    P2<S> fn[kSlotsB3][VC];
+    S gn[HV], hn[HV];
+    auto load_next = [&](int ch) {
+        int st = -1;
+        const Chunk c = chunk_of(a, ch, VC, st);
+        if (has_cell) {
+#pragma unroll
+            for (int i = 0; i < HV; ++i) {
+                const size_t kk = (size_t)(c.vrow + c.ix0 + min(half * HV + i, c.nvv - 1)) * g.nC + cell;
+                gn[i] = gT[kk];
+                hn[i] = hT[kk];
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < kSlotsB3; ++s) {
+            if (sfc[s] < 0) continue;
+            const int own_slot = bi.fs_off + threadIdx.x + s * kNTB3;
+#pragma unroll
+            for (int vv = 0; vv < VC; ++vv)
+                fn[s][vv] = F[(size_t)(c.vrow + c.ix0 + min(vv, c.nvv - 1)) * a.n_slots + own_slot];
+        }
+    };
+    if (ch0 < ch1) load_next(ch0);
+    int stripe = -1;
+    for (int ch = ch0; ch < ch1; ++ch) {
+        const Chunk ck = chunk_of(a, ch, VC, stripe);
+        const d3 xr = cload(g.xi, ck.vrow);
+        P2<S> fv[kSlotsB3][VC];
+        S gt[HV], ht[HV];
+#pragma unroll
+        for (int s = 0; s < kSlotsB3; ++s)
+#pragma unroll
+            for (int vv = 0; vv < VC; ++vv) fv[s][vv] = fn[s][vv];
+#pragma unroll
+        for (int i = 0; i < HV; ++i) {
+            gt[i] = gn[i];
+            ht[i] = hn[i];
+        }
+        if (ch + 1 < ch1) load_next(ch + 1);
+        P2<double> *fl = fl2 + ((ch - ch0) & 1) * VC * a.max_nfs;
+        // stage 1: f-bar-plus of the thread's half of the chunk
+        double gb[HV], hb[HV];
+        if (has_cell) {
+#pragma unroll
+            for (int i = 0; i < HV; ++i) {
+                const int v = ck.vrow + ck.ix0 + min(half * HV + i, ck.nvv - 1);
+                double gEq, hEq;
+                shakhov_eq(g, kc, cload(g.xi, v), gEq, hEq);
+                if (sizeof(S) == 4) {
+                    gEq -= cload(g.ref_g, v);
+                    hEq -= cload(g.ref_h, v);
+                }
+                gb[i] = (1.0 - af) * (double)gt[i] + af * gEq;
+                hb[i] = (1.0 - af) * (double)ht[i] + af * hEq;
+            }
+        }
+        // stage 2: the thread's face slots -> flux sigma * f_f in LDS
+#pragma unroll
+        for (int s = 0; s < kSlotsB3; ++s) {
+            if (sfc[s] < 0) continue;
+            const int j = threadIdx.x + s * kNTB3;
+#pragma unroll
+            for (int vv = 0; vv < VC; ++vv) {
+                if (vv >= ck.nvv) break;
+                const VelK<S> vk = vel_k<S>(g, ck.vrow + ck.ix0 + vv);
+                fl[vv * a.max_nfs + j] = slot_flux<S>(g, a, fv[s][vv], sfc[s], vk, sS[s], sk[s], rho_b);
+            }
+        }
+        __syncthreads();
+        // stage 3: cell update of the thread's half of the chunk, in ascending face order, in place
+        if (has_cell) {
+            RowSums rs{};
+#pragma unroll
+            for (int i = 0; i < HV; ++i) {
+                const int vv = half * HV + i;
+                if (vv >= ck.nvv) break;
+                const int v = ck.vrow + ck.ix0 + vv;
+                double ag = -1.0 / 3 * (double)gt[i] + 4.0 / 3 * gb[i];
+                double ah = -1.0 / 3 * (double)ht[i] + 4.0 / 3 * hb[i];
+#pragma unroll
+                for (int j = 0; j < kNb; ++j) {
+                    if (j < nf) {
+                        const unsigned e = (fpk[j / 2] >> (16 * (j % 2))) & 0xffffu;
+                        const P2<double> q = fl[vv * a.max_nfs + (int)(e & 0x7fffu)];
+                        const double tg = q.g * dtV;
+                        const double th = q.h * dtV;
+                        if (e & 0x8000u) {
+                            ag += tg;
+                            ah += th;
+                        } else {
+                            ag -= tg;
+                            ah -= th;
+                        }
+                    }
+                }
+                const S sg = (S)ag, sh = (S)ah;
+                const size_t kk = (size_t)v * g.nC + cell;
+                gT[kk] = sg;
+                hT[kk] = sh;
+                row_add(rs, cload(g.w, v) * (double)sg, cload(g.w, v) * (double)sh, cload(g.xi, v).x);
+            }
+            fold_row<3>(mom, rs, xr.y, xr.z);
+        }
+    }
+    __syncthreads();  // every thread is past its last flux-tile read
+    // the second half's moments through LDS, added to the first half's in a fixed order
+    double *mx = (double *)smem;  // [NM][kNTB]
+    if (half == 1 && has_cell) {
+#pragma unroll
+        for (int k = 0; k < NM; ++k) mx[k * kNTB + my] = mom[k];
+    }
+    __syncthreads();
+    if (half == 0 && has_cell) {
+        double *o = a.cpart + ((size_t)vg * g.nC + cell) * NM;
+#pragma unroll
+        for (int k = 0; k < NM; ++k) o[k] = mom[k] + mx[k * kNTB + my];
+    }
+}
+
 // ------------------------------------------------------------------- cell macros
 template <int D>
 __global__ void __launch_bounds__(kBlk) k_cell_macros(Geo g, FArgs a, Macros cm, const double *dtp)
@@ -1172,7 +1393,8 @@ __global__ void k_ref_moments(Geo g, double *cref)
 template <class S>
 size_t lds_tiles_a(const FusedPlan &p, int D, int VC)
 {
-    return (size_t)VC * (2 * p.max_nl + 2 * p.max_nc * D) * sizeof(S);
+    const int ge = sizeof(S) == 4 ? D + 1 : D;
+    return (size_t)VC * (2 * p.max_nl + 2 * p.max_nc * ge) * sizeof(S);
 }
 template <class S>
 size_t lds_pass_a(const FusedPlan &p, int D, int VC)
@@ -1181,6 +1403,11 @@ size_t lds_pass_a(const FusedPlan &p, int D, int VC)
     return lds_tiles_a<S>(p, D, VC) + cex;
 }
 size_t lds_pass_b(const FusedPlan &p, int VC) { return (size_t)VC * 2 * p.max_nfs * sizeof(double); }
+// 3-D pass B: two flux tiles, which the two halves' moment exchange reuses at the end
+size_t lds_pass_b3(const FusedPlan &p, int VC)
+{
+    return std::max(2 * lds_pass_b(p, VC), (size_t)17 * kFusedThreadsB * sizeof(double));
+}
 inline int vc_for_b(int vc) { return std::min(vc, kFusedVcB); }
 
 constexpr size_t kLdsBudget = 80 * 1024;
@@ -1188,7 +1415,7 @@ constexpr size_t kLdsBudget = 80 * 1024;
 // chunks may use most of the CU's LDS: 8 or 6 velocities per chunk (pass B keeps 4) cut the
 // per-chunk barriers and loads (211k-cell sphere pass A, FP32 storage 99.3 -> 93.8 ms with
 // 8; FP64 148.0 -> 145.0 ms with 6, the largest whose tiles fit)
-constexpr size_t kLdsBudgetA3 = 150 * 1024;
+constexpr size_t kLdsBudgetA3 = 160 * 1024;
 
 template <class S>
 int choose_vc(const FusedPlan &p, int D)
@@ -1219,6 +1446,14 @@ void launch_b(const Geo &g, FusedPlan &p, S *gT, S *hT, const double *rho_b, con
                        (const P2<S> *)p.F, rho_b, dt);
 }
 
+template <class S, int VC>
+void launch_b3(const Geo &g, FusedPlan &p, S *gT, S *hT, const double *rho_b, const double *dt, hipStream_t st)
+{
+    const size_t lds = lds_pass_b3(p, VC);
+    hipLaunchKernelGGL((k_pass_b3<S, VC>), dim3(grid_b(p)), dim3(kNTB3), lds, st, g, fargs_b(p), gT, hT,
+                       (const P2<S> *)p.F, rho_b, dt);
+}
+
 template <class S>
 bool set_lds_attrs(int D, int vc, const FusedPlan &p)
 {
@@ -1234,6 +1469,7 @@ bool set_lds_attrs(int D, int vc, const FusedPlan &p)
     // 3-D pass A alone takes 6 or 8 velocities per chunk (pass B keeps 4)
     if (D == 3 && vc == 6) ok &= set((const void *)k_pass_a<S, 3, 6>, lds_pass_a<S>(p, 3, 6));
     if (D == 3 && vc == 8) ok &= set((const void *)k_pass_a<S, 3, 8>, lds_pass_a<S>(p, 3, 8));
+    if (D == 3 && vc_for_b(vc) == 4) ok &= set((const void *)k_pass_b3<S, 4>, lds_pass_b3(p, 4));
     return ok;
 }
 
@@ -1423,6 +1659,8 @@ template <class S>
 void Fused<S>::update_pass(const Geo &g, FusedPlan &p, S *gT, S *hT, const double *rho_b, const double *dt,
                            hipStream_t st)
 {
+    if (sizeof(S) == 4 && g.D == 3 && p.vc_b == 4 && p.max_nfs <= kSlotsB3 * kNTB3)
+        return launch_b3<S, 4>(g, p, gT, hT, rho_b, dt, st);
 #define DUGKS_B(DD, VV) \
     if (g.D == DD && p.vc_b == VV) return launch_b<S, DD, VV>(g, p, gT, hT, rho_b, dt, st);
     DUGKS_B(2, 1) DUGKS_B(2, 2) DUGKS_B(2, 4) DUGKS_B(3, 1) DUGKS_B(3, 2) DUGKS_B(3, 4)
