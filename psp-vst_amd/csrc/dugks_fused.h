@@ -104,6 +104,9 @@ struct FusedPlan {
     // face values [Nv][slots], (g, h) pairs: written by the block that reconstructs them into
     // its own slot
     void *F = nullptr;
+    // 3-D with FP32 storage: u = 4/3 f-bar-plus - 1/3 g~ of every cell, [Nv][nC] (g, h) pairs,
+    // written by pass A and read by pass B instead of g~ (null on the other paths)
+    void *U = nullptr;
     int n_slots = 0;
     d3 *fs_sf = nullptr;                  // [slots] face area vectors, formed at upload
     EqCoef *fs_coef = nullptr;            // [slots] face records, copied each step

@@ -59,6 +59,28 @@ struct alignas(2 * sizeof(S)) P2 {
     S g, h;
 };
 
+
+// Streamed (g, h) pairs: face values and u are written once per step and read once by the
+// other pass, and the new g~, h~ once by the next step, so their loads and stores carry the
+// non-temporal hint and leave the L2 to the halo, record and table lines that are re-read
+// (sphere, FP32 storage: pass A 94.3 -> 90.1 ms, pass B 64.4 -> 59.9 ms)
+template <class S>
+__device__ inline P2<S> ld_nt(const P2<S> *p)
+{
+    typedef S v2 __attribute__((ext_vector_type(2)));
+    const v2 x = __builtin_nontemporal_load((const v2 *)p);
+    return P2<S>{x.x, x.y};
+}
+template <class S>
+__device__ inline void st_nt(P2<S> *p, const P2<S> v)
+{
+    typedef S v2 __attribute__((ext_vector_type(2)));
+    v2 x;
+    x.x = v.g;
+    x.y = v.h;
+    __builtin_nontemporal_store(x, (v2 *)p);
+}
+
 struct FArgs {
     const BlockInfo *blk;
     const int *lcell, *gs_beg, *gs_nb, *fs_face, *fs_twin, *fs_lo, *fs_ln, *fbc_beg, *fbc_idx;
@@ -442,11 +464,13 @@ __device__ inline void cell_bar_eq(const Geo &g, const EqCoef &k, int c, const S
 // HK: the cells' equilibrium records are held in kc for the kernel, else loaded per chunk
 // CL: the x-factors live in LDS, cexL[vv * max_nl + l] (each thread touches only its own
 // cells' entries), else in cex
+// With U, the block cells also write u = 4/3 f-bar-plus - 1/3 g~ (and the same of h), the
+// part of the cell update that needs no fluxes, for pass B.
 template <class S, int VC, int NT, int CPT, bool D3, bool PRE, bool HK, bool CL>
 __device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &bi, const S *__restrict__ gT,
                                  const S *__restrict__ hT, double dt, const Chunk &ck, const int (&lc)[CPT],
                                  const S (&pg)[CPT][VC], const S (&ph)[CPT][VC], const EqCoef (&kc)[CPT],
-                                 double (&cex)[CPT][VC], double *cexL, P2<S> *fb)
+                                 double (&cex)[CPT][VC], double *cexL, P2<S> *fb, P2<S> *__restrict__ U)
 {
     const d3 xr = cload(g.xi, ck.vrow);
 #pragma unroll
@@ -472,10 +496,15 @@ __device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &
             const EqCoef k = a.cc[c];
             cell_bar<S, VC, !PRE>(g, a, k, c, gT, hT, dt, ck, xr, cx, gt, ht, gb, hb, cs);
         }
+        const bool wu = U != nullptr && l < bi.nc;
 #pragma unroll
         for (int vv = 0; vv < VC; ++vv) {
             if (vv >= ck.nvv) break;
             fb[vv * a.max_nl + l] = P2<S>{(S)gb[vv], (S)hb[vv]};
+            if (wu)
+                st_nt(U + (size_t)(ck.vrow + ck.ix0 + vv) * g.nC + c,
+                      P2<S>{(S)(-1.0 / 3 * (double)gt[vv] + 4.0 / 3 * gb[vv]),
+                            (S)(-1.0 / 3 * (double)ht[vv] + 4.0 / 3 * hb[vv])});
         }
     }
 }
@@ -587,7 +616,7 @@ __device__ inline P2<S> recon_l(const P2<S> *fb, const P2<S> *gr, int l, double 
 template <class S, int D, int VC>
 __global__ void __launch_bounds__(kNTA)
     k_pass_a(Geo g, FArgs a, const S *__restrict__ gT, const S *__restrict__ hT, P2<S> *__restrict__ F,
-             const double *dtp)
+             P2<S> *__restrict__ U, const double *dtp)
 {
     constexpr int NMA = (D == 3 ? 17 : 11) + 1;
     constexpr int CPT = fused_cells_per_thread_a(D);
@@ -682,7 +711,7 @@ __global__ void __launch_bounds__(kNTA)
     for (int ch = ch0; ch < ch1; ++ch) {
         const Chunk ck = chunk_of(a, ch, VC, stripe);
         stage_bar<S, VC, kNTA, CPT, D == 3, kAhead, kHoldK, kCexLds>(g, a, bi, gT, hT, dt, ck, lc, gt, ht, kc, cex,
-                                                                     cexL, fb);
+                                                                     cexL, fb, U);
         __syncthreads();
         stage_grad<S, D, VC, kNTA>(g, a, bi, ck, hx, gk, fb, gr);
         __syncthreads();
@@ -694,6 +723,8 @@ __global__ void __launch_bounds__(kNTA)
             const bool own_in = fmeta[s] & kSlotOwnIn, nei_in = fmeta[s] & kSlotNeiIn;
             const bool split = s == 1 && items;
             const int vb = split ? vo : 0, ve = split ? min(vo + 1, ck.nvv) : ck.nvv;
+            // one velocity per iteration: unrolling by two overlaps the loads of two velocities
+            // but spills once the block cells also write u (sphere pass A 94.3 -> 97.8 ms)
 #pragma unroll 1
             for (int vv = vb; vv < ve; ++vv) {
                 const int v = ck.vrow + ck.ix0 + vv;
@@ -782,9 +813,9 @@ __global__ void __launch_bounds__(kNTA)
                     // into this block's slot and, for a face shared with an adjacent block,
                     // into that block's slot too: pass B then reads its own slots only
                     const P2<S> val{(S)gf, (S)hf};
-                    F[(size_t)v * a.n_slots + (bi.fs_off + slot_of(s))] = val;  // own slot
+                    st_nt(F + (size_t)v * a.n_slots + (bi.fs_off + slot_of(s)), val);  // own slot
                     const int twin = (fmeta[s] >> 2) - 1;
-                    if (twin >= 0) F[(size_t)v * a.n_slots + twin] = val;
+                    if (twin >= 0) st_nt(F + (size_t)v * a.n_slots + twin, val);
                 }
                 if (ftyp[s] == kMaxwellWall) acc[s][NMA - 1] += w * xs * gf;
                 row_add(rs, w * gf, w * hf, xi.x);
@@ -1031,7 +1062,7 @@ __global__ void __launch_bounds__(kNTB)
             P2<S> fv[VC];
 #pragma unroll
             for (int vv = 0; vv < VC; ++vv)
-                fv[vv] = F[(size_t)(ck.vrow + ck.ix0 + min(vv, ck.nvv - 1)) * a.n_slots + own_slot];
+                fv[vv] = ld_nt(F + (size_t)(ck.vrow + ck.ix0 + min(vv, ck.nvv - 1)) * a.n_slots + own_slot);
             const int f = a.fs_face[own_slot];
             const d3 Sf = (D == 2 || sizeof(S) == 4) ? a.fs_sf[own_slot] : g.Sf[f];
             const EqCoef k = a.fs_coef[own_slot];
@@ -1049,7 +1080,7 @@ __global__ void __launch_bounds__(kNTB)
                 const int vv = i - q * ck.nvv;
                 const int j = FW * kNTB + q;
                 const int own_slot = bi.fs_off + j;
-                const P2<S> fv = F[(size_t)(ck.vrow + ck.ix0 + vv) * a.n_slots + own_slot];
+                const P2<S> fv = ld_nt(F + (size_t)(ck.vrow + ck.ix0 + vv) * a.n_slots + own_slot);
                 const int f = a.fs_face[own_slot];
                 const d3 Sf = (D == 2 || sizeof(S) == 4) ? a.fs_sf[own_slot] : g.Sf[f];
                 const EqCoef k = a.fs_coef[own_slot];
@@ -1083,8 +1114,8 @@ __global__ void __launch_bounds__(kNTB)
                 }
                 const S sg = (S)ag, sh = (S)ah;
                 const size_t kk = (size_t)v * g.nC + cell;
-                gT[kk] = sg;
-                hT[kk] = sh;
+                __builtin_nontemporal_store(sg, gT + kk);
+                __builtin_nontemporal_store(sh, hT + kk);
                 const double w = cload(g.w, v);
                 row_add(rs, w * (double)sg, w * (double)sh, cload(g.xi, v).x);  // deviations
             }
@@ -1105,17 +1136,20 @@ __global__ void __launch_bounds__(kNTB)
 
 // ------------------------------------------------------------------ pass B, 3-D
 // 512 threads per work-group: two threads per block cell, each taking half of the chunk's
-// velocities in stages 1 and 3, and two face slots per thread in stage 2.  What does not
-// change over the velocities is loaded once per kernel and held in registers: the cell's
-// equilibrium record and face table, each slot's face record and area vector.  The next
-// chunk's stored face values and g~, h~ are loaded while the current chunk is computed.  The
-// cell's raw moments are summed per half and the halves added in a fixed order at the end.
+// velocities in the update, and two face slots per thread for the fluxes.  Pass A wrote
+// u = 4/3 f-bar-plus - 1/3 g~ of every block cell (U), so the update is g~ <- u - dt/V
+// sum(+-flux) and this pass needs no cell equilibrium.  What does not change over the
+// velocities is loaded once per kernel and held in registers: the cell's face table, each
+// slot's face record and area vector.  The next chunk's stored face values and u are loaded
+// while the current chunk is computed.  The cell's raw moments are summed per half and the
+// halves added in a fixed order at the end.
 constexpr int kNTB3 = 2 * kFusedThreadsB;
 constexpr int kSlotsB3 = 2;
 
 template <class S, int VC>
 __global__ void __launch_bounds__(kNTB3)
-    k_pass_b3(Geo g, FArgs a, S *gT, S *hT, const P2<S> *__restrict__ F, const double *rho_b, const double *dtp)
+    k_pass_b3(Geo g, FArgs a, S *gT, S *hT, const P2<S> *__restrict__ F, const P2<S> *__restrict__ U,
+              const double *rho_b, const double *dtp)
 {
     constexpr int NM = 17;
     constexpr int HV = VC / 2;
@@ -1137,11 +1171,10 @@ __global__ void __launch_bounds__(kNTB3)
     double mom[NM];
 #pragma unroll
     for (int k = 0; k < NM; ++k) mom[k] = 0.0;
-    double dtV = 0.0, af = 0.0;
+    double dtV = 0.0;
     int nf = 0;
     // the cell's faces as block slot indices (< 2^15) with the owner flag in bit 15, two per word
     unsigned fpk[kNb / 2] = {};
-    EqCoef kc{};
     if (has_cell) {
         dtV = dt / g.V[cell];
         const int k0 = a.fbc_beg[bi.fbc_off + my];
@@ -1153,8 +1186,6 @@ __global__ void __launch_bounds__(kNTB3)
                 fpk[j / 2] |= e << (16 * (j % 2));
             }
         }
-        kc = a.cc[cell];
-        af = 1.5 * dt / (2.0 * kc.tr + dt);
     }
     int sfc[kSlotsB3];
     d3 sS[kSlotsB3];
@@ -1173,19 +1204,16 @@ __global__ void __launch_bounds__(kNTB3)
         }
     }
 
-    // next chunk's stored face values of the thread's slots and g~, h~ of its cell
+    // next chunk's stored face values of the thread's slots and u of its cell
     P2<S> fn[kSlotsB3][VC];
-    S gn[HV], hn[HV];
+    P2<S> un[HV];
     auto load_next = [&](int ch) {
         int st = -1;
         const Chunk c = chunk_of(a, ch, VC, st);
         if (has_cell) {
 #pragma unroll
-            for (int i = 0; i < HV; ++i) {
-                const size_t kk = (size_t)(c.vrow + c.ix0 + min(half * HV + i, c.nvv - 1)) * g.nC + cell;
-                gn[i] = gT[kk];
-                hn[i] = hT[kk];
-            }
+            for (int i = 0; i < HV; ++i)
+                un[i] = ld_nt(U + (size_t)(c.vrow + c.ix0 + min(half * HV + i, c.nvv - 1)) * g.nC + cell);
         }
 #pragma unroll
         for (int s = 0; s < kSlotsB3; ++s) {
@@ -1193,7 +1221,7 @@ __global__ void __launch_bounds__(kNTB3)
             const int own_slot = bi.fs_off + threadIdx.x + s * kNTB3;
 #pragma unroll
             for (int vv = 0; vv < VC; ++vv)
-                fn[s][vv] = F[(size_t)(c.vrow + c.ix0 + min(vv, c.nvv - 1)) * a.n_slots + own_slot];
+                fn[s][vv] = ld_nt(F + (size_t)(c.vrow + c.ix0 + min(vv, c.nvv - 1)) * a.n_slots + own_slot);
         }
     };
     if (ch0 < ch1) load_next(ch0);
@@ -1202,35 +1230,16 @@ __global__ void __launch_bounds__(kNTB3)
         const Chunk ck = chunk_of(a, ch, VC, stripe);
         const d3 xr = cload(g.xi, ck.vrow);
         P2<S> fv[kSlotsB3][VC];
-        S gt[HV], ht[HV];
+        P2<S> ut[HV];
 #pragma unroll
         for (int s = 0; s < kSlotsB3; ++s)
 #pragma unroll
             for (int vv = 0; vv < VC; ++vv) fv[s][vv] = fn[s][vv];
 #pragma unroll
-        for (int i = 0; i < HV; ++i) {
-            gt[i] = gn[i];
-            ht[i] = hn[i];
-        }
+        for (int i = 0; i < HV; ++i) ut[i] = un[i];
         if (ch + 1 < ch1) load_next(ch + 1);
         P2<double> *fl = fl2 + ((ch - ch0) & 1) * VC * a.max_nfs;
-        // stage 1: f-bar-plus of the thread's half of the chunk
-        double gb[HV], hb[HV];
-        if (has_cell) {
-#pragma unroll
-            for (int i = 0; i < HV; ++i) {
-                const int v = ck.vrow + ck.ix0 + min(half * HV + i, ck.nvv - 1);
-                double gEq, hEq;
-                shakhov_eq(g, kc, cload(g.xi, v), gEq, hEq);
-                if (sizeof(S) == 4) {
-                    gEq -= cload(g.ref_g, v);
-                    hEq -= cload(g.ref_h, v);
-                }
-                gb[i] = (1.0 - af) * (double)gt[i] + af * gEq;
-                hb[i] = (1.0 - af) * (double)ht[i] + af * hEq;
-            }
-        }
-        // stage 2: the thread's face slots -> flux sigma * f_f in LDS
+        // the thread's face slots -> flux sigma * f_f in LDS
 #pragma unroll
         for (int s = 0; s < kSlotsB3; ++s) {
             if (sfc[s] < 0) continue;
@@ -1243,7 +1252,7 @@ __global__ void __launch_bounds__(kNTB3)
             }
         }
         __syncthreads();
-        // stage 3: cell update of the thread's half of the chunk, in ascending face order, in place
+        // cell update of the thread's half of the chunk, in ascending face order, in place
         if (has_cell) {
             RowSums rs{};
 #pragma unroll
@@ -1251,8 +1260,8 @@ __global__ void __launch_bounds__(kNTB3)
                 const int vv = half * HV + i;
                 if (vv >= ck.nvv) break;
                 const int v = ck.vrow + ck.ix0 + vv;
-                double ag = -1.0 / 3 * (double)gt[i] + 4.0 / 3 * gb[i];
-                double ah = -1.0 / 3 * (double)ht[i] + 4.0 / 3 * hb[i];
+                double ag = (double)ut[i].g;
+                double ah = (double)ut[i].h;
 #pragma unroll
                 for (int j = 0; j < kNb; ++j) {
                     if (j < nf) {
@@ -1271,8 +1280,8 @@ __global__ void __launch_bounds__(kNTB3)
                 }
                 const S sg = (S)ag, sh = (S)ah;
                 const size_t kk = (size_t)v * g.nC + cell;
-                gT[kk] = sg;
-                hT[kk] = sh;
+                __builtin_nontemporal_store(sg, gT + kk);
+                __builtin_nontemporal_store(sh, hT + kk);
                 row_add(rs, cload(g.w, v) * (double)sg, cload(g.w, v) * (double)sh, cload(g.xi, v).x);
             }
             fold_row<3>(mom, rs, xr.y, xr.z);
@@ -1435,7 +1444,8 @@ template <class S, int D, int VC>
 void launch_a(const Geo &g, FusedPlan &p, const S *gT, const S *hT, const double *dt, hipStream_t st)
 {
     const size_t lds = lds_pass_a<S>(p, D, VC);
-    hipLaunchKernelGGL((k_pass_a<S, D, VC>), dim3(grid_a(p)), dim3(kNTA), lds, st, g, fargs(p), gT, hT, (P2<S> *)p.F, dt);
+    hipLaunchKernelGGL((k_pass_a<S, D, VC>), dim3(grid_a(p)), dim3(kNTA), lds, st, g, fargs(p), gT, hT, (P2<S> *)p.F,
+                       (P2<S> *)p.U, dt);
 }
 
 template <class S, int D, int VC>
@@ -1451,7 +1461,7 @@ void launch_b3(const Geo &g, FusedPlan &p, S *gT, S *hT, const double *rho_b, co
 {
     const size_t lds = lds_pass_b3(p, VC);
     hipLaunchKernelGGL((k_pass_b3<S, VC>), dim3(grid_b(p)), dim3(kNTB3), lds, st, g, fargs_b(p), gT, hT,
-                       (const P2<S> *)p.F, rho_b, dt);
+                       (const P2<S> *)p.F, (const P2<S> *)p.U, rho_b, dt);
 }
 
 template <class S>
@@ -1576,6 +1586,12 @@ bool fused_plan_upload(FusedPlan &p, const Geo &g, size_t es, std::string &err)
         !dbuf(p, &fc, (size_t)g.nF * sizeof(EqCoef), err) || !dbuf(p, &bc, (size_t)g.nB * sizeof(BndCoef), err) ||
         !dbuf(p, &p.F, (size_t)g.Nv * p.n_slots * 2 * es, err))
         return false;
+    // 3-D with FP32 storage: pass A hands the flux-free part of the cell update to pass B
+    // (k_pass_b3) through U
+    p.U = nullptr;
+    if (es == 4 && g.D == 3 && p.vc_b == 4 && p.max_nfs <= kSlotsB3 * kNTB3 &&
+        !dbuf(p, &p.U, (size_t)g.Nv * g.nC * 2 * es, err))
+        return false;
     p.fpart = (double *)fp;
     p.cpart = (double *)cp;
     p.bin = (double *)bi;
@@ -1659,8 +1675,7 @@ template <class S>
 void Fused<S>::update_pass(const Geo &g, FusedPlan &p, S *gT, S *hT, const double *rho_b, const double *dt,
                            hipStream_t st)
 {
-    if (sizeof(S) == 4 && g.D == 3 && p.vc_b == 4 && p.max_nfs <= kSlotsB3 * kNTB3)
-        return launch_b3<S, 4>(g, p, gT, hT, rho_b, dt, st);
+    if (p.U != nullptr) return launch_b3<S, 4>(g, p, gT, hT, rho_b, dt, st);
 #define DUGKS_B(DD, VV) \
     if (g.D == DD && p.vc_b == VV) return launch_b<S, DD, VV>(g, p, gT, hT, rho_b, dt, st);
     DUGKS_B(2, 1) DUGKS_B(2, 2) DUGKS_B(2, 4) DUGKS_B(3, 1) DUGKS_B(3, 2) DUGKS_B(3, 4)
