@@ -566,10 +566,13 @@ __device__ inline void stage_grad(const Geo &g, const FArgs &a, const BlockInfo 
                     }
                 }
             }
-            P2<S> *o = gr + (vv * a.max_nc + l) * GE;
+            // component-major tile, [vv][component][kNTB]: consecutive cells in consecutive
+            // words, so neither these writes nor the gathers of stage 3 pile up on a few LDS
+            // banks; the fixed stride keeps the component offsets in the instructions
+            P2<S> *o = gr + vv * GE * kNTB + l;
             o[0] = P2<S>{gx, hx};
-            o[1] = P2<S>{gy, hy};
-            if (D == 3) o[2] = P2<S>{gz, hz};
+            o[kNTB] = P2<S>{gy, hy};
+            if (D == 3) o[2 * kNTB] = P2<S>{gz, hz};
             if (GE > D) {
                 const d3 xi = cload(g.xi, ck.vrow + ck.ix0 + vv);
                 const S sx = (S)(xi.x * hdt), sy = (S)(xi.y * hdt), sz = (S)(xi.z * hdt);
@@ -578,7 +581,7 @@ __device__ inline void stage_grad(const Geo &g, const FArgs &a, const BlockInfo 
                     dg += sz * gz;
                     dh += sz * hz;
                 }
-                o[D] = P2<S>{c.g - dg, c.h - dh};
+                o[D * kNTB] = P2<S>{c.g - dg, c.h - dh};
             }
         }
     }
@@ -589,27 +592,30 @@ __device__ inline void stage_grad(const Geo &g, const FArgs &a, const BlockInfo 
 template <class S, int D>
 __device__ inline P2<S> recon_s(const P2<S> *gr, int l, const V3<S> &ar)
 {
-    const P2<S> *gl = gr + l * (D + 1);
-    const P2<S> f = gl[D];
-    if (D == 3)
-        return P2<S>{f.g + (gl[0].g * ar.x + gl[1].g * ar.y + gl[2].g * ar.z),
-                     f.h + (gl[0].h * ar.x + gl[1].h * ar.y + gl[2].h * ar.z)};
-    return P2<S>{f.g + (gl[0].g * ar.x + gl[1].g * ar.y), f.h + (gl[0].h * ar.x + gl[1].h * ar.y)};
+    constexpr int nc = kNTB;
+    const P2<S> *gl = gr + l;
+    const P2<S> f = gl[D * nc], g0 = gl[0], g1 = gl[nc];
+    if (D == 3) {
+        const P2<S> g2 = gl[2 * nc];
+        return P2<S>{f.g + (g0.g * ar.x + g1.g * ar.y + g2.g * ar.z), f.h + (g0.h * ar.x + g1.h * ar.y + g2.h * ar.z)};
+    }
+    return P2<S>{f.g + (g0.g * ar.x + g1.g * ar.y), f.h + (g0.h * ar.x + g1.h * ar.y)};
 }
 
 // g and h at arm (ax, ay, az) from local cell l: f + grad f . arm
 template <class S, int D>
 __device__ inline P2<S> recon_l(const P2<S> *fb, const P2<S> *gr, int l, double ax, double ay, double az)
 {
-    const P2<S> *gl = gr + l * D;
+    constexpr int nc = kNTB;
+    const P2<S> *gl = gr + l;
     const P2<S> f = fb[l];
     const S sx = (S)ax, sy = (S)ay;
     if (D == 3) {
         const S sz = (S)az;
-        return P2<S>{f.g + (gl[0].g * sx + gl[1].g * sy + gl[2].g * sz),
-                     f.h + (gl[0].h * sx + gl[1].h * sy + gl[2].h * sz)};
+        return P2<S>{f.g + (gl[0].g * sx + gl[nc].g * sy + gl[2 * nc].g * sz),
+                     f.h + (gl[0].h * sx + gl[nc].h * sy + gl[2 * nc].h * sz)};
     }
-    return P2<S>{f.g + (gl[0].g * sx + gl[1].g * sy), f.h + (gl[0].h * sx + gl[1].h * sy)};
+    return P2<S>{f.g + (gl[0].g * sx + gl[nc].g * sy), f.h + (gl[0].h * sx + gl[nc].h * sy)};
 }
 
 // ------------------------------------------------------------------------- pass A
@@ -627,11 +633,11 @@ __global__ void __launch_bounds__(kNTA)
     const int ch0 = vg * a.cpg, ch1 = min(a.nchunk, ch0 + a.cpg);
     P2<S> *fb = (P2<S> *)smem;        // [VC][max_nl]
     constexpr int GE = grad_entries<S, D>();
-    P2<S> *gr = fb + VC * a.max_nl;    // [VC][max_nc][GE]
+    P2<S> *gr = fb + VC * a.max_nl;    // [VC][GE][kNTB]
     // 3-D: the local cells' x-factors in LDS, [VC][max_nl] (registers are short there; the
     // 512-thread work-group is alone on its CU, so the LDS is free)
     constexpr bool kCexLds = D == 3;
-    double *cexL = (double *)(gr + VC * a.max_nc * GE);
+    double *cexL = (double *)(gr + VC * kNTB * GE);
     const double dt = *dtp;
     const double hx = 0.5 * dt;
 
@@ -731,7 +737,7 @@ __global__ void __launch_bounds__(kNTA)
                 const d3 xi = cload(g.xi, v);
                 const double w = cload(g.w, v);
                 const double xs = sigma_dot(xi, fS[s]);
-                const P2<S> *pf = fb + vv * a.max_nl, *pq = gr + vv * a.max_nc * GE;
+                const P2<S> *pf = fb + vv * a.max_nl, *pq = gr + vv * kNTB * GE;
                 double gf, hf;
                 bool store = true;
                 // FP32 storage: the upwind side (the owner for xs >= VSMALL and outgoing boundary
@@ -975,6 +981,30 @@ __device__ inline P2<double> slot_flux(const Geo &g, const FArgs &a, const P2<S>
     return P2<double>{0.0, 0.0};
 }
 
+// slot_flux with the face Maxwellian factor gM = A exp(-|xi - U|^2 / 2RT) given by the caller
+template <class S>
+__device__ inline P2<double> slot_flux_m(const Geo &g, const FArgs &a, const P2<S> fv, int f, const VelK<S> &vk,
+                                         const d3 &Sf, const EqCoef &k, double gM, const double *rho_b)
+{
+    const d3 xi = vk.xi;
+    const double xs = sigma_dot(xi, Sf);
+    const int b = f - g.nIF;
+    if (b < 0 ? (xs >= kVSmall || xs < -kVSmall) : xs > 0) {
+        double gEq, hEq;
+        shakhov_poly(g, k, xi, gM, gEq, hEq);
+        return P2<double>{xs * ((1.0 - k.tr) * ((double)fv.g + vk.rg) + k.tr * gEq),
+                          xs * ((1.0 - k.tr) * ((double)fv.h + vk.rh) + k.tr * hEq)};
+    }
+    if (b >= 0) {  // incoming half of a wall or inlet face
+        const BndCoef kb = a.bc[b];
+        const double scale = g.bf_type[b] == kMaxwellWall ? rho_b[b] : 1.0;
+        const double cx = xi.x - kb.ux, cy = xi.y - kb.uy, cz = xi.z - kb.uz;
+        const double gv = scale * (kb.A * exp_cw(-0.5 * ((cx * cx + cy * cy + cz * cz) * kb.invRT)));
+        return P2<double>{xs * gv, xs * (gv * kb.hfac)};
+    }
+    return P2<double>{0.0, 0.0};
+}
+
 template <class S, int D, int VC>
 __global__ void __launch_bounds__(kNTB)
     k_pass_b(Geo g, FArgs a, S *gT, S *hT, const P2<S> *__restrict__ F, const double *rho_b,
@@ -1162,6 +1192,9 @@ __global__ void __launch_bounds__(kNTB3)
     // two flux tiles [VC][max_nfs] (sigma * f_f), one per chunk parity: stage 2 of the next
     // chunk writes the other tile, so one barrier per chunk suffices
     P2<double> *fl2 = (P2<double> *)smem;
+    // the slots' Maxwellian x-factors A exp(-(xi_x - U_x)^2 / 2RT) of the current stripe,
+    // [VC][max_nfs]; each thread reads only the entries it wrote (pass B 59.8 -> 57.9 ms)
+    double *exs = (double *)(fl2 + 2 * VC * a.max_nfs);
     const double dt = *dtp;
 
     const int my = threadIdx.x % kNTB;
@@ -1239,16 +1272,20 @@ __global__ void __launch_bounds__(kNTB3)
         for (int i = 0; i < HV; ++i) ut[i] = un[i];
         if (ch + 1 < ch1) load_next(ch + 1);
         P2<double> *fl = fl2 + ((ch - ch0) & 1) * VC * a.max_nfs;
-        // the thread's face slots -> flux sigma * f_f in LDS
+        // the thread's face slots -> flux sigma * f_f in LDS, each face Maxwellian the slot's
+        // stripe x-factor times one row factor per chunk
 #pragma unroll
         for (int s = 0; s < kSlotsB3; ++s) {
             if (sfc[s] < 0) continue;
             const int j = threadIdx.x + s * kNTB3;
+            if (ck.new_stripe) stripe_factors<VC>(g, sk[s], ck.ix0, a.n1, exs + j, a.max_nfs);
+            const double eyz = row_factor(sk[s], xr);
 #pragma unroll
             for (int vv = 0; vv < VC; ++vv) {
                 if (vv >= ck.nvv) break;
                 const VelK<S> vk = vel_k<S>(g, ck.vrow + ck.ix0 + vv);
-                fl[vv * a.max_nfs + j] = slot_flux<S>(g, a, fv[s][vv], sfc[s], vk, sS[s], sk[s], rho_b);
+                fl[vv * a.max_nfs + j] =
+                    slot_flux_m<S>(g, a, fv[s][vv], sfc[s], vk, sS[s], sk[s], exs[vv * a.max_nfs + j] * eyz, rho_b);
             }
         }
         __syncthreads();
@@ -1403,7 +1440,7 @@ template <class S>
 size_t lds_tiles_a(const FusedPlan &p, int D, int VC)
 {
     const int ge = sizeof(S) == 4 ? D + 1 : D;
-    return (size_t)VC * (2 * p.max_nl + 2 * p.max_nc * ge) * sizeof(S);
+    return (size_t)VC * (2 * p.max_nl + 2 * kFusedThreadsB * ge) * sizeof(S);
 }
 template <class S>
 size_t lds_pass_a(const FusedPlan &p, int D, int VC)
@@ -1412,10 +1449,12 @@ size_t lds_pass_a(const FusedPlan &p, int D, int VC)
     return lds_tiles_a<S>(p, D, VC) + cex;
 }
 size_t lds_pass_b(const FusedPlan &p, int VC) { return (size_t)VC * 2 * p.max_nfs * sizeof(double); }
-// 3-D pass B: two flux tiles, which the two halves' moment exchange reuses at the end
+// 3-D pass B: two flux tiles, which the two halves' moment exchange reuses at the end, and
+// the slots' x-factors
 size_t lds_pass_b3(const FusedPlan &p, int VC)
 {
-    return std::max(2 * lds_pass_b(p, VC), (size_t)17 * kFusedThreadsB * sizeof(double));
+    return std::max(2 * lds_pass_b(p, VC), (size_t)17 * kFusedThreadsB * sizeof(double)) +
+           (size_t)VC * p.max_nfs * sizeof(double);
 }
 inline int vc_for_b(int vc) { return std::min(vc, kFusedVcB); }
 
