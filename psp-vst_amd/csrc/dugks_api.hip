@@ -48,11 +48,11 @@ enum Schedule { kSchedFaceValues = 1, kSchedRecompute = 2 };
 
 enum Phase {
     kPhBar, kPhFaceRecon, kPhWallRho, kPhWallIn, kPhFaceMom, kPhFaceRelax, kPhUpdate, kPhCellMom, kPhDt,
-    kPhFusedFace, kPhFaceMacro, kPhFusedUpdate, kPhCellMacro, kNumPhases
+    kPhFusedFace, kPhFaceMacro, kPhFusedUpdate, kPhCellMacro, kPhFusedBar, kNumPhases
 };
 const char *kPhaseNames[kNumPhases] = {
     "bar", "face_recon", "wall_rho", "wall_in", "face_moments", "face_relax", "update", "cell_moments", "dt_update",
-    "fused_face", "face_macros", "fused_update", "cell_macros"};
+    "fused_face", "face_macros", "fused_update", "cell_macros", "fused_bar"};
 
 }  // namespace
 
@@ -198,6 +198,7 @@ int fused_step(dugks_handle *h)
         PHASE(kPhCellMacro, rc::Fused<S>::cell_macros(g, p, d.cell, d.dt, st));
     } else {
         FusedPlan &p = h->plan;
+        if (p.Fb != nullptr) PHASE(kPhFusedBar, Fused<S>::bar_pass(g, p, (const S *)d.gT, (const S *)d.hT, d.dt, st));
         PHASE(kPhFusedFace, Fused<S>::face_pass(g, p, (const S *)d.gT, (const S *)d.hT, d.dt, st));
         PHASE(kPhFaceMacro, Fused<S>::face_macros(g, p, d.face, d.rho_b, d.dt, d.cfl_part, st));
         PHASE(kPhFusedUpdate, Fused<S>::update_pass(g, p, (S *)d.gT, (S *)d.hT, d.rho_b, d.dt, st));

@@ -104,9 +104,10 @@ struct FusedPlan {
     // face values [Nv][slots], (g, h) pairs: written by the block that reconstructs them into
     // its own slot
     void *F = nullptr;
-    // 3-D with FP32 storage: u = 4/3 f-bar-plus - 1/3 g~ of every cell, [Nv][nC] (g, h) pairs,
-    // written by pass A and read by pass B instead of g~ (null on the other paths)
-    void *U = nullptr;
+    // 3-D with FP32 storage: f-bar-plus (Fb) and u = 4/3 f-bar-plus - 1/3 g~ (U) of every
+    // cell, [Nv][nC] (g, h) pairs, written by bar_pass; pass A reads Fb, pass B reads U instead
+    // of g~ (null on the other paths)
+    void *U = nullptr, *Fb = nullptr;
     int n_slots = 0;
     d3 *fs_sf = nullptr;                  // [slots] face area vectors, formed at upload
     EqCoef *fs_coef = nullptr;            // [slots] face records, copied each step
@@ -137,6 +138,8 @@ void fused_plan_free(FusedPlan &p);
 
 template <class S>
 struct Fused {
+    // f-bar-plus and u of every cell (3-D with FP32 storage; nothing on the other paths)
+    static void bar_pass(const Geo &g, FusedPlan &p, const S *gT, const S *hT, const double *dt, hipStream_t st);
     static void face_pass(const Geo &g, FusedPlan &p, const S *gT, const S *hT, const double *dt, hipStream_t st);
     static void face_macros(const Geo &g, FusedPlan &p, Macros fm, double *rho_b, const double *dt, double *cfl_part,
                             hipStream_t st);

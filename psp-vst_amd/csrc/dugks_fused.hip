@@ -464,13 +464,11 @@ __device__ inline void cell_bar_eq(const Geo &g, const EqCoef &k, int c, const S
 // HK: the cells' equilibrium records are held in kc for the kernel, else loaded per chunk
 // CL: the x-factors live in LDS, cexL[vv * max_nl + l] (each thread touches only its own
 // cells' entries), else in cex
-// With U, the block cells also write u = 4/3 f-bar-plus - 1/3 g~ (and the same of h), the
-// part of the cell update that needs no fluxes, for pass B.
 template <class S, int VC, int NT, int CPT, bool D3, bool PRE, bool HK, bool CL>
 __device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &bi, const S *__restrict__ gT,
                                  const S *__restrict__ hT, double dt, const Chunk &ck, const int (&lc)[CPT],
                                  const S (&pg)[CPT][VC], const S (&ph)[CPT][VC], const EqCoef (&kc)[CPT],
-                                 double (&cex)[CPT][VC], double *cexL, P2<S> *fb, P2<S> *__restrict__ U)
+                                 double (&cex)[CPT][VC], double *cexL, P2<S> *fb)
 {
     const d3 xr = cload(g.xi, ck.vrow);
 #pragma unroll
@@ -496,15 +494,10 @@ __device__ inline void stage_bar(const Geo &g, const FArgs &a, const BlockInfo &
             const EqCoef k = a.cc[c];
             cell_bar<S, VC, !PRE>(g, a, k, c, gT, hT, dt, ck, xr, cx, gt, ht, gb, hb, cs);
         }
-        const bool wu = U != nullptr && l < bi.nc;
 #pragma unroll
         for (int vv = 0; vv < VC; ++vv) {
             if (vv >= ck.nvv) break;
             fb[vv * a.max_nl + l] = P2<S>{(S)gb[vv], (S)hb[vv]};
-            if (wu)
-                st_nt(U + (size_t)(ck.vrow + ck.ix0 + vv) * g.nC + c,
-                      P2<S>{(S)(-1.0 / 3 * (double)gt[vv] + 4.0 / 3 * gb[vv]),
-                            (S)(-1.0 / 3 * (double)ht[vv] + 4.0 / 3 * hb[vv])});
         }
     }
 }
@@ -619,10 +612,12 @@ __device__ inline P2<S> recon_l(const P2<S> *fb, const P2<S> *gr, int l, double 
 }
 
 // ------------------------------------------------------------------------- pass A
-template <class S, int D, int VC>
+// BARIN: f-bar-plus of every cell comes from Fb (k_bar3), so stage 1 only gathers the local
+// cells' values; otherwise stage 1 forms them from g~, h~ and the cell records.
+template <class S, int D, int VC, bool BARIN>
 __global__ void __launch_bounds__(kNTA)
     k_pass_a(Geo g, FArgs a, const S *__restrict__ gT, const S *__restrict__ hT, P2<S> *__restrict__ F,
-             P2<S> *__restrict__ U, const double *dtp)
+             const P2<S> *__restrict__ Fb, const double *dtp)
 {
     constexpr int NMA = (D == 3 ? 17 : 11) + 1;
     constexpr int CPT = fused_cells_per_thread_a(D);
@@ -696,16 +691,20 @@ __global__ void __launch_bounds__(kNTA)
     const int2 gk = grad_range<kNTA>(a, bi);
     // 2-D: the next chunk's g~, h~ are loaded before the chunk's last barrier, so their
     // latency overlaps the barrier wait (3-D has no registers to spare)
-    constexpr bool kAhead = D == 2;
+    constexpr bool kAhead = D == 2 && !BARIN;
     S gt[CPT][VC] = {}, ht[CPT][VC] = {};
     // 2-D: the local cells' equilibrium records are read once per kernel, not per chunk
-    constexpr bool kHoldK = D == 2;
+    constexpr bool kHoldK = D == 2 && !BARIN;
     EqCoef kc[CPT] = {};
     if (kHoldK) {
 #pragma unroll
         for (int s = 0; s < CPT; ++s)
             if (threadIdx.x + s * kNTA < bi.nl) kc[s] = a.cc[lc[s]];
     }
+    // BARIN: the local cells' f-bar-plus of the next chunk, loaded before the chunk's last
+    // barrier so the gathers' latency overlaps it (211k-cell sphere pass A 93 -> 65 ms with the
+    // two-velocity face loop)
+    P2<S> pfb[CPT][VC];
     int stripe = -1;
     if (kAhead && ch0 < ch1) {
         int st = stripe;
@@ -716,8 +715,31 @@ __global__ void __launch_bounds__(kNTA)
     }
     for (int ch = ch0; ch < ch1; ++ch) {
         const Chunk ck = chunk_of(a, ch, VC, stripe);
-        stage_bar<S, VC, kNTA, CPT, D == 3, kAhead, kHoldK, kCexLds>(g, a, bi, gT, hT, dt, ck, lc, gt, ht, kc, cex,
-                                                                     cexL, fb, U);
+        if constexpr (BARIN) {
+            if (ch == ch0) {
+#pragma unroll
+                for (int s = 0; s < CPT; ++s) {
+                    const int l = threadIdx.x + s * kNTA;
+                    if (l >= bi.nl) break;
+#pragma unroll
+                    for (int vv = 0; vv < VC; ++vv)
+                        pfb[s][vv] = Fb[(size_t)(ck.vrow + ck.ix0 + min(vv, ck.nvv - 1)) * g.nC + lc[s]];
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < CPT; ++s) {
+                const int l = threadIdx.x + s * kNTA;
+                if (l >= bi.nl) break;
+#pragma unroll
+                for (int vv = 0; vv < VC; ++vv) {
+                    if (vv >= ck.nvv) break;
+                    fb[vv * a.max_nl + l] = pfb[s][vv];
+                }
+            }
+        } else {
+            stage_bar<S, VC, kNTA, CPT, D == 3, kAhead, kHoldK, kCexLds>(g, a, bi, gT, hT, dt, ck, lc, gt, ht, kc,
+                                                                         cex, cexL, fb);
+        }
         __syncthreads();
         stage_grad<S, D, VC, kNTA>(g, a, bi, ck, hx, gk, fb, gr);
         __syncthreads();
@@ -729,9 +751,10 @@ __global__ void __launch_bounds__(kNTA)
             const bool own_in = fmeta[s] & kSlotOwnIn, nei_in = fmeta[s] & kSlotNeiIn;
             const bool split = s == 1 && items;
             const int vb = split ? vo : 0, ve = split ? min(vo + 1, ck.nvv) : ck.nvv;
-            // one velocity per iteration: unrolling by two overlaps the loads of two velocities
-            // but spills once the block cells also write u (sphere pass A 94.3 -> 97.8 ms)
-#pragma unroll 1
+            // BARIN: two velocities per iteration, so the scalar and LDS loads of both overlap
+            // (the forms that build f-bar-plus themselves spill with it)
+            constexpr int kUnroll = BARIN ? 2 : 1;
+#pragma unroll kUnroll
             for (int vv = vb; vv < ve; ++vv) {
                 const int v = ck.vrow + ck.ix0 + vv;
                 const d3 xi = cload(g.xi, v);
@@ -828,6 +851,17 @@ __global__ void __launch_bounds__(kNTA)
             }
             fold_row<D>(acc[s], rs, xr.y, xr.z);
         }
+        if (BARIN && ch + 1 < ch1) {
+            int st = stripe;
+            const Chunk nx = chunk_of(a, ch + 1, VC, st);
+#pragma unroll
+            for (int s = 0; s < CPT; ++s) {
+                if (threadIdx.x + s * kNTA >= bi.nl) break;
+#pragma unroll
+                for (int vv = 0; vv < VC; ++vv)
+                    pfb[s][vv] = Fb[(size_t)(nx.vrow + nx.ix0 + min(vv, nx.nvv - 1)) * g.nC + lc[s]];
+            }
+        }
         if (kAhead && ch + 1 < ch1) {
             int st = stripe;
             const Chunk nx = chunk_of(a, ch + 1, VC, st);
@@ -855,6 +889,56 @@ __global__ void __launch_bounds__(kNTA)
         double *o = a.fpart + (((size_t)vg * g.nF + fid[s]) * 2 + side) * NMA;
 #pragma unroll
         for (int k = 0; k < NMA; ++k) o[k] = acc[s][k];
+    }
+}
+
+// ---------------------------------------------------------- f-bar-plus of every cell
+// 3-D with FP32 storage: f-bar-plus = (1 - a) g~ + a g^S (fvDVM::updateGHbarPvol,
+// fvDVM.C:648-683, FP64) and u = 4/3 f-bar-plus - 1/3 g~ of every cell, both stored as
+// deviations in FP32 for the two passes.  One thread per cell, a range of velocity rows per
+// work-group row; the cell's Maxwellian is its x-factor (per abscissa, in LDS) times one row
+// factor per row, and the record stays in registers.  Loads and stores coalesce over cells.
+constexpr int kBarRowGroups = 8;
+
+template <class S>
+__global__ void __launch_bounds__(kBlk)
+    k_bar3(Geo g, const EqCoef *__restrict__ cc, int n1, const S *__restrict__ gT, const S *__restrict__ hT,
+           P2<S> *__restrict__ Fb, P2<S> *__restrict__ U, const double *dtp)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    double *ex = (double *)smem;  // [n1][kBlk] x-factors of the thread's cell
+    const int c = blockIdx.x * kBlk + threadIdx.x;
+    const bool live = c < g.nC;
+    const int cs = live ? c : g.nC - 1;
+    const EqCoef k = cc[cs];
+    const double dt = *dtp;
+    const double af = 1.5 * dt / (2.0 * k.tr + dt);
+    for (int ix = 0; ix < n1; ++ix) {
+        const double dx = cload(g.xi, ix).x - k.ux;
+        ex[ix * kBlk + threadIdx.x] = k.A * exp_cw(-0.5 * ((dx * dx) * k.invRT));
+    }
+    const int nrow = n1 * n1;
+    const int rpg = (nrow + kBarRowGroups - 1) / kBarRowGroups;
+    const int r0 = blockIdx.y * rpg, r1 = min(nrow, r0 + rpg);
+    for (int row = r0; row < r1; ++row) {
+        const int vrow = row * n1;
+        const double eyz = row_factor(k, cload(g.xi, vrow));
+#pragma unroll 4
+        for (int ix = 0; ix < n1; ++ix) {
+            const int v = vrow + ix;
+            const size_t kk = (size_t)v * g.nC + cs;
+            const S gt = gT[kk], ht = hT[kk];
+            double gEq, hEq;
+            shakhov_poly(g, k, cload(g.xi, v), ex[ix * kBlk + threadIdx.x] * eyz, gEq, hEq);
+            gEq -= cload(g.ref_g, v);
+            hEq -= cload(g.ref_h, v);
+            const double gb = (1.0 - af) * (double)gt + af * gEq;
+            const double hb = (1.0 - af) * (double)ht + af * hEq;
+            if (live) {
+                st_nt(Fb + kk, P2<S>{(S)gb, (S)hb});
+                st_nt(U + kk, P2<S>{(S)(-1.0 / 3 * (double)gt + 4.0 / 3 * gb), (S)(-1.0 / 3 * (double)ht + 4.0 / 3 * hb)});
+            }
+        }
     }
 }
 
@@ -1166,8 +1250,8 @@ __global__ void __launch_bounds__(kNTB)
 
 // ------------------------------------------------------------------ pass B, 3-D
 // 512 threads per work-group: two threads per block cell, each taking half of the chunk's
-// velocities in the update, and two face slots per thread for the fluxes.  Pass A wrote
-// u = 4/3 f-bar-plus - 1/3 g~ of every block cell (U), so the update is g~ <- u - dt/V
+// velocities in the update, and two face slots per thread for the fluxes.  k_bar3 wrote
+// u = 4/3 f-bar-plus - 1/3 g~ of every cell (U), so the update is g~ <- u - dt/V
 // sum(+-flux) and this pass needs no cell equilibrium.  What does not change over the
 // velocities is loaded once per kernel and held in registers: the cell's face table, each
 // slot's face record and area vector.  The next chunk's stored face values and u are loaded
@@ -1443,9 +1527,9 @@ size_t lds_tiles_a(const FusedPlan &p, int D, int VC)
     return (size_t)VC * (2 * p.max_nl + 2 * kFusedThreadsB * ge) * sizeof(S);
 }
 template <class S>
-size_t lds_pass_a(const FusedPlan &p, int D, int VC)
+size_t lds_pass_a(const FusedPlan &p, int D, int VC, bool barin = false)
 {
-    const size_t cex = (D == 3) ? (size_t)VC * p.max_nl * sizeof(double) : 0;
+    const size_t cex = (D == 3 && !barin) ? (size_t)VC * p.max_nl * sizeof(double) : 0;
     return lds_tiles_a<S>(p, D, VC) + cex;
 }
 size_t lds_pass_b(const FusedPlan &p, int VC) { return (size_t)VC * 2 * p.max_nfs * sizeof(double); }
@@ -1466,11 +1550,12 @@ constexpr size_t kLdsBudget = 80 * 1024;
 constexpr size_t kLdsBudgetA3 = 160 * 1024;
 
 template <class S>
-int choose_vc(const FusedPlan &p, int D)
+int choose_vc(const FusedPlan &p, int D, bool barin = false)
 {
     if (D == 3)
         for (int vc : {8, 6})
-            if (lds_pass_a<S>(p, D, vc) <= kLdsBudgetA3 && lds_pass_b(p, vc_for_b(vc)) <= kLdsBudget) return vc;
+            if (lds_pass_a<S>(p, D, vc, barin) <= kLdsBudgetA3 && lds_pass_b(p, vc_for_b(vc)) <= kLdsBudget)
+                return vc;
     for (int vc : {4, 2, 1})
         if (lds_tiles_a<S>(p, D, vc) <= kLdsBudget && lds_pass_b(p, vc_for_b(vc)) <= kLdsBudget) return vc;
     return 0;
@@ -1482,9 +1567,17 @@ inline int grid_b(const FusedPlan &p) { return kFusedXcds * p.b8 * p.nvg_b; }
 template <class S, int D, int VC>
 void launch_a(const Geo &g, FusedPlan &p, const S *gT, const S *hT, const double *dt, hipStream_t st)
 {
+    if constexpr (sizeof(S) == 4 && D == 3) {
+        if (p.Fb != nullptr) {
+            const size_t lds = lds_pass_a<S>(p, D, VC, true);
+            hipLaunchKernelGGL((k_pass_a<S, D, VC, true>), dim3(grid_a(p)), dim3(kNTA), lds, st, g, fargs(p), gT, hT,
+                               (P2<S> *)p.F, (const P2<S> *)p.Fb, dt);
+            return;
+        }
+    }
     const size_t lds = lds_pass_a<S>(p, D, VC);
-    hipLaunchKernelGGL((k_pass_a<S, D, VC>), dim3(grid_a(p)), dim3(kNTA), lds, st, g, fargs(p), gT, hT, (P2<S> *)p.F,
-                       (P2<S> *)p.U, dt);
+    hipLaunchKernelGGL((k_pass_a<S, D, VC, false>), dim3(grid_a(p)), dim3(kNTA), lds, st, g, fargs(p), gT, hT,
+                       (P2<S> *)p.F, (const P2<S> *)nullptr, dt);
 }
 
 template <class S, int D, int VC>
@@ -1511,13 +1604,18 @@ bool set_lds_attrs(int D, int vc, const FusedPlan &p)
     };
     bool ok = true;
 #define DUGKS_SET(DD, VV)                                                                              \
-    if (D == DD && vc == VV) ok &= set((const void *)k_pass_a<S, DD, VV>, lds_pass_a<S>(p, DD, VV));   \
+    if (D == DD && vc == VV) ok &= set((const void *)k_pass_a<S, DD, VV, false>, lds_pass_a<S>(p, DD, VV));   \
     if (D == DD && vc_for_b(vc) == VV) ok &= set((const void *)k_pass_b<S, DD, VV>, lds_pass_b(p, VV));
     DUGKS_SET(2, 1) DUGKS_SET(2, 2) DUGKS_SET(2, 4) DUGKS_SET(3, 1) DUGKS_SET(3, 2) DUGKS_SET(3, 4)
 #undef DUGKS_SET
     // 3-D pass A alone takes 6 or 8 velocities per chunk (pass B keeps 4)
-    if (D == 3 && vc == 6) ok &= set((const void *)k_pass_a<S, 3, 6>, lds_pass_a<S>(p, 3, 6));
-    if (D == 3 && vc == 8) ok &= set((const void *)k_pass_a<S, 3, 8>, lds_pass_a<S>(p, 3, 8));
+    if (D == 3 && vc == 6) ok &= set((const void *)k_pass_a<S, 3, 6, false>, lds_pass_a<S>(p, 3, 6));
+    if (D == 3 && vc == 8) ok &= set((const void *)k_pass_a<S, 3, 8, false>, lds_pass_a<S>(p, 3, 8));
+    if constexpr (sizeof(S) == 4) {
+        if (D == 3 && vc == 6) ok &= set((const void *)k_pass_a<S, 3, 6, true>, lds_pass_a<S>(p, 3, 6, true));
+        if (D == 3 && vc == 8) ok &= set((const void *)k_pass_a<S, 3, 8, true>, lds_pass_a<S>(p, 3, 8, true));
+        if (D == 3) ok &= set((const void *)k_bar3<S>, (size_t)p.n1 * kBlk * sizeof(double));
+    }
     if (D == 3 && vc_for_b(vc) == 4) ok &= set((const void *)k_pass_b3<S, 4>, lds_pass_b3(p, 4));
     return ok;
 }
@@ -1585,7 +1683,14 @@ void fused_velocity_groups(FusedPlan &p, int D, int n1, int vc)
 
 bool fused_plan_upload(FusedPlan &p, const Geo &g, size_t es, std::string &err)
 {
-    const int vc = es == 4 ? choose_vc<float>(p, g.D) : choose_vc<double>(p, g.D);
+    // 3-D with FP32 storage: f-bar-plus and u of every cell from k_bar3 (Fb, U), pass B in
+    // the 512-thread form (k_pass_b3)
+    bool fbar = es == 4 && g.D == 3 && p.max_nfs <= kSlotsB3 * kNTB3;
+    int vc = es == 4 ? choose_vc<float>(p, g.D, fbar) : choose_vc<double>(p, g.D);
+    if (fbar && vc_for_b(vc) != 4) {
+        fbar = false;
+        vc = choose_vc<float>(p, g.D);
+    }
     if (vc == 0) {
         err = "block tables exceed the LDS budget";
         return false;
@@ -1625,11 +1730,8 @@ bool fused_plan_upload(FusedPlan &p, const Geo &g, size_t es, std::string &err)
         !dbuf(p, &fc, (size_t)g.nF * sizeof(EqCoef), err) || !dbuf(p, &bc, (size_t)g.nB * sizeof(BndCoef), err) ||
         !dbuf(p, &p.F, (size_t)g.Nv * p.n_slots * 2 * es, err))
         return false;
-    // 3-D with FP32 storage: pass A hands the flux-free part of the cell update to pass B
-    // (k_pass_b3) through U
-    p.U = nullptr;
-    if (es == 4 && g.D == 3 && p.vc_b == 4 && p.max_nfs <= kSlotsB3 * kNTB3 &&
-        !dbuf(p, &p.U, (size_t)g.Nv * g.nC * 2 * es, err))
+    p.U = p.Fb = nullptr;
+    if (fbar && (!dbuf(p, &p.U, (size_t)g.Nv * g.nC * 2 * es, err) || !dbuf(p, &p.Fb, (size_t)g.Nv * g.nC * 2 * es, err)))
         return false;
     p.fpart = (double *)fp;
     p.cpart = (double *)cp;
@@ -1652,7 +1754,7 @@ bool fused_plan_upload(FusedPlan &p, const Geo &g, size_t es, std::string &err)
         err = "clearing the face partials failed";
         return false;
     }
-    p.lds_a = es == 4 ? lds_pass_a<float>(p, g.D, vc) : lds_pass_a<double>(p, g.D, vc);
+    p.lds_a = es == 4 ? lds_pass_a<float>(p, g.D, vc, fbar) : lds_pass_a<double>(p, g.D, vc);
     p.lds_b = lds_pass_b(p, p.vc_b);
     return true;
 }
@@ -1694,6 +1796,17 @@ void Fused<S>::face_pass(const Geo &g, FusedPlan &p, const S *gT, const S *hT, c
     if (g.D == DD && p.vc == VV) return launch_a<S, DD, VV>(g, p, gT, hT, dt, st);
     DUGKS_A(2, 1) DUGKS_A(2, 2) DUGKS_A(2, 4) DUGKS_A(3, 1) DUGKS_A(3, 2) DUGKS_A(3, 4) DUGKS_A(3, 6) DUGKS_A(3, 8)
 #undef DUGKS_A
+}
+
+template <class S>
+void Fused<S>::bar_pass(const Geo &g, FusedPlan &p, const S *gT, const S *hT, const double *dt, hipStream_t st)
+{
+    if constexpr (sizeof(S) == 4) {
+        if (p.Fb == nullptr) return;
+        const dim3 grid((g.nC + kBlk - 1) / kBlk, kBarRowGroups);
+        hipLaunchKernelGGL(k_bar3<S>, grid, dim3(kBlk), (size_t)p.n1 * kBlk * sizeof(double), st, g, p.cc, p.n1, gT,
+                           hT, (P2<S> *)p.Fb, (P2<S> *)p.U, dt);
+    }
 }
 
 template <class S>
