@@ -1273,12 +1273,17 @@ __global__ void __launch_bounds__(kNTB3)
     extern __shared__ __align__(16) unsigned char smem[];
     const BlockInfo bi = cload(a.blk, blkid);
     const int ch0 = vg * a.cpg, ch1 = min(a.nchunk, ch0 + a.cpg);
-    // two flux tiles [VC][max_nfs] (sigma * f_f), one per chunk parity: stage 2 of the next
-    // chunk writes the other tile, so one barrier per chunk suffices
+    // two flux tiles [VC][FS] (sigma * f_f), one per chunk parity: stage 2 of the next chunk
+    // writes the other tile, so one barrier per chunk suffices.  Entry max_nfs of every row
+    // stays zero: the cells' face lists are padded to kNb entries with it, so the gather has
+    // neither a face-count nor a sign branch (those conditions, hoisted out of the chunk
+    // loop as lane masks, overflowed the scalar registers)
+    const int FS = a.max_nfs + 1;
     P2<double> *fl2 = (P2<double> *)smem;
     // the slots' Maxwellian x-factors A exp(-(xi_x - U_x)^2 / 2RT) of the current stripe,
     // [VC][max_nfs]; each thread reads only the entries it wrote (pass B 59.8 -> 57.9 ms)
-    double *exs = (double *)(fl2 + 2 * VC * a.max_nfs);
+    double *exs = (double *)(fl2 + 2 * VC * FS);
+    if (threadIdx.x < 2 * VC) fl2[threadIdx.x * FS + a.max_nfs] = P2<double>{0.0, 0.0};
     const double dt = *dtp;
 
     const int my = threadIdx.x % kNTB;
@@ -1289,18 +1294,20 @@ __global__ void __launch_bounds__(kNTB3)
 #pragma unroll
     for (int k = 0; k < NM; ++k) mom[k] = 0.0;
     double dtV = 0.0;
-    int nf = 0;
-    // the cell's faces as block slot indices (< 2^15) with the owner flag in bit 15, two per word
-    unsigned fpk[kNb / 2] = {};
+    // the cell's faces as block slot indices (< 2^15) with the owner flag in bit 15, two per
+    // word, padded with the zero entry
+    unsigned fpk[kNb / 2];
+#pragma unroll
+    for (int j = 0; j < kNb / 2; ++j) fpk[j] = (unsigned)a.max_nfs | ((unsigned)a.max_nfs << 16);
     if (has_cell) {
         dtV = dt / g.V[cell];
         const int k0 = a.fbc_beg[bi.fbc_off + my];
-        nf = a.fbc_beg[bi.fbc_off + my + 1] - k0;
+        const int nf = a.fbc_beg[bi.fbc_off + my + 1] - k0;
 #pragma unroll
         for (int j = 0; j < kNb; ++j) {
             if (j < nf) {
                 const unsigned e = (unsigned)a.fbc_idx[k0 + j] | (a.fbc_sgn[k0 + j] > 0 ? 0x8000u : 0u);
-                fpk[j / 2] |= e << (16 * (j % 2));
+                fpk[j / 2] = (fpk[j / 2] & ~(0xffffu << (16 * (j % 2)))) | (e << (16 * (j % 2)));
             }
         }
     }
@@ -1355,7 +1362,7 @@ __global__ void __launch_bounds__(kNTB3)
 #pragma unroll
         for (int i = 0; i < HV; ++i) ut[i] = un[i];
         if (ch + 1 < ch1) load_next(ch + 1);
-        P2<double> *fl = fl2 + ((ch - ch0) & 1) * VC * a.max_nfs;
+        P2<double> *fl = fl2 + ((ch - ch0) & 1) * VC * FS;
         // the thread's face slots -> flux sigma * f_f in LDS, each face Maxwellian the slot's
         // stripe x-factor times one row factor per chunk
 #pragma unroll
@@ -1368,7 +1375,7 @@ __global__ void __launch_bounds__(kNTB3)
             for (int vv = 0; vv < VC; ++vv) {
                 if (vv >= ck.nvv) break;
                 const VelK<S> vk = vel_k<S>(g, ck.vrow + ck.ix0 + vv);
-                fl[vv * a.max_nfs + j] =
+                fl[vv * FS + j] =
                     slot_flux_m<S>(g, a, fv[s][vv], sfc[s], vk, sS[s], sk[s], exs[vv * a.max_nfs + j] * eyz, rho_b);
             }
         }
@@ -1385,19 +1392,13 @@ __global__ void __launch_bounds__(kNTB3)
                 double ah = (double)ut[i].h;
 #pragma unroll
                 for (int j = 0; j < kNb; ++j) {
-                    if (j < nf) {
-                        const unsigned e = (fpk[j / 2] >> (16 * (j % 2))) & 0xffffu;
-                        const P2<double> q = fl[vv * a.max_nfs + (int)(e & 0x7fffu)];
-                        const double tg = q.g * dtV;
-                        const double th = q.h * dtV;
-                        if (e & 0x8000u) {
-                            ag += tg;
-                            ah += th;
-                        } else {
-                            ag -= tg;
-                            ah -= th;
-                        }
-                    }
+                    const unsigned e = (fpk[j / 2] >> (16 * (j % 2))) & 0xffffu;
+                    const P2<double> q = fl[vv * FS + (int)(e & 0x7fffu)];
+                    // owner faces add, neighbour faces subtract: the sign bit of the scaled
+                    // flux is flipped unless bit 15 is set
+                    const unsigned long long flip = (unsigned long long)(~e & 0x8000u) << 48;
+                    ag += __longlong_as_double(__double_as_longlong(q.g * dtV) ^ flip);
+                    ah += __longlong_as_double(__double_as_longlong(q.h * dtV) ^ flip);
                 }
                 const S sg = (S)ag, sh = (S)ah;
                 const size_t kk = (size_t)v * g.nC + cell;
@@ -1537,7 +1538,7 @@ size_t lds_pass_b(const FusedPlan &p, int VC) { return (size_t)VC * 2 * p.max_nf
 // the slots' x-factors
 size_t lds_pass_b3(const FusedPlan &p, int VC)
 {
-    return std::max(2 * lds_pass_b(p, VC), (size_t)17 * kFusedThreadsB * sizeof(double)) +
+    return std::max((size_t)2 * VC * (p.max_nfs + 1) * 2 * sizeof(double), (size_t)17 * kFusedThreadsB * sizeof(double)) +
            (size_t)VC * p.max_nfs * sizeof(double);
 }
 inline int vc_for_b(int vc) { return std::min(vc, kFusedVcB); }
@@ -1685,7 +1686,7 @@ bool fused_plan_upload(FusedPlan &p, const Geo &g, size_t es, std::string &err)
 {
     // 3-D with FP32 storage: f-bar-plus and u of every cell from k_bar3 (Fb, U), pass B in
     // the 512-thread form (k_pass_b3)
-    bool fbar = es == 4 && g.D == 3 && p.max_nfs <= kSlotsB3 * kNTB3;
+    bool fbar = es == 4 && g.D == 3 && p.max_nfs <= kSlotsB3 * kNTB3 && lds_pass_b3(p, 4) <= kLdsBudgetA3;
     int vc = es == 4 ? choose_vc<float>(p, g.D, fbar) : choose_vc<double>(p, g.D);
     if (fbar && vc_for_b(vc) != 4) {
         fbar = false;
