@@ -209,6 +209,23 @@ __device__ inline void shakhov_poly(const Geo &g, const EqCoef &k, const d3 &xi,
     hEq = (kd + omp * cq * ((s - g.D) * kd - 2 * g.K)) * gM * k.RTh;
 }
 
+// sigma_dot(xi, Sf) for a velocity of the chunk's row: the row's y and z products py =
+// xi_y Sf_y, pz = xi_z Sf_z are formed once per face and chunk (row_products), then
+// (xi_x Sf_x + py) + pz -- the same roundings in the same order, so the upwind side and the
+// outgoing / incoming split match sigma_dot bit for bit in both passes
+__device__ inline void row_products(const d3 &xr, const d3 &Sf, double &py, double &pz)
+{
+#pragma clang fp contract(off)
+    py = xr.y * Sf.y;
+    pz = xr.z * Sf.z;
+}
+__device__ inline double sigma_row(double xix, double sfx, double py, double pz)
+{
+#pragma clang fp contract(off)
+    const double px = xix * sfx;
+    return (px + py) + pz;
+}
+
 // exp(-((xi_y - U_y)^2 + (xi_z - U_z)^2) / 2RT) of the chunk's row (xr = xi of the row)
 __device__ inline double row_factor(const EqCoef &k, const d3 &xr)
 {
@@ -754,12 +771,14 @@ __global__ void __launch_bounds__(kNTA)
             // BARIN: two velocities per iteration, so the scalar and LDS loads of both overlap
             // (the forms that build f-bar-plus themselves spill with it)
             constexpr int kUnroll = BARIN ? 2 : 1;
+            double spy, spz;
+            row_products(xr, fS[s], spy, spz);
 #pragma unroll kUnroll
             for (int vv = vb; vv < ve; ++vv) {
                 const int v = ck.vrow + ck.ix0 + vv;
-                const d3 xi = cload(g.xi, v);
+                const d3 xi{cload((const double *)g.xi, 3 * v), xr.y, xr.z};
                 const double w = cload(g.w, v);
-                const double xs = sigma_dot(xi, fS[s]);
+                const double xs = sigma_row(xi.x, fS[s].x, spy, spz);
                 const P2<S> *pf = fb + vv * a.max_nl, *pq = gr + vv * kNTB * GE;
                 double gf, hf;
                 bool store = true;
@@ -1068,10 +1087,11 @@ __device__ inline P2<double> slot_flux(const Geo &g, const FArgs &a, const P2<S>
 // slot_flux with the face Maxwellian factor gM = A exp(-|xi - U|^2 / 2RT) given by the caller
 template <class S>
 __device__ inline P2<double> slot_flux_m(const Geo &g, const FArgs &a, const P2<S> fv, int f, const VelK<S> &vk,
-                                         const d3 &Sf, const EqCoef &k, double gM, const double *rho_b)
+                                         const d3 &Sf, double spy, double spz, const EqCoef &k, double gM,
+                                         const double *rho_b)
 {
     const d3 xi = vk.xi;
-    const double xs = sigma_dot(xi, Sf);
+    const double xs = sigma_row(xi.x, Sf.x, spy, spz);
     const int b = f - g.nIF;
     if (b < 0 ? (xs >= kVSmall || xs < -kVSmall) : xs > 0) {
         double gEq, hEq;
@@ -1371,12 +1391,15 @@ __global__ void __launch_bounds__(kNTB3)
             const int j = threadIdx.x + s * kNTB3;
             if (ck.new_stripe) stripe_factors<VC>(g, sk[s], ck.ix0, a.n1, exs + j, a.max_nfs);
             const double eyz = row_factor(sk[s], xr);
+            double spy, spz;
+            row_products(xr, sS[s], spy, spz);
 #pragma unroll
             for (int vv = 0; vv < VC; ++vv) {
                 if (vv >= ck.nvv) break;
                 const VelK<S> vk = vel_k<S>(g, ck.vrow + ck.ix0 + vv);
                 fl[vv * FS + j] =
-                    slot_flux_m<S>(g, a, fv[s][vv], sfc[s], vk, sS[s], sk[s], exs[vv * a.max_nfs + j] * eyz, rho_b);
+                    slot_flux_m<S>(g, a, fv[s][vv], sfc[s], vk, sS[s], spy, spz, sk[s], exs[vv * a.max_nfs + j] * eyz,
+                                   rho_b);
             }
         }
         __syncthreads();
