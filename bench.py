@@ -41,14 +41,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "psp-vst_amd"))
 
 HBM_PEAK_GBS = 8000.0
-KERNEL_SYMBOLS = {"fused_update": "k_pass_b", "fused_face": "k_pass_a", "face_recon": "face_recon",
-                  "update": "k_update", "bar": "k_bar", "cell_moments": "k_cell_moments"}
+KERNEL_SYMBOLS = {"fused_update": "k_pass_b", "fused_face": "k_pass_a", "fused_bar": "k_bar3",
+                  "face_recon": "face_recon", "update": "k_update", "bar": "k_bar<",
+                  "cell_moments": "k_cell_moments"}
 PROBE_WIDTHS = (4, 8, 16)
 # access width (bytes per lane) that carries most of each pass's HBM bytes: the face-value
 # pass B reads (g, h) FP32 face-value pairs (8 B) for ~3.7 face slots per cell-velocity of
-# the sphere; pass A and the recompute schedule's passes read FP32 g~ and h~ one value per
-# lane (4 B)
-DOMINANT_WIDTH = {"fused_update": 8, "fused_face": 4}
+# the sphere, and so does pass A write them; fused_bar reads FP32 g~ and h~ one value per
+# lane (4 B), as do the recompute schedule's passes
+DOMINANT_WIDTH = {"fused_update": 8, "fused_face": 8, "fused_bar": 4}
 
 
 def workload_tag(cells: int, ndv: int, kn: float) -> str:
@@ -143,9 +144,11 @@ def algorithmic_bytes(kernel: str, nC: int, nF: int, Nv: int, s: int) -> float |
         "face_relax": 4 * fv * s,                # read + write face g,h
         "update": 6 * cv * s + 2 * fv * s,       # read g~,h~,g+,h+, face g,h; write g~,h~
         "cell_moments": 2 * cv * s,              # read g~,h~
-        # fused path
-        "fused_face": 2 * cv * s,                # read g~,h~
-        "fused_update": 4 * cv * s,              # read g~,h~; write g~,h~
+        # fused path (3-D FP32 storage: fused_bar forms f-bar-plus and u of every cell, pass A
+        # reads f-bar-plus instead of g~,h~ and pass B reads u instead of g~,h~)
+        "fused_bar": 6 * cv * s,                 # read g~,h~; write f-bar-plus and u pairs
+        "fused_face": 2 * cv * s,                # read g~,h~ (or f-bar-plus pairs)
+        "fused_update": 4 * cv * s,              # read g~,h~ (or u pairs); write g~,h~
     }
     return table.get(kernel)
 

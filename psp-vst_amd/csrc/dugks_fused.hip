@@ -630,7 +630,8 @@ __device__ inline P2<S> recon_l(const P2<S> *fb, const P2<S> *gr, int l, double 
 
 // ------------------------------------------------------------------------- pass A
 // BARIN: f-bar-plus of every cell comes from Fb (k_bar3), so stage 1 only gathers the local
-// cells' values; otherwise stage 1 forms them from g~, h~ and the cell records.
+// cells' values, each cell's VC values of a chunk one contiguous run; otherwise stage 1 forms
+// them from g~, h~ and the cell records.
 template <class S, int D, int VC, bool BARIN>
 __global__ void __launch_bounds__(kNTA)
     k_pass_a(Geo g, FArgs a, const S *__restrict__ gT, const S *__restrict__ hT, P2<S> *__restrict__ F,
@@ -740,7 +741,7 @@ __global__ void __launch_bounds__(kNTA)
                     if (l >= bi.nl) break;
 #pragma unroll
                     for (int vv = 0; vv < VC; ++vv)
-                        pfb[s][vv] = Fb[(size_t)(ck.vrow + ck.ix0 + min(vv, ck.nvv - 1)) * g.nC + lc[s]];
+                        pfb[s][vv] = Fb[((size_t)ch * g.nC + lc[s]) * VC + vv];
                 }
             }
 #pragma unroll
@@ -871,14 +872,12 @@ __global__ void __launch_bounds__(kNTA)
             fold_row<D>(acc[s], rs, xr.y, xr.z);
         }
         if (BARIN && ch + 1 < ch1) {
-            int st = stripe;
-            const Chunk nx = chunk_of(a, ch + 1, VC, st);
 #pragma unroll
             for (int s = 0; s < CPT; ++s) {
                 if (threadIdx.x + s * kNTA >= bi.nl) break;
 #pragma unroll
                 for (int vv = 0; vv < VC; ++vv)
-                    pfb[s][vv] = Fb[(size_t)(nx.vrow + nx.ix0 + min(vv, nx.nvv - 1)) * g.nC + lc[s]];
+                    pfb[s][vv] = Fb[((size_t)(ch + 1) * g.nC + lc[s]) * VC + vv];
             }
         }
         if (kAhead && ch + 1 < ch1) {
@@ -915,48 +914,83 @@ __global__ void __launch_bounds__(kNTA)
 // 3-D with FP32 storage: f-bar-plus = (1 - a) g~ + a g^S (fvDVM::updateGHbarPvol,
 // fvDVM.C:648-683, FP64) and u = 4/3 f-bar-plus - 1/3 g~ of every cell, both stored as
 // deviations in FP32 for the two passes.  One thread per cell, a range of velocity rows per
-// work-group row; the cell's Maxwellian is its x-factor (per abscissa, in LDS) times one row
-// factor per row, and the record stays in registers.  Loads and stores coalesce over cells.
+// work-group row, stripe by stripe; the cell's Maxwellian is its x-factor (per abscissa of
+// the stripe, in registers) times one row factor per row, and the record stays in registers.  u is stored [v][cell] like g~ (pass B
+// reads it over cells); f-bar-plus in pass A's chunks, [chunk][cell][VC] with chunk = stripe
+// * nrow + row: a cell's VC values of a chunk are one contiguous 8 VC-byte run, so pass A's
+// gathers of halo cells, which lie scattered in memory, move whole runs (sphere pass A HBM
+// fetch 116 GB per launch with the [v][cell] layout).
 constexpr int kBarRowGroups = 8;
 
-template <class S>
+template <class S, int VC>
 __global__ void __launch_bounds__(kBlk)
     k_bar3(Geo g, const EqCoef *__restrict__ cc, int n1, const S *__restrict__ gT, const S *__restrict__ hT,
            P2<S> *__restrict__ Fb, P2<S> *__restrict__ U, const double *dtp)
 {
+    static_assert(16 % sizeof(P2<S>) == 0 && (VC * sizeof(P2<S>)) % 16 == 0, "16-byte pieces of whole pairs");
     extern __shared__ __align__(16) unsigned char smem[];
-    double *ex = (double *)smem;  // [n1][kBlk] x-factors of the thread's cell
+    const int lane = threadIdx.x % 64;
+    // the wave's store patch, [64][VC] pairs
+    P2<S> *patch = (P2<S> *)smem + (threadIdx.x / 64) * 64 * VC;
     const int c = blockIdx.x * kBlk + threadIdx.x;
+    const int wave_c0 = c - lane;
     const bool live = c < g.nC;
     const int cs = live ? c : g.nC - 1;
     const EqCoef k = cc[cs];
     const double dt = *dtp;
     const double af = 1.5 * dt / (2.0 * k.tr + dt);
-    for (int ix = 0; ix < n1; ++ix) {
-        const double dx = cload(g.xi, ix).x - k.ux;
-        ex[ix * kBlk + threadIdx.x] = k.A * exp_cw(-0.5 * ((dx * dx) * k.invRT));
-    }
     const int nrow = n1 * n1;
+    const int nstripe = (n1 + VC - 1) / VC;
     const int rpg = (nrow + kBarRowGroups - 1) / kBarRowGroups;
     const int r0 = blockIdx.y * rpg, r1 = min(nrow, r0 + rpg);
-    for (int row = r0; row < r1; ++row) {
-        const int vrow = row * n1;
-        const double eyz = row_factor(k, cload(g.xi, vrow));
-#pragma unroll 4
-        for (int ix = 0; ix < n1; ++ix) {
-            const int v = vrow + ix;
-            const size_t kk = (size_t)v * g.nC + cs;
-            const S gt = gT[kk], ht = hT[kk];
-            double gEq, hEq;
-            shakhov_poly(g, k, cload(g.xi, v), ex[ix * kBlk + threadIdx.x] * eyz, gEq, hEq);
-            gEq -= cload(g.ref_g, v);
-            hEq -= cload(g.ref_h, v);
-            const double gb = (1.0 - af) * (double)gt + af * gEq;
-            const double hb = (1.0 - af) * (double)ht + af * hEq;
-            if (live) {
-                st_nt(Fb + kk, P2<S>{(S)gb, (S)hb});
-                st_nt(U + kk, P2<S>{(S)(-1.0 / 3 * (double)gt + 4.0 / 3 * gb), (S)(-1.0 / 3 * (double)ht + 4.0 / 3 * hb)});
+    for (int st = 0; st < nstripe; ++st) {
+        double ex[VC];  // the stripe's x-factors of the cell
+#pragma unroll
+        for (int vv = 0; vv < VC; ++vv) {
+            const double dx = cload(g.xi, min(st * VC + vv, n1 - 1)).x - k.ux;
+            ex[vv] = k.A * exp_cw(-0.5 * ((dx * dx) * k.invRT));
+        }
+        for (int row = r0; row < r1; ++row) {
+            const int vrow = row * n1;
+            const double eyz = row_factor(k, cload(g.xi, vrow));
+            P2<S> fbv[VC];
+#pragma unroll
+            for (int vv = 0; vv < VC; ++vv) {
+                const int ix = min(st * VC + vv, n1 - 1);  // a short last stripe repeats its last abscissa
+                const int v = vrow + ix;
+                const size_t kk = (size_t)v * g.nC + cs;
+                const S gt = gT[kk], ht = hT[kk];
+                double gEq, hEq;
+                shakhov_poly(g, k, cload(g.xi, v), ex[vv] * eyz, gEq, hEq);
+                gEq -= cload(g.ref_g, v);
+                hEq -= cload(g.ref_h, v);
+                const double gb = (1.0 - af) * (double)gt + af * gEq;
+                const double hb = (1.0 - af) * (double)ht + af * hEq;
+                fbv[vv] = P2<S>{(S)gb, (S)hb};
+                if (live && st * VC + vv < n1)
+                    st_nt(U + kk, P2<S>{(S)(-1.0 / 3 * (double)gt + 4.0 / 3 * gb),
+                                        (S)(-1.0 / 3 * (double)ht + 4.0 / 3 * hb)});
             }
+            // the wave's 64 cells hold one contiguous run of 64 VC pairs: staged through the
+            // wave's LDS patch so that each store instruction writes whole consecutive
+            // 16-byte pieces (per-lane runs written straight out leave every instruction with
+            // a quarter of each segment)
+#pragma unroll
+            for (int vv = 0; vv < VC; ++vv) patch[lane * VC + vv] = fbv[vv];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            P2<S> *o = Fb + ((size_t)(st * nrow + row) * g.nC + wave_c0) * VC;
+            constexpr int PP = 16 / sizeof(P2<S>);  // pairs per 16-byte piece
+#pragma unroll
+            for (int q = 0; q < VC / PP; ++q) {
+                const int e = (q * 64 + lane) * PP;  // first pair of the lane's piece
+                if (wave_c0 + e / VC < g.nC) {
+                    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+                    __builtin_nontemporal_store(*(const u4 *)(patch + e), (u4 *)(o + e));
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
         }
     }
 }
@@ -1557,6 +1591,12 @@ size_t lds_pass_a(const FusedPlan &p, int D, int VC, bool barin = false)
     return lds_tiles_a<S>(p, D, VC) + cex;
 }
 size_t lds_pass_b(const FusedPlan &p, int VC) { return (size_t)VC * 2 * p.max_nfs * sizeof(double); }
+// k_bar3: the waves' store patches
+template <class S>
+size_t lds_bar3(const FusedPlan &, int VC)
+{
+    return (size_t)kBlk * VC * 2 * sizeof(S);
+}
 // 3-D pass B: two flux tiles, which the two halves' moment exchange reuses at the end, and
 // the slots' x-factors
 size_t lds_pass_b3(const FusedPlan &p, int VC)
@@ -1638,7 +1678,8 @@ bool set_lds_attrs(int D, int vc, const FusedPlan &p)
     if constexpr (sizeof(S) == 4) {
         if (D == 3 && vc == 6) ok &= set((const void *)k_pass_a<S, 3, 6, true>, lds_pass_a<S>(p, 3, 6, true));
         if (D == 3 && vc == 8) ok &= set((const void *)k_pass_a<S, 3, 8, true>, lds_pass_a<S>(p, 3, 8, true));
-        if (D == 3) ok &= set((const void *)k_bar3<S>, (size_t)p.n1 * kBlk * sizeof(double));
+        if (D == 3 && vc == 6) ok &= set((const void *)k_bar3<S, 6>, lds_bar3<S>(p, 6));
+        if (D == 3 && vc == 8) ok &= set((const void *)k_bar3<S, 8>, lds_bar3<S>(p, 8));
     }
     if (D == 3 && vc_for_b(vc) == 4) ok &= set((const void *)k_pass_b3<S, 4>, lds_pass_b3(p, 4));
     return ok;
@@ -1711,7 +1752,7 @@ bool fused_plan_upload(FusedPlan &p, const Geo &g, size_t es, std::string &err)
     // the 512-thread form (k_pass_b3)
     bool fbar = es == 4 && g.D == 3 && p.max_nfs <= kSlotsB3 * kNTB3 && lds_pass_b3(p, 4) <= kLdsBudgetA3;
     int vc = es == 4 ? choose_vc<float>(p, g.D, fbar) : choose_vc<double>(p, g.D);
-    if (fbar && vc_for_b(vc) != 4) {
+    if (fbar && (vc_for_b(vc) != 4 || (vc != 6 && vc != 8))) {  // the k_bar3 / BARIN instances
         fbar = false;
         vc = choose_vc<float>(p, g.D);
     }
@@ -1755,7 +1796,8 @@ bool fused_plan_upload(FusedPlan &p, const Geo &g, size_t es, std::string &err)
         !dbuf(p, &p.F, (size_t)g.Nv * p.n_slots * 2 * es, err))
         return false;
     p.U = p.Fb = nullptr;
-    if (fbar && (!dbuf(p, &p.U, (size_t)g.Nv * g.nC * 2 * es, err) || !dbuf(p, &p.Fb, (size_t)g.Nv * g.nC * 2 * es, err)))
+    if (fbar && (!dbuf(p, &p.U, (size_t)g.Nv * g.nC * 2 * es, err) ||
+                 !dbuf(p, &p.Fb, (size_t)p.nchunk * vc * g.nC * 2 * es, err)))
         return false;
     p.fpart = (double *)fp;
     p.cpart = (double *)cp;
@@ -1828,8 +1870,13 @@ void Fused<S>::bar_pass(const Geo &g, FusedPlan &p, const S *gT, const S *hT, co
     if constexpr (sizeof(S) == 4) {
         if (p.Fb == nullptr) return;
         const dim3 grid((g.nC + kBlk - 1) / kBlk, kBarRowGroups);
-        hipLaunchKernelGGL(k_bar3<S>, grid, dim3(kBlk), (size_t)p.n1 * kBlk * sizeof(double), st, g, p.cc, p.n1, gT,
-                           hT, (P2<S> *)p.Fb, (P2<S> *)p.U, dt);
+        const size_t lds = lds_bar3<S>(p, p.vc);
+        if (p.vc == 8)
+            hipLaunchKernelGGL((k_bar3<S, 8>), grid, dim3(kBlk), lds, st, g, p.cc, p.n1, gT, hT, (P2<S> *)p.Fb,
+                               (P2<S> *)p.U, dt);
+        else if (p.vc == 6)
+            hipLaunchKernelGGL((k_bar3<S, 6>), grid, dim3(kBlk), lds, st, g, p.cc, p.n1, gT, hT, (P2<S> *)p.Fb,
+                               (P2<S> *)p.U, dt);
     }
 }
 

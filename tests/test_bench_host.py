@@ -47,9 +47,9 @@ def _probe_rows(probe_kib, fetch_div, write_div):
 
 
 def test_traffic_summary_scales_by_the_probe_of_the_dominant_width(tmp_path):
-    """Per-width probe scales: the face-value pass B (8-byte face-value pairs) takes the
-    8-byte scale, pass A and the recompute schedule's pass B (4-byte values) the 4-byte one;
-    kernel figures are per-dispatch means times those scales."""
+    """Per-width probe scales: the face-value passes (8-byte face-value pairs) take the
+    8-byte scale, the recompute schedule's pass B (4-byte values) the 4-byte one; kernel
+    figures are per-dispatch means times those scales."""
     b = _bench()
     mib = 64
     probe_kib = (mib << 20) / 1024.0
@@ -69,8 +69,8 @@ def test_traffic_summary_scales_by_the_probe_of_the_dominant_width(tmp_path):
     assert kb["write_bytes"] == pytest.approx(400.0 * 1024)
     assert kb["traffic_bytes"] == pytest.approx(kb["fetch_bytes"] + kb["write_bytes"])
     ka = out["kernels"]["fused_face"]
-    assert ka["scale_width"] == 4
-    assert ka["traffic_bytes"] == pytest.approx(500.0 * 1024 * 2.0 + 10.0 * 1024)
+    assert ka["scale_width"] == 8
+    assert ka["traffic_bytes"] == pytest.approx(500.0 * 1024 * 1.6 + 10.0 * 1024)
 
 
 def test_recompute_schedule_pass_b_takes_the_4_byte_scale(tmp_path):
