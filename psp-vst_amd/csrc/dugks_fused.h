@@ -111,6 +111,7 @@ struct FusedPlan {
     int n_slots = 0;
     d3 *fs_sf = nullptr;                  // [slots] face area vectors, formed at upload
     EqCoef *fs_coef = nullptr;            // [slots] face records, copied each step
+    void *rw = nullptr;                   // [Nv] power-sum weights w xi_x^k, k = 0..3
     std::vector<void *> allocs;
 };
 
