@@ -121,6 +121,11 @@ typedef struct dugks_options {
                                      schedule chosen by dimension and precision;
                                   2: fused, stored-face-value schedule required;
                                   3: fused, recompute schedule required;
+                                  4: fused, stored-face-value schedule in its two-pass
+                                     form, each pass forming f-bar-plus itself (what FP64
+                                     storage and 2-D meshes run, and what 3-D FP32 storage
+                                     falls back to when the f-bar-plus / u arrays of its
+                                     three-kernel form do not fit in device memory);
                                   0: staged path with materialised face distributions;
                                   -1: fused when available, else staged */
     int macro_flux;            /* updateMacroVol (fvDVM.C:1521-1682): 0 macroFlux no: cell

@@ -267,7 +267,7 @@ int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patch
     o.fused = -1;
     if (opt) o = *opt;
     if (o.precision != DUGKS_FP64 && o.precision != DUGKS_MIXED) return fail("dugks_create: unknown precision");
-    if (o.fused < -1 || o.fused > 3) return fail("dugks_create: options.fused must be -1, 0, 1, 2 or 3");
+    if (o.fused < -1 || o.fused > 4) return fail("dugks_create: options.fused must be -1, 0, 1, 2, 3 or 4");
     if (o.macro_flux < 0 || o.macro_flux > 2) return fail("dugks_create: options.macro_flux must be 0, 1 or 2");
     // velocity lists and the macroFlux update run on the staged path
     if (vlist || o.macro_flux != 0) {
@@ -364,8 +364,8 @@ int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patch
     // internal numbering) whose halo fits the chosen fused schedule's work-group shapes.
     // Faces keep the caller's numbering; every cell-indexed array crossing the C-ABI is
     // permuted on the way in and out.
-    const int sched = o.fused == 2   ? kSchedFaceValues
-                      : o.fused == 3 ? kSchedRecompute
+    const int sched = (o.fused == 2 || o.fused == 4) ? kSchedFaceValues
+                      : o.fused == 3                   ? kSchedRecompute
                       : (D == 2 && o.precision == DUGKS_MIXED) ? kSchedRecompute
                                                                 : kSchedFaceValues;
     // Blocks of 256 cells (192 for 3-D meshes with FP64 storage, whose LDS tiles then hold
@@ -557,11 +557,15 @@ int dugks_create(const dugks_mesh *mesh, const dugks_patch *patches, int n_patch
     h->fused = o.fused != 0;
     h->sched = sched;
     if (h->fused) {
+        auto face_values = [&]() {
+            if (!fused_plan_build(h->plan, g, prep, block_start, h->es, 0, err)) return false;
+            h->plan.two_pass = o.fused == 4;
+            return fused_plan_upload(h->plan, g, h->es, err);
+        };
         const bool ok = sched == kSchedRecompute
                             ? rc::fused_plan_build(h->plan_rc, g, prep, block_start, h->es, 0, err) &&
                                   rc::fused_plan_upload(h->plan_rc, g, h->es, err)
-                            : fused_plan_build(h->plan, g, prep, block_start, h->es, 0, err) &&
-                                  fused_plan_upload(h->plan, g, h->es, err);
+                            : face_values();
         if (!ok) {
             if (o.fused >= 1) return bail(fail("dugks_create: fused path unavailable: " + err));
             fused_plan_free(h->plan);

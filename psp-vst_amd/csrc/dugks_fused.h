@@ -72,6 +72,7 @@ constexpr int kSlotNeiIn = 2;   // neighbour cell in the block (internal faces)
 
 struct FusedPlan {
     bool ready = false;
+    bool two_pass = false;             // set between build and upload: no k_bar3 form even where it applies
     int nblk = 0, nvg = 0;             // blocks, velocity groups
     int vc = 0;                        // velocities per LDS chunk
     // Velocities are visited in chunks (x-stripe of vc abscissae, row = (iy, iz)); chunk id

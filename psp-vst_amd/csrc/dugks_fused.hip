@@ -1781,7 +1781,8 @@ bool fused_plan_upload(FusedPlan &p, const Geo &g, size_t es, std::string &err)
 {
     // 3-D with FP32 storage: f-bar-plus and u of every cell from k_bar3 (Fb, U), pass B in
     // the 512-thread form (k_pass_b3)
-    bool fbar = es == 4 && g.D == 3 && p.max_nfs <= kSlotsB3 * kNTB3 && lds_pass_b3(p, 4) <= kLdsBudgetA3;
+    bool fbar = !p.two_pass && es == 4 && g.D == 3 && p.max_nfs <= kSlotsB3 * kNTB3 &&
+                lds_pass_b3(p, 4) <= kLdsBudgetA3;
     int vc = es == 4 ? choose_vc<float>(p, g.D, fbar) : choose_vc<double>(p, g.D);
     if (fbar && (vc_for_b(vc) != 4 || (vc != 6 && vc != 8))) {  // the k_bar3 / BARIN instances
         fbar = false;
@@ -1796,6 +1797,30 @@ bool fused_plan_upload(FusedPlan &p, const Geo &g, size_t es, std::string &err)
         return false;
     }
     fused_velocity_groups(p, g.D, g.n1d, vc);
+    // the face values, then the k_bar3 form's two more [Nv][nC] pair arrays: without the
+    // memory for those the two-pass form runs (it needs 2 Nv nC es bytes less than that)
+    if (!dbuf(p, &p.F, (size_t)g.Nv * p.n_slots * 2 * es, err)) return false;
+    p.U = p.Fb = nullptr;
+    if (fbar) {
+        void *u = nullptr, *fb = nullptr;
+        if (hipMalloc(&u, (size_t)g.Nv * g.nC * 2 * es) == hipSuccess &&
+            hipMalloc(&fb, (size_t)p.nchunk * vc * g.nC * 2 * es) == hipSuccess) {
+            p.allocs.push_back(u);
+            p.allocs.push_back(fb);
+            p.U = u;
+            p.Fb = fb;
+        } else {
+            if (u) (void)hipFree(u);
+            (void)hipGetLastError();
+            fbar = false;
+            vc = choose_vc<float>(p, g.D);
+            if (vc == 0) {
+                err = "block tables exceed the LDS budget";
+                return false;
+            }
+            fused_velocity_groups(p, g.D, g.n1d, vc);
+        }
+    }
     if (!(es == 4 ? set_lds_attrs<float>(g.D, vc, p) : set_lds_attrs<double>(g.D, vc, p))) {
         err = "cannot raise the dynamic LDS limit";
         return false;
@@ -1823,12 +1848,7 @@ bool fused_plan_upload(FusedPlan &p, const Geo &g, size_t es, std::string &err)
     if (!dbuf(p, &fp, nfpart * 8, err) || !dbuf(p, &cp, (size_t)p.nvg_b * g.nC * p.nm * 8, err) ||
         !dbuf(p, &bi, (size_t)g.nB * p.nm * 8, err) || !dbuf(p, &br, (size_t)g.nB * p.nma * 8, err) ||
         !dbuf(p, &cr, (size_t)p.nma * 8, err) || !dbuf(p, &cc, (size_t)g.nC * sizeof(EqCoef), err) ||
-        !dbuf(p, &fc, (size_t)g.nF * sizeof(EqCoef), err) || !dbuf(p, &bc, (size_t)g.nB * sizeof(BndCoef), err) ||
-        !dbuf(p, &p.F, (size_t)g.Nv * p.n_slots * 2 * es, err))
-        return false;
-    p.U = p.Fb = nullptr;
-    if (fbar && (!dbuf(p, &p.U, (size_t)g.Nv * g.nC * 2 * es, err) ||
-                 !dbuf(p, &p.Fb, (size_t)p.nchunk * vc * g.nC * 2 * es, err)))
+        !dbuf(p, &fc, (size_t)g.nF * sizeof(EqCoef), err) || !dbuf(p, &bc, (size_t)g.nB * sizeof(BndCoef), err))
         return false;
     p.fpart = (double *)fp;
     p.cpart = (double *)cp;

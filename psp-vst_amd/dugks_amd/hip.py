@@ -123,7 +123,9 @@ class FvDVM:
 
     ``fused`` is ``dugks_options.fused``: -1 fused when available, else staged; 0 staged;
     1 fused with the default schedule; 2 fused with stored face values; 3 fused recomputing
-    the halo rings."""
+    the halo rings; 4 stored face values in the two-pass form (each pass forms f-bar-plus
+    itself; the form 3-D FP32 storage falls back to when its f-bar-plus / u arrays do not
+    fit)."""
 
     def __init__(self, case, precision: int = FP64, fused: int = -1, device: int = 0, q1_compat=None,
                  ls_vectors=None):

@@ -9,7 +9,8 @@ Tolerances (relative to the field's max magnitude, see conftest.field_rel_err):
 
 `fused` values follow dugks_options.fused: 0 staged, 1 fused with the default schedule
 (2-D: recompute for FP32 storage, stored face values for FP64 storage; 3-D: stored face
-values), 2 stored-face-value schedule, 3 recompute schedule.
+values; 3-D FP32 storage in its three-kernel form), 2 stored-face-value schedule, 3
+recompute schedule, 4 stored face values in the two-pass form.
 """
 import numpy as np
 import pytest
@@ -183,21 +184,35 @@ def test_small_sphere_fp64_matches_oracle(fused):
     assert field_rel_err(g.boundary_rho(), o.wall_rho()) < 1e-12
 
 
-def test_small_sphere_mixed_matches_oracle():
+def _three_kernel_form(g):
+    """True when the handle's step ran the f-bar-plus kernel (the 3-D FP32 form)."""
+    g.profile(True)
+    g.run(1)
+    names = set(g.kernel_times())
+    g.profile(False)
+    return "fused_bar" in names
+
+
+# fused 1: 3-D FP32 storage takes the three-kernel form (k_bar3, pass A gathering f-bar-plus,
+# the 512-thread pass B); fused 4: the two-pass form it falls back to without memory
+@pytest.mark.parametrize("fused", [1, 4])
+def test_small_sphere_mixed_matches_oracle(fused):
     c = _cases().sphere_case(SMALL_SPHERE, "GH", 6, 1.0, 0.1)
-    o, g, ho, hg = _run_pair(c, 15, 1, 1)
+    o, g, ho, hg = _run_pair(c, 15, 1, fused)
     e = _errs(o, g)
-    print("small sphere mixed", e)
+    print("small sphere mixed", fused, e)
     for k in ("rho", "U", "T"):
         assert e[k] < 1e-5, (k, e[k])
+    assert _three_kernel_form(g) == (fused == 1)
 
 
-def test_small_sphere_odd_velocity_grid_mixed_matches_oracle():
+@pytest.mark.parametrize("fused", [1, 4])
+def test_small_sphere_odd_velocity_grid_mixed_matches_oracle(fused):
     """3-D with an odd Simpson 3/8 set (xi_d = 0 on every axis): velocities tangent to the
     axis-aligned box faces and to the O-grid shell faces reach the FP32 tangent branch of
-    the face-value pass A."""
+    the face-value pass A; 7 abscissae make a short last stripe in every chunked array."""
     c = _cases().sphere_case(SMALL_SPHERE, "NC38", 7, 4.0, 0.1)
-    o, g, ho, hg = _run_pair(c, 10, 1, 1)
+    o, g, ho, hg = _run_pair(c, 10, 1, fused)
     e = _errs(o, g)
     print("small sphere odd grid mixed", e)
     for k in ("rho", "U", "T"):
