@@ -1139,18 +1139,42 @@ __device__ inline P2<double> slot_flux(const Geo &g, const FArgs &a, const P2<S>
     return P2<double>{0.0, 0.0};
 }
 
+// Shakhov equilibrium (fvDVM.C:673-679) for a velocity of the chunk's row given the row's
+// parts of s = |c|^2 / RT and c.q (c = xi - U): sy = (c_y^2 + c_z^2) / RT and qy = c_y q_y +
+// c_z q_z (RowEq, formed once per face and chunk), and gM = A exp(-|c|^2 / 2RT); three
+// operations per velocity for s and c.q instead of ten (3-D pass B 52.9 -> 51.6 ms)
+struct RowEq {
+    double sy, qy;
+};
+__device__ inline RowEq row_eq(const EqCoef &k, const d3 &xr)
+{
+    const double cy = xr.y - k.uy, cz = xr.z - k.uz;
+    return RowEq{(cy * cy + cz * cz) * k.invRT, cy * k.qy + cz * k.qz};
+}
+__device__ inline void shakhov_row(const Geo &g, const EqCoef &k, const RowEq &r, double xix, double gM, double &gEq,
+                                   double &hEq)
+{
+    const double cx = xix - k.ux;
+    const double s = fma(cx * cx, k.invRT, r.sy);
+    const double cq = fma(cx, k.qx, r.qy);
+    const double omp = 1.0 - g.Pr;
+    const double kd = g.K + 3.0 - g.D;
+    gEq = (1.0 + omp * cq * (s - g.D - 2.0)) * gM;
+    hEq = (kd + omp * cq * ((s - g.D) * kd - 2 * g.K)) * gM * k.RTh;
+}
+
 // slot_flux with the face Maxwellian factor gM = A exp(-|xi - U|^2 / 2RT) given by the caller
 template <class S>
 __device__ inline P2<double> slot_flux_m(const Geo &g, const FArgs &a, const P2<S> fv, int f, const VelK<S> &vk,
-                                         const d3 &Sf, double spy, double spz, const EqCoef &k, double gM,
-                                         const double *rho_b)
+                                         const d3 &Sf, double spy, double spz, const EqCoef &k, const RowEq &re,
+                                         double gM, const double *rho_b)
 {
     const d3 xi = vk.xi;
     const double xs = sigma_row(xi.x, Sf.x, spy, spz);
     const int b = f - g.nIF;
     if (b < 0 ? (xs >= kVSmall || xs < -kVSmall) : xs > 0) {
         double gEq, hEq;
-        shakhov_poly(g, k, xi, gM, gEq, hEq);
+        shakhov_row(g, k, re, xi.x, gM, gEq, hEq);
         return P2<double>{xs * ((1.0 - k.tr) * ((double)fv.g + vk.rg) + k.tr * gEq),
                           xs * ((1.0 - k.tr) * ((double)fv.h + vk.rh) + k.tr * hEq)};
     }
@@ -1448,13 +1472,14 @@ __global__ void __launch_bounds__(kNTB3)
             const double eyz = row_factor(sk[s], xr);
             double spy, spz;
             row_products(xr, sS[s], spy, spz);
+            const RowEq re = row_eq(sk[s], xr);
 #pragma unroll
             for (int vv = 0; vv < VC; ++vv) {
                 if (vv >= ck.nvv) break;
                 const VelK<S> vk = vel_k<S>(g, ck.vrow + ck.ix0 + vv);
                 fl[vv * FS + j] =
-                    slot_flux_m<S>(g, a, fv[s][vv], sfc[s], vk, sS[s], spy, spz, sk[s], exs[vv * a.max_nfs + j] * eyz,
-                                   rho_b);
+                    slot_flux_m<S>(g, a, fv[s][vv], sfc[s], vk, sS[s], spy, spz, sk[s], re,
+                                   exs[vv * a.max_nfs + j] * eyz, rho_b);
             }
         }
         __syncthreads();
