@@ -781,6 +781,71 @@ __global__ void __launch_bounds__(kNTA)
         stage_grad<S, D, VC, kNTA>(g, a, bi, ck, hx, gk, fb, gr);
         __syncthreads();
         const d3 xr = cload(g.xi, ck.vrow);
+        if constexpr (BARIN) {
+            // (FP32 storage, f-bar-plus gathered) velocities outer, the thread's slots inner: the
+            // slots share each velocity's scalar loads and their independent work overlaps
+            // (sphere pass A 61.1 -> 59.2 ms); the same face logic as the general loop below,
+            // in its FP32 form
+            RowSums rs[kSlotsA];
+            double spy[kSlotsA], spz[kSlotsA];
+#pragma unroll
+            for (int s = 0; s < kSlotsA; ++s) {
+                rs[s] = RowSums{};
+                row_products(xr, fS[s], spy[s], spz[s]);
+            }
+#pragma unroll 2
+            for (int vv = 0; vv < ck.nvv; ++vv) {
+                const int v = ck.vrow + ck.ix0 + vv;
+                const double xix = cload((const double *)g.xi, 3 * v);
+                const double w = cload(g.w, v);
+                const P2<S> *pq = gr + vv * kNTB * GE;
+#pragma unroll
+                for (int s = 0; s < kSlotsA; ++s) {
+                    if (fid[s] < 0) continue;
+                    const bool own_in = fmeta[s] & kSlotOwnIn, nei_in = fmeta[s] & kSlotNeiIn;
+                    const double xs = sigma_row(xix, fS[s].x, spy[s], spz[s]);
+                    double gf, hf;
+                    bool store = true;
+                    if (ftyp[s] != 0 && !(xs > 0)) continue;  // incoming half added from constants
+                    const bool tangent = ftyp[s] == 0 && xs < kVSmall && xs >= -kVSmall;
+                    if (!tangent) {
+                        const bool own = ftyp[s] != 0 || xs >= kVSmall;
+                        if (!(own ? own_in : nei_in)) continue;
+                        const V3<S> ar = own ? aO[s] : aN[s];
+                        const P2<S> r = recon_s<S, D>(pq, own ? flo[s] : fln[s], ar);
+                        gf = (double)r.g;
+                        hf = (double)r.h;
+                    } else {
+                        store = false;
+                        if (own_in && nei_in) {
+                            const P2<S> rn = recon_s<S, D>(pq, fln[s], aN[s]);
+                            const P2<S> ro = recon_s<S, D>(pq, flo[s], aO[s]);
+                            gf = 0.5 * ((double)rn.g + (double)ro.g);
+                            hf = 0.5 * ((double)rn.h + (double)ro.h);
+                        } else if (own_in) {
+                            const P2<S> r = recon_s<S, D>(pq, flo[s], aO[s]);
+                            gf = 0.5 * (double)r.g;
+                            hf = 0.5 * (double)r.h;
+                        } else {
+                            const P2<S> r = recon_s<S, D>(pq, fln[s], aN[s]);
+                            gf = 0.5 * (double)r.g;
+                            hf = 0.5 * (double)r.h;
+                        }
+                    }
+                    if (store) {
+                        const P2<S> val{(S)gf, (S)hf};
+                        st_nt(F + (size_t)v * a.n_slots + (bi.fs_off + slot_of(s)), val);
+                        const int twin = (fmeta[s] >> 2) - 1;
+                        if (twin >= 0) st_nt(F + (size_t)v * a.n_slots + twin, val);
+                    }
+                    if (ftyp[s] == kMaxwellWall) acc[s][NMA - 1] += w * xs * gf;
+                    row_add(rs[s], w * gf, w * hf, xix);
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < kSlotsA; ++s)
+                if (fid[s] >= 0) fold_row<D>(acc[s], rs[s], xr.y, xr.z);
+        } else
 #pragma unroll
         for (int s = 0; s < kSlotsA; ++s) {
             if (fid[s] < 0) continue;
