@@ -783,9 +783,10 @@ __global__ void __launch_bounds__(kNTA)
         const d3 xr = cload(g.xi, ck.vrow);
         if constexpr (BARIN) {
             // (FP32 storage, f-bar-plus gathered) velocities outer, the thread's slots inner: the
-            // slots share each velocity's scalar loads and their independent work overlaps
-            // (sphere pass A 61.1 -> 59.2 ms); the same face logic as the general loop below,
-            // in its FP32 form
+            // slots share each velocity's scalar loads and their independent work overlaps;
+            // unrolled over the chunk, so the velocities' scalar loads are issued together
+            // (sphere pass A 61.1 -> 59.2 -> 58.0 ms); the same face logic as the general loop
+            // below, in its FP32 form
             RowSums rs[kSlotsA];
             double spy[kSlotsA], spz[kSlotsA];
 #pragma unroll
@@ -793,8 +794,9 @@ __global__ void __launch_bounds__(kNTA)
                 rs[s] = RowSums{};
                 row_products(xr, fS[s], spy[s], spz[s]);
             }
-#pragma unroll 2
-            for (int vv = 0; vv < ck.nvv; ++vv) {
+#pragma unroll
+            for (int vv = 0; vv < VC; ++vv) {
+                if (vv >= ck.nvv) break;
                 const int v = ck.vrow + ck.ix0 + vv;
                 const double xix = cload((const double *)g.xi, 3 * v);
                 const double w = cload(g.w, v);
