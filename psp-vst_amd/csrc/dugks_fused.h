@@ -16,6 +16,10 @@
 // A face value is written once per velocity, by the block holding its upwind cell; pass B
 // reads it for both adjacent cells.  No atomics: every sum has a fixed association, so the
 // FP64 path is bitwise reproducible run to run.
+//
+// 3-D with FP32 storage (bar_pass + the passes): f-bar-plus and u = 4/3 f-bar-plus - 1/3 g~
+// of every cell are formed once per step by bar_pass; pass A gathers f-bar-plus instead of
+// forming it, pass B updates g~ <- u - dt/V sum(+-flux) with no cell equilibrium.
 #pragma once
 
 #include <string>

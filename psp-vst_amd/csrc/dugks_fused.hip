@@ -18,18 +18,28 @@
 //     stage 3  g~ <- 4/3 f-bar-plus - 1/3 g~ - dt/V sum(+-flux) in ascending face order
 //              (fvDVM.C:1462-1519), in place; FP64 raw moments of the new g~, h~ per cell
 //
+// 3-D with FP32 storage runs a three-kernel form of the same step:
+//   k_bar3     f-bar-plus and u = 4/3 f-bar-plus - 1/3 g~ of every cell, once (streaming)
+//   pass A     (BARIN) stage 1 gathers f-bar-plus of the block and halo a chunk ahead into
+//              registers; stage 2 also forms f* = f-bar-plus - (dt/2) xi.grad, so stage 3's
+//              face value f* + grad.(x_f - x_cell) has a velocity-independent arm
+//   pass B     (k_pass_b3) 512 threads, records in registers, u instead of stage 1:
+//              g~ <- u - dt/V sum(+-flux)
+// FP64 storage and 2-D keep the two-pass form above.
+//
 // Each thread owns fixed cells / faces for the whole work-group.  The passes wait on memory
 // latency more than on arithmetic, so the loads of a stage are issued together before their
-// values are used, the chunk index is wave-uniform (per-velocity tables are scalar loads),
-// and exponentials use a short exp (exp_cw).  Pass A keeps the tensor-set factorisation of
-// the cells' Maxwellians: A exp(-(xi_x - U_x)^2 / 2RT) formed once per x-stripe and kept in
-// registers, one exp per chunk for the y/z part.  Pass B forms each face and cell
-// velocity's Maxwellian factor on its own, which frees the registers the cached factors
-// held for the batched loads.  Work-group ids are mapped so that the velocity groups of a block run on
-// the same accelerator die and share its L2.  Reconstruction arithmetic runs in the storage
-// type S (FP32 for the mixed path); equilibria, relaxation, update and moments in FP64.
-// Entries with |xi.S| < VSMALL carry no flux (sigma f = 0 in double precision); their face
-// value is the average of both sides' reconstructions, which only the face moments use.
+// values are used (or a chunk ahead), the chunk index is wave-uniform (per-velocity tables
+// are scalar loads), and exponentials use a short exp (exp_cw).  Cell Maxwellians use the
+// tensor-set factorisation A exp(-(xi_x - U_x)^2 / 2RT) per x-stripe times one exp per chunk
+// for the y/z part (pass A in LDS, k_bar3 in registers, pass B3 for its face slots in LDS);
+// the two-pass pass B forms each face and cell velocity's factor on its own, which frees the
+// registers the cached factors held for the batched loads.  Work-group ids are mapped so
+// that the velocity groups of a block run on the same accelerator die and share its L2.
+// Reconstruction arithmetic runs in the storage type S (FP32 for the mixed path);
+// equilibria, relaxation, update and moments in FP64.  Entries with |xi.S| < VSMALL carry no
+// flux (sigma f = 0 in double precision); their face value is the average of both sides'
+// reconstructions, which only the face moments use.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
