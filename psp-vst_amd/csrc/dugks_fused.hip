@@ -566,9 +566,27 @@ __device__ inline int2 grad_range(const FArgs &a, const BlockInfo &bi)
 template <class S, int D>
 constexpr int grad_entries() { return sizeof(S) == 4 ? D + 1 : D; }
 
-template <class S, int D, int VC, int NT>
+// The block cells' gradient stencils as an LDS table, [kNb][kNTB] entries (least-squares
+// vector, local neighbour index as the fourth word), written once per kernel by the first
+// thread of each cell (FP32 storage): the gradient stage of every chunk reads it instead of
+// global memory, whose latency followed every chunk's first barrier.
+__device__ inline void stencil_table(const FArgs &a, const BlockInfo &bi, int2 gk, float4 *gsl)
+{
+    const int l = threadIdx.x;
+    if (l >= bi.nc) return;
+    const V3<float> *ls = (const V3<float> *)a.gs_ls;
+#pragma unroll
+    for (int j = 0; j < kNb; ++j) {
+        if (j < gk.y) {
+            const V3<float> L = ls[gk.x + j];
+            gsl[j * kNTB + l] = float4{L.x, L.y, L.z, __int_as_float(a.gs_nb[gk.x + j])};
+        }
+    }
+}
+
+template <class S, int D, int VC, int NT, bool GSL = false>
 __device__ inline void stage_grad(const Geo &g, const FArgs &a, const BlockInfo &bi, const Chunk &ck, double hdt,
-                                  int2 gk, const P2<S> *fb, P2<S> *gr)
+                                  int2 gk, const P2<S> *fb, P2<S> *gr, const float4 *gsl = nullptr)
 {
     constexpr int GE = grad_entries<S, D>();
     const int nvv = ck.nvv;
@@ -583,8 +601,14 @@ __device__ inline void stage_grad(const Geo &g, const FArgs &a, const BlockInfo 
 #pragma unroll
         for (int j = 0; j < kNb; ++j) {
             if (j < n) {
-                nb[j] = a.gs_nb[k0 + j];
-                L[j] = ls[k0 + j];
+                if constexpr (GSL) {
+                    const float4 e = gsl[j * kNTB + l];
+                    nb[j] = __float_as_int(e.w);
+                    L[j] = V3<S>{e.x, e.y, e.z};
+                } else {
+                    nb[j] = a.gs_nb[k0 + j];
+                    L[j] = ls[k0 + j];
+                }
             }
         }
 #pragma unroll 1
@@ -666,6 +690,7 @@ __global__ void __launch_bounds__(kNTA)
     k_pass_a(Geo g, FArgs a, const S *__restrict__ gT, const S *__restrict__ hT, P2<S> *__restrict__ F,
              const P2<S> *__restrict__ Fb, const double *dtp)
 {
+    static_assert(!BARIN || (sizeof(S) == 4 && D == 3), "f-bar-plus is gathered in 3-D with FP32 storage only");
     constexpr int NMA = (D == 3 ? 17 : 11) + 1;
     constexpr int CPT = fused_cells_per_thread_a(D);
     int blkid, vg;
@@ -673,7 +698,9 @@ __global__ void __launch_bounds__(kNTA)
     extern __shared__ __align__(16) unsigned char smem[];
     const BlockInfo bi = cload(a.blk, blkid);
     const int ch0 = vg * a.cpg, ch1 = min(a.nchunk, ch0 + a.cpg);
-    P2<S> *fb = (P2<S> *)smem;        // [VC][max_nl]
+    // BARIN (FP32 storage): the gradient stencils' LDS table first, then the tiles
+    float4 *gsl = (float4 *)smem;      // [kNb][kNTB]
+    P2<S> *fb = (P2<S> *)(smem + (BARIN ? kNb * kNTB * sizeof(float4) : 0));  // [VC][max_nl]
     constexpr int GE = grad_entries<S, D>();
     P2<S> *gr = fb + VC * a.max_nl;    // [VC][GE][kNTB]
     // 3-D: the local cells' x-factors in LDS, [VC][max_nl] (registers are short there; the
@@ -736,6 +763,29 @@ __global__ void __launch_bounds__(kNTA)
         lc[s] = l < bi.nl ? a.lcell[bi.l_off + l] : 0;
     }
     const int2 gk = grad_range<kNTA>(a, bi);
+    if constexpr (BARIN) stencil_table(a, bi, gk, gsl);  // read after the first chunk's barrier
+    // BARIN face loop: sigma > tO picks the owner side (in the block), sigma < tN the
+    // neighbour side, tT <= sigma < VSMALL a tangent internal face; neither: nothing (the
+    // incoming half of a boundary face, the upwind cell outside the block, an empty slot).
+    // wf = 1 on walls (outgoing mass flux).  flO, flN: the sides' local cells, 0 where the
+    // side is not in the block, so every lane reads inside the gradient tile.
+    double tO[kSlotsA], tN[kSlotsA], tT[kSlotsA], wf[kSlotsA];
+    int flO[kSlotsA], flN[kSlotsA];
+    if constexpr (BARIN) {
+        const double inf = __builtin_huge_val();
+        const double vsmall_below = __longlong_as_double(__double_as_longlong(kVSmall) - 1);  // x > it: x >= VSMALL
+#pragma unroll
+        for (int s = 0; s < kSlotsA; ++s) {
+            const bool valid = fid[s] >= 0, bnd = ftyp[s] != 0;
+            const bool oin = fmeta[s] & kSlotOwnIn, nin = fmeta[s] & kSlotNeiIn;
+            tO[s] = (!valid || !oin) ? inf : bnd ? 0.0 : vsmall_below;
+            tN[s] = (!valid || bnd || !nin) ? -inf : -kVSmall;
+            tT[s] = (!valid || bnd) ? inf : -kVSmall;
+            wf[s] = (valid && ftyp[s] == kMaxwellWall) ? 1.0 : 0.0;
+            flO[s] = oin ? flo[s] : 0;
+            flN[s] = nin ? fln[s] : 0;
+        }
+    }
     // 2-D: the next chunk's g~, h~ are loaded before the chunk's last barrier, so their
     // latency overlaps the barrier wait (3-D has no registers to spare)
     constexpr bool kAhead = D == 2 && !BARIN;
@@ -788,71 +838,76 @@ __global__ void __launch_bounds__(kNTA)
                                                                          cex, cexL, fb);
         }
         __syncthreads();
-        stage_grad<S, D, VC, kNTA>(g, a, bi, ck, hx, gk, fb, gr);
+        stage_grad<S, D, VC, kNTA, BARIN>(g, a, bi, ck, hx, gk, fb, gr, gsl);
         __syncthreads();
         const d3 xr = cload(g.xi, ck.vrow);
         if constexpr (BARIN) {
             // (FP32 storage, f-bar-plus gathered) velocities outer, the thread's slots inner: the
             // slots share each velocity's scalar loads and their independent work overlaps;
             // unrolled over the chunk, so the velocities' scalar loads are issued together
-            // (sphere pass A 61.1 -> 59.2 -> 58.0 ms); the same face logic as the general loop
-            // below, in its FP32 form
+            // (sphere pass A 61.1 -> 59.2 -> 58.0 ms).  The face logic of the general loop below
+            // as compares of sigma with the slot's thresholds and selects: every lane runs the
+            // one reconstruction (inactive lanes read a block cell and add zeros), so only the
+            // stores branch (58.2 -> 53.8 ms); tangent faces are added after the chunk's loop,
+            // a wall's outgoing mass flux sum w sigma g is formed from the row sums and the
+            // power sums take the weight table (53.8 -> 50.2 ms).
             RowSums rs[kSlotsA];
             double spy[kSlotsA], spz[kSlotsA];
+            bool tg[kSlotsA];  // some velocity of the chunk is tangent to the slot's face
 #pragma unroll
             for (int s = 0; s < kSlotsA; ++s) {
                 rs[s] = RowSums{};
                 row_products(xr, fS[s], spy[s], spz[s]);
+                tg[s] = false;
             }
 #pragma unroll
             for (int vv = 0; vv < VC; ++vv) {
                 if (vv >= ck.nvv) break;
                 const int v = ck.vrow + ck.ix0 + vv;
                 const double xix = cload((const double *)g.xi, 3 * v);
-                const double w = cload(g.w, v);
                 const P2<S> *pq = gr + vv * kNTB * GE;
 #pragma unroll
                 for (int s = 0; s < kSlotsA; ++s) {
-                    if (fid[s] < 0) continue;
-                    const bool own_in = fmeta[s] & kSlotOwnIn, nei_in = fmeta[s] & kSlotNeiIn;
                     const double xs = sigma_row(xix, fS[s].x, spy[s], spz[s]);
-                    double gf, hf;
-                    bool store = true;
-                    if (ftyp[s] != 0 && !(xs > 0)) continue;  // incoming half added from constants
-                    const bool tangent = ftyp[s] == 0 && xs < kVSmall && xs >= -kVSmall;
-                    if (!tangent) {
-                        const bool own = ftyp[s] != 0 || xs >= kVSmall;
-                        if (!(own ? own_in : nei_in)) continue;
-                        const V3<S> ar = own ? aO[s] : aN[s];
-                        const P2<S> r = recon_s<S, D>(pq, own ? flo[s] : fln[s], ar);
-                        gf = (double)r.g;
-                        hf = (double)r.h;
-                    } else {
-                        store = false;
+                    const bool po = xs > tO[s], pn = xs < tN[s];  // owner / neighbour upwind, in the block
+                    const bool act = po || pn;
+                    tg[s] = tg[s] || (xs >= tT[s] && xs < kVSmall);
+                    const P2<S> r = recon_s<S, D>(pq, po ? flO[s] : flN[s], po ? aO[s] : aN[s]);
+                    if (act) {
+                        st_nt(F + (size_t)v * a.n_slots + (bi.fs_off + slot_of(s)), r);
+                        const int twin = (fmeta[s] >> 2) - 1;
+                        if (twin >= 0) st_nt(F + (size_t)v * a.n_slots + twin, r);
+                    }
+                    const double gf = (double)(act ? r.g : (S)0), hf = (double)(act ? r.h : (S)0);
+                    row_add_w(rs[s], cload(a.rw, v), gf, hf);
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < kSlotsA; ++s) {
+                if (tg[s]) {  // tangent internal faces: no flux; each side in the block adds half
+                    const bool own_in = fmeta[s] & kSlotOwnIn, nei_in = fmeta[s] & kSlotNeiIn;
+                    for (int vv = 0; vv < ck.nvv; ++vv) {
+                        const int v = ck.vrow + ck.ix0 + vv;
+                        const double xix = cload((const double *)g.xi, 3 * v);
+                        const double xs = sigma_row(xix, fS[s].x, spy[s], spz[s]);
+                        if (!(xs >= tT[s] && xs < kVSmall)) continue;
+                        const P2<S> *pq = gr + vv * kNTB * GE;
+                        double gf, hf;
                         if (own_in && nei_in) {
                             const P2<S> rn = recon_s<S, D>(pq, fln[s], aN[s]);
                             const P2<S> ro = recon_s<S, D>(pq, flo[s], aO[s]);
                             gf = 0.5 * ((double)rn.g + (double)ro.g);
                             hf = 0.5 * ((double)rn.h + (double)ro.h);
-                        } else if (own_in) {
-                            const P2<S> r = recon_s<S, D>(pq, flo[s], aO[s]);
-                            gf = 0.5 * (double)r.g;
-                            hf = 0.5 * (double)r.h;
                         } else {
-                            const P2<S> r = recon_s<S, D>(pq, fln[s], aN[s]);
-                            gf = 0.5 * (double)r.g;
-                            hf = 0.5 * (double)r.h;
+                            const P2<S> rt = own_in ? recon_s<S, D>(pq, flo[s], aO[s]) : recon_s<S, D>(pq, fln[s], aN[s]);
+                            gf = 0.5 * (double)rt.g;
+                            hf = 0.5 * (double)rt.h;
                         }
+                        row_add_w(rs[s], cload(a.rw, v), gf, hf);
                     }
-                    if (store) {
-                        const P2<S> val{(S)gf, (S)hf};
-                        st_nt(F + (size_t)v * a.n_slots + (bi.fs_off + slot_of(s)), val);
-                        const int twin = (fmeta[s] >> 2) - 1;
-                        if (twin >= 0) st_nt(F + (size_t)v * a.n_slots + twin, val);
-                    }
-                    if (ftyp[s] == kMaxwellWall) acc[s][NMA - 1] += w * xs * gf;
-                    row_add(rs[s], w * gf, w * hf, xix);
                 }
+                // walls: sum w sigma g = S_x sum w xi_x g + (xi_y S_y + xi_z S_z) sum w g
+                acc[s][NMA - 1] += wf[s] * (fS[s].x * rs[s].g1 + (spy[s] + spz[s]) * rs[s].g0);
             }
 #pragma unroll
             for (int s = 0; s < kSlotsA; ++s)
@@ -1721,7 +1776,8 @@ template <class S>
 size_t lds_pass_a(const FusedPlan &p, int D, int VC, bool barin = false)
 {
     const size_t cex = (D == 3 && !barin) ? (size_t)VC * p.max_nl * sizeof(double) : 0;
-    return lds_tiles_a<S>(p, D, VC) + cex;
+    const size_t gsl = barin ? (size_t)kNb * kNTB * 16 : 0;  // the gradient stencils' table
+    return lds_tiles_a<S>(p, D, VC) + cex + gsl;
 }
 size_t lds_pass_b(const FusedPlan &p, int VC) { return (size_t)VC * 2 * p.max_nfs * sizeof(double); }
 // k_bar3: the waves' store patches
