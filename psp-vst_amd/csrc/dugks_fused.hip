@@ -1559,6 +1559,22 @@ __global__ void __launch_bounds__(kNTB3)
         }
     }
 
+    // the flux cases as compares of sigma = xi . S_f: relaxed face value where sigma > tP or
+    // sigma < tM (|sigma| >= VSMALL on internal faces, sigma > 0 on boundary faces), the
+    // incoming half where sigma <= tI (boundary faces), zero otherwise (tangent internal faces)
+    double tP[kSlotsB3], tM[kSlotsB3], tI[kSlotsB3];
+    {
+        const double inf = __builtin_huge_val();
+        const double vsmall_below = __longlong_as_double(__double_as_longlong(kVSmall) - 1);  // x > it: x >= VSMALL
+#pragma unroll
+        for (int s = 0; s < kSlotsB3; ++s) {
+            const bool bnd = sfc[s] >= g.nIF;
+            tP[s] = bnd ? 0.0 : vsmall_below;
+            tM[s] = bnd ? -inf : -kVSmall;
+            tI[s] = bnd ? 0.0 : -inf;
+        }
+    }
+
     // next chunk's stored face values of the thread's slots and u of its cell
     P2<S> fn[kSlotsB3][VC];
     P2<S> un[HV];
@@ -1595,7 +1611,9 @@ __global__ void __launch_bounds__(kNTB3)
         if (ch + 1 < ch1) load_next(ch + 1);
         P2<double> *fl = fl2 + ((ch - ch0) & 1) * VC * FS;
         // the thread's face slots -> flux sigma * f_f in LDS, each face Maxwellian the slot's
-        // stripe x-factor times one row factor per chunk
+        // stripe x-factor times one row factor per chunk.  Every velocity runs the relaxation
+        // and keeps it where sigma > tP or sigma < tM (no branch); the incoming halves of
+        // boundary faces (sigma <= tI) are written after the slot's loop
 #pragma unroll
         for (int s = 0; s < kSlotsB3; ++s) {
             if (sfc[s] < 0) continue;
@@ -1605,13 +1623,33 @@ __global__ void __launch_bounds__(kNTB3)
             double spy, spz;
             row_products(xr, sS[s], spy, spz);
             const RowEq re = row_eq(sk[s], xr);
+            const EqCoef &k = sk[s];
+            bool inc = false;
 #pragma unroll
             for (int vv = 0; vv < VC; ++vv) {
                 if (vv >= ck.nvv) break;
                 const VelK<S> vk = vel_k<S>(g, ck.vrow + ck.ix0 + vv);
-                fl[vv * FS + j] =
-                    slot_flux_m<S>(g, a, fv[s][vv], sfc[s], vk, sS[s], spy, spz, sk[s], re,
-                                   exs[vv * a.max_nfs + j] * eyz, rho_b);
+                const double xs = sigma_row(vk.xi.x, sS[s].x, spy, spz);
+                const bool rel = xs > tP[s] || xs < tM[s];
+                inc = inc || xs <= tI[s];
+                double gEq, hEq;
+                shakhov_row(g, k, re, vk.xi.x, exs[vv * a.max_nfs + j] * eyz, gEq, hEq);
+                const double fg = xs * ((1.0 - k.tr) * ((double)fv[s][vv].g + vk.rg) + k.tr * gEq);
+                const double fh = xs * ((1.0 - k.tr) * ((double)fv[s][vv].h + vk.rh) + k.tr * hEq);
+                fl[vv * FS + j] = P2<double>{rel ? fg : 0.0, rel ? fh : 0.0};
+            }
+            if (inc) {  // incoming half of a wall or inlet face
+                const int b = sfc[s] - g.nIF;
+                const BndCoef kb = a.bc[b];
+                const double scale = g.bf_type[b] == kMaxwellWall ? rho_b[b] : 1.0;
+                for (int vv = 0; vv < ck.nvv; ++vv) {
+                    const d3 xi = cload(g.xi, ck.vrow + ck.ix0 + vv);
+                    const double xs = sigma_row(xi.x, sS[s].x, spy, spz);
+                    if (!(xs <= tI[s])) continue;
+                    const double cx = xi.x - kb.ux, cy = xi.y - kb.uy, cz = xi.z - kb.uz;
+                    const double gv = scale * (kb.A * exp_cw(-0.5 * ((cx * cx + cy * cy + cz * cz) * kb.invRT)));
+                    fl[vv * FS + j] = P2<double>{xs * gv, xs * (gv * kb.hfac)};
+                }
             }
         }
         __syncthreads();
