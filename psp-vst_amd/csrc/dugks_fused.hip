@@ -650,6 +650,66 @@ __device__ inline void stage_grad(const Geo &g, const FArgs &a, const BlockInfo 
     }
 }
 
+// Gradient stage of the 3-D FP32 pass A that gathers f-bar-plus: the chunk's f-bar-plus as
+// (g, h) of velocity pairs, fq[p][nlp] = {g(2p), h(2p), g(2p+1), h(2p+1)}, one 16-byte gather
+// per neighbour and pair; each velocity's gradient and f* take the operations of stage_grad
+// in the same order.  The two threads of a cell take alternate pairs.
+template <int VC, int NT>
+__device__ inline void stage_grad_q(const Geo &g, const BlockInfo &bi, const Chunk &ck, double hdt, int2 gk,
+                                    const float4 *fq, int nlp, P2<float> *gr, const float4 *gsl)
+{
+    constexpr int GE = grad_entries<float, 3>();
+    const int l = threadIdx.x % (NT / 2);
+    // the same for every lane of a wave: the velocity tables are read with scalar loads
+    const int h = __builtin_amdgcn_readfirstlane(threadIdx.x / (NT / 2));
+    if (l >= bi.nc) return;
+    const int n = gk.y;
+    int nb[kNb];
+    V3<float> L[kNb];
+#pragma unroll
+    for (int j = 0; j < kNb; ++j) {
+        if (j < n) {
+            const float4 e = gsl[j * kNTB + l];
+            nb[j] = __float_as_int(e.w);
+            L[j] = V3<float>{e.x, e.y, e.z};
+        }
+    }
+    auto put = [&](int vv, float cg, float ch, float gx, float gy, float gz, float hx, float hy, float hz) {
+        P2<float> *o = gr + vv * GE * kNTB + l;
+        o[0] = P2<float>{gx, hx};
+        o[kNTB] = P2<float>{gy, hy};
+        o[2 * kNTB] = P2<float>{gz, hz};
+        const d3 xi = cload(g.xi, ck.vrow + ck.ix0 + vv);
+        const float sx = (float)(xi.x * hdt), sy = (float)(xi.y * hdt), sz = (float)(xi.z * hdt);
+        float dg = sx * gx + sy * gy, dh = sx * hx + sy * hy;
+        dg += sz * gz;
+        dh += sz * hz;
+        o[3 * kNTB] = P2<float>{cg - dg, ch - dh};
+    };
+#pragma unroll 1
+    for (int p = h; 2 * p < ck.nvv; p += 2) {
+        const float4 *q = fq + p * nlp;
+        const float4 c = q[l];
+        float ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;  // velocity 2p: grad g, grad h
+        float cx = 0, cy = 0, cz = 0, dx = 0, dy = 0, dz = 0;  // velocity 2p + 1
+#pragma unroll
+        for (int j = 0; j < kNb; ++j) {
+            if (j < n) {
+                const float4 e = q[nb[j]];
+                const float dg0 = e.x - c.x, dh0 = e.y - c.y, dg1 = e.z - c.z, dh1 = e.w - c.w;
+                ax += L[j].x * dg0; ay += L[j].y * dg0;
+                bx += L[j].x * dh0; by += L[j].y * dh0;
+                az += L[j].z * dg0; bz += L[j].z * dh0;
+                cx += L[j].x * dg1; cy += L[j].y * dg1;
+                dx += L[j].x * dh1; dy += L[j].y * dh1;
+                cz += L[j].z * dg1; dz += L[j].z * dh1;
+            }
+        }
+        put(2 * p, c.x, c.y, ax, ay, az, bx, by, bz);
+        if (2 * p + 1 < ck.nvv) put(2 * p + 1, c.z, c.w, cx, cy, cz, dx, dy, dz);
+    }
+}
+
 // g and h of local cell l at the face with arm a = x_f - x_cell: f* + grad f . a, from the
 // gradient tile with f* entries (FP32 storage)
 template <class S, int D>
@@ -701,8 +761,12 @@ __global__ void __launch_bounds__(kNTA)
     // BARIN (FP32 storage): the gradient stencils' LDS table first, then the tiles
     float4 *gsl = (float4 *)smem;      // [kNb][kNTB]
     P2<S> *fb = (P2<S> *)(smem + (BARIN ? kNb * kNTB * sizeof(float4) : 0));  // [VC][max_nl]
+    // BARIN: f-bar-plus of the local cells as velocity pairs instead, fq[VC / 2][nlp] (the local
+    // cells padded to whole waves), filled by LDS DMA
+    const int nlp = (a.max_nl + 63) & ~63;
+    float4 *fq = (float4 *)fb;
     constexpr int GE = grad_entries<S, D>();
-    P2<S> *gr = fb + VC * a.max_nl;    // [VC][GE][kNTB]
+    P2<S> *gr = fb + VC * (BARIN ? nlp : a.max_nl);    // [VC][GE][kNTB]
     // 3-D: the local cells' x-factors in LDS, [VC][max_nl] (registers are short there; the
     // 512-thread work-group is alone on its CU, so the LDS is free)
     constexpr bool kCexLds = D == 3;
@@ -798,10 +862,27 @@ __global__ void __launch_bounds__(kNTA)
         for (int s = 0; s < CPT; ++s)
             if (threadIdx.x + s * kNTA < bi.nl) kc[s] = a.cc[lc[s]];
     }
-    // BARIN: the local cells' f-bar-plus of the next chunk, loaded before the chunk's last
-    // barrier so the gathers' latency overlaps it (211k-cell sphere pass A 93 -> 65 ms with the
-    // two-velocity face loop)
-    P2<S> pfb[CPT][VC];
+    // BARIN: chunk ch of f-bar-plus of the local cells into fq by LDS DMA, issued after the
+    // gradient stage of the chunk before, so the gathers run while its faces are formed and
+    // need no registers (the first barrier of the next chunk waits for them); 16 bytes, one
+    // velocity pair of one cell, per lane: a wave fills 64 consecutive local cells of a pair,
+    // lanes past the last cell read cell 0's run into the padding
+    auto fill_fq = [&](int ch) {
+        if constexpr (BARIN) {
+#pragma unroll
+            for (int s = 0; s < CPT; ++s) {
+                const int l0 = (int)(threadIdx.x & ~63u) + s * kNTA;
+                if (l0 >= bi.nl) break;
+                const P2<S> *src = Fb + ((size_t)ch * g.nC + lc[s]) * VC;
+#pragma unroll
+                for (int p = 0; p < VC / 2; ++p)
+                    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(src + 2 * p),
+                                                     (__attribute__((address_space(3))) void *)(fq + p * nlp + l0),
+                                                     16, 0, 0);
+            }
+        }
+    };
+    if (BARIN && ch0 < ch1) fill_fq(ch0);
     int stripe = -1;
     if (kAhead && ch0 < ch1) {
         int st = stripe;
@@ -812,34 +893,16 @@ __global__ void __launch_bounds__(kNTA)
     }
     for (int ch = ch0; ch < ch1; ++ch) {
         const Chunk ck = chunk_of(a, ch, VC, stripe);
-        if constexpr (BARIN) {
-            if (ch == ch0) {
-#pragma unroll
-                for (int s = 0; s < CPT; ++s) {
-                    const int l = threadIdx.x + s * kNTA;
-                    if (l >= bi.nl) break;
-#pragma unroll
-                    for (int vv = 0; vv < VC; ++vv)
-                        pfb[s][vv] = Fb[((size_t)ch * g.nC + lc[s]) * VC + vv];
-                }
-            }
-#pragma unroll
-            for (int s = 0; s < CPT; ++s) {
-                const int l = threadIdx.x + s * kNTA;
-                if (l >= bi.nl) break;
-#pragma unroll
-                for (int vv = 0; vv < VC; ++vv) {
-                    if (vv >= ck.nvv) break;
-                    fb[vv * a.max_nl + l] = pfb[s][vv];
-                }
-            }
-        } else {
+        if constexpr (!BARIN)
             stage_bar<S, VC, kNTA, CPT, D == 3, kAhead, kHoldK, kCexLds>(g, a, bi, gT, hT, dt, ck, lc, gt, ht, kc,
                                                                          cex, cexL, fb);
-        }
         __syncthreads();
-        stage_grad<S, D, VC, kNTA, BARIN>(g, a, bi, ck, hx, gk, fb, gr, gsl);
+        if constexpr (BARIN)
+            stage_grad_q<VC, kNTA>(g, bi, ck, hx, gk, fq, nlp, (P2<float> *)gr, gsl);
+        else
+            stage_grad<S, D, VC, kNTA>(g, a, bi, ck, hx, gk, fb, gr);
         __syncthreads();
+        if (ch + 1 < ch1) fill_fq(ch + 1);
         const d3 xr = cload(g.xi, ck.vrow);
         if constexpr (BARIN) {
             // (FP32 storage, f-bar-plus gathered) velocities outer, the thread's slots inner: the
@@ -1024,15 +1087,6 @@ __global__ void __launch_bounds__(kNTA)
             }
             fold_row<D>(acc[s], rs, xr.y, xr.z);
         }
-        if (BARIN && ch + 1 < ch1) {
-#pragma unroll
-            for (int s = 0; s < CPT; ++s) {
-                if (threadIdx.x + s * kNTA >= bi.nl) break;
-#pragma unroll
-                for (int vv = 0; vv < VC; ++vv)
-                    pfb[s][vv] = Fb[((size_t)(ch + 1) * g.nC + lc[s]) * VC + vv];
-            }
-        }
         if (kAhead && ch + 1 < ch1) {
             int st = stripe;
             const Chunk nx = chunk_of(a, ch + 1, VC, st);
@@ -1040,7 +1094,8 @@ __global__ void __launch_bounds__(kNTA)
             for (int s = 0; s < CPT; ++s)
                 if (threadIdx.x + s * kNTA < bi.nl) load_cell<S, VC>(g, lc[s], gT, hT, nx, gt[s], ht[s]);
         }
-        __syncthreads();
+        // BARIN: the next chunk's first barrier ends the chunk (no stage 1 writes before it)
+        if constexpr (!BARIN) __syncthreads();
     }
     if (items) {  // uniform over the work-group: every lane takes part in the shuffles
 #pragma unroll
@@ -1813,9 +1868,13 @@ size_t lds_tiles_a(const FusedPlan &p, int D, int VC)
 template <class S>
 size_t lds_pass_a(const FusedPlan &p, int D, int VC, bool barin = false)
 {
-    const size_t cex = (D == 3 && !barin) ? (size_t)VC * p.max_nl * sizeof(double) : 0;
-    const size_t gsl = barin ? (size_t)kNb * kNTB * 16 : 0;  // the gradient stencils' table
-    return lds_tiles_a<S>(p, D, VC) + cex + gsl;
+    if (barin) {  // the gradient stencils' table, f-bar-plus pairs (local cells padded to whole waves), gradient tile
+        const size_t nlp = (size_t)((p.max_nl + 63) & ~63);
+        return (size_t)kNb * kNTB * 16 + (size_t)VC * nlp * 2 * sizeof(S) +
+               (size_t)VC * kFusedThreadsB * (D + 1) * 2 * sizeof(S);
+    }
+    const size_t cex = D == 3 ? (size_t)VC * p.max_nl * sizeof(double) : 0;
+    return lds_tiles_a<S>(p, D, VC) + cex;
 }
 size_t lds_pass_b(const FusedPlan &p, int VC) { return (size_t)VC * 2 * p.max_nfs * sizeof(double); }
 // k_bar3: the waves' store patches
