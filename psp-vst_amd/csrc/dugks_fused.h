@@ -90,12 +90,12 @@ struct FusedPlan {
     size_t lds_a = 0, lds_b = 0;       // dynamic LDS bytes of the two passes
     // host tables
     std::vector<BlockInfo> blk;
-    std::vector<int> lcell, gs_beg, gs_nb, fs_face, fs_twin, fs_lo, fs_ln, fbc_beg, fbc_idx;
+    std::vector<int> lcell, gs_beg, gs_nb, fs_face, fs_col, fs_lo, fs_ln, fbc_beg, fbc_idx;
     std::vector<signed char> fs_flag, fbc_sgn;
     std::vector<d3> gs_ls;
     // device tables
     BlockInfo *d_blk = nullptr;
-    int *d_lcell = nullptr, *d_gs_beg = nullptr, *d_gs_nb = nullptr, *d_fs_face = nullptr, *d_fs_twin = nullptr,
+    int *d_lcell = nullptr, *d_gs_beg = nullptr, *d_gs_nb = nullptr, *d_fs_face = nullptr, *d_fs_col = nullptr,
         *d_fs_lo = nullptr, *d_fs_ln = nullptr, *d_fbc_beg = nullptr, *d_fbc_idx = nullptr;
     signed char *d_fs_flag = nullptr, *d_fbc_sgn = nullptr;
     void *d_gs_ls = nullptr;           // d3 (FP64 path) or float3-as-3-floats (mixed)
@@ -106,14 +106,14 @@ struct FusedPlan {
     double *cref = nullptr;            // [nma] all-velocity moments of the storage reference
     EqCoef *cc = nullptr, *fc = nullptr;  // equilibrium coefficients of cells / faces
     BndCoef *bc = nullptr;                // incoming-half coefficients of boundary faces
-    // face values [Nv][slots], (g, h) pairs: written by the block that reconstructs them into
-    // its own slot
+    // face values [Nv][n_cols], (g, h) pairs, one column per face: written by the block holding
+    // the upwind cell, read by the passes B of both blocks of a shared face
     void *F = nullptr;
     // 3-D with FP32 storage: f-bar-plus (Fb) and u = 4/3 f-bar-plus - 1/3 g~ (U) of every
     // cell, [Nv][nC] (g, h) pairs, written by bar_pass; pass A reads Fb, pass B reads U instead
     // of g~ (null on the other paths)
     void *U = nullptr, *Fb = nullptr;
-    int n_slots = 0;
+    int n_slots = 0, n_cols = 0;         // face slots; face-value columns (one per face with a slot)
     d3 *fs_sf = nullptr;                  // [slots] face area vectors, formed at upload
     EqCoef *fs_coef = nullptr;            // [slots] face records, copied each step
     void *rw = nullptr;                   // [Nv] power-sum weights w xi_x^k, k = 0..3

@@ -98,7 +98,7 @@ struct alignas(16) RowW {
 
 struct FArgs {
     const BlockInfo *blk;
-    const int *lcell, *gs_beg, *gs_nb, *fs_face, *fs_twin, *fs_lo, *fs_ln, *fbc_beg, *fbc_idx;
+    const int *lcell, *gs_beg, *gs_nb, *fs_face, *fs_col, *fs_lo, *fs_ln, *fbc_beg, *fbc_idx;
     d3 *fs_sf;
     EqCoef *fs_coef;
     const signed char *fs_flag, *fbc_sgn;
@@ -108,7 +108,7 @@ struct FArgs {
     const RowW *rw;  // [Nv] power-sum weights
     EqCoef *cc, *fc;
     const BndCoef *bc;
-    int nblk, b8, nvg, n1, nrow, nchunk, cpg, n_slots;
+    int nblk, b8, nvg, n1, nrow, nchunk, cpg, n_slots, n_cols;
     int max_nl, max_nfs, max_nc, nm, nma;
 };
 
@@ -116,9 +116,10 @@ FArgs fargs(const FusedPlan &p)
 {
     FArgs a;
     a.blk = p.d_blk; a.lcell = p.d_lcell; a.gs_beg = p.d_gs_beg; a.gs_nb = p.d_gs_nb;
-    a.fs_face = p.d_fs_face; a.fs_twin = p.d_fs_twin; a.fs_lo = p.d_fs_lo; a.fs_ln = p.d_fs_ln;
+    a.fs_face = p.d_fs_face; a.fs_col = p.d_fs_col; a.fs_lo = p.d_fs_lo; a.fs_ln = p.d_fs_ln;
     a.fs_sf = p.fs_sf; a.fs_coef = p.fs_coef;
     a.n_slots = p.n_slots;
+    a.n_cols = p.n_cols;
     a.fbc_beg = p.d_fbc_beg; a.fbc_idx = p.d_fbc_idx; a.fs_flag = p.d_fs_flag; a.fbc_sgn = p.d_fbc_sgn;
     a.gs_ls = p.d_gs_ls;
     a.fpart = p.fpart; a.cpart = p.cpart; a.bin = p.bin; a.bref = p.bref; a.cref = p.cref; a.cc = p.cc; a.fc = p.fc; a.bc = p.bc;
@@ -784,9 +785,8 @@ __global__ void __launch_bounds__(kNTA)
     const int vo = threadIdx.x % VC;
     auto slot_of = [&](int s) { return (s == 1 && items) ? kNTA + (int)threadIdx.x / VC : (int)threadIdx.x + s * kNTA; };
     double acc[kSlotsA][NMA];
-    // fmeta: (twin slot + 1) << 2 | slot flags; the twin is the adjacent block's slot of a
-    // shared face (-1: none)
-    int fid[kSlotsA], fmeta[kSlotsA], flo[kSlotsA], fln[kSlotsA], ftyp[kSlotsA];
+    // fmeta: slot flags; fcol: the face's column of the face-value array
+    int fid[kSlotsA], fmeta[kSlotsA], fcol[kSlotsA], flo[kSlotsA], fln[kSlotsA], ftyp[kSlotsA];
     d3 fS[kSlotsA], armO[kSlotsA], armN[kSlotsA];
     V3<S> aO[kSlotsA], aN[kSlotsA];  // the arms in the storage type (FP32 storage: recon_s)
 #pragma unroll
@@ -795,14 +795,15 @@ __global__ void __launch_bounds__(kNTA)
         for (int k = 0; k < NMA; ++k) acc[s][k] = 0.0;
         const int j = slot_of(s);
         fid[s] = -1;
-        flo[s] = fln[s] = fmeta[s] = ftyp[s] = 0;
+        flo[s] = fln[s] = fmeta[s] = fcol[s] = ftyp[s] = 0;
         fS[s] = armO[s] = armN[s] = d3{0, 0, 0};
         if (j < bi.nfs) {
             const int f = a.fs_face[bi.fs_off + j];
             fid[s] = f;
             flo[s] = a.fs_lo[bi.fs_off + j];
             fln[s] = a.fs_ln[bi.fs_off + j];
-            fmeta[s] = ((a.fs_twin[bi.fs_off + j] + 1) << 2) | a.fs_flag[bi.fs_off + j];
+            fmeta[s] = a.fs_flag[bi.fs_off + j];
+            fcol[s] = a.fs_col[bi.fs_off + j];
             ftyp[s] = f < g.nIF ? 0 : g.bf_type[f - g.nIF];
             fS[s] = g.Sf[f];
             const d3 Cf = g.Cf[f];
@@ -936,11 +937,7 @@ __global__ void __launch_bounds__(kNTA)
                     const bool act = po || pn;
                     tg[s] = tg[s] || (xs >= tT[s] && xs < kVSmall);
                     const P2<S> r = recon_s<S, D>(pq, po ? flO[s] : flN[s], po ? aO[s] : aN[s]);
-                    if (act) {
-                        st_nt(F + (size_t)v * a.n_slots + (bi.fs_off + slot_of(s)), r);
-                        const int twin = (fmeta[s] >> 2) - 1;
-                        if (twin >= 0) st_nt(F + (size_t)v * a.n_slots + twin, r);
-                    }
+                    if (act) st_nt(F + (size_t)v * a.n_cols + fcol[s], r);
                     const double gf = (double)(act ? r.g : (S)0), hf = (double)(act ? r.h : (S)0);
                     row_add_w(rs[s], cload(a.rw, v), gf, hf);
                 }
@@ -1072,14 +1069,8 @@ __global__ void __launch_bounds__(kNTA)
                     gf = (double)r.g;
                     hf = (double)r.h;
                 }
-                if (store) {  // face values are stored as deviations
-                    // into this block's slot and, for a face shared with an adjacent block,
-                    // into that block's slot too: pass B then reads its own slots only
-                    const P2<S> val{(S)gf, (S)hf};
-                    st_nt(F + (size_t)v * a.n_slots + (bi.fs_off + slot_of(s)), val);  // own slot
-                    const int twin = (fmeta[s] >> 2) - 1;
-                    if (twin >= 0) st_nt(F + (size_t)v * a.n_slots + twin, val);
-                }
+                if (store)  // face values are stored as deviations, one column per face
+                    st_nt(F + (size_t)v * a.n_cols + fcol[s], P2<S>{(S)gf, (S)hf});
                 if (ftyp[s] == kMaxwellWall) acc[s][NMA - 1] += w * xs * gf;
                 // (the weight table of pass B measured slower here: its scalar load sits on
                 // the loop's path)
@@ -1452,8 +1443,8 @@ __global__ void __launch_bounds__(kNTB)
         // issued together before any is used.
         // Per-slot copies are loads that do not wait on the face label: the face records
         // always, the area vectors except in 3-D with FP64 storage (there they cost the
-        // registers of the second wave per SIMD).  Pass A wrote the upwind value into the
-        // slots of both blocks of a shared face.
+        // registers of the second wave per SIMD).  A shared face's value is in the face's one
+        // column, written by whichever block held the upwind cell.
 #pragma unroll
         for (int s = 0; s < FW; ++s) {
             const int j = threadIdx.x + s * kNTB;
@@ -1462,7 +1453,7 @@ __global__ void __launch_bounds__(kNTB)
             P2<S> fv[VC];
 #pragma unroll
             for (int vv = 0; vv < VC; ++vv)
-                fv[vv] = ld_nt(F + (size_t)(ck.vrow + ck.ix0 + min(vv, ck.nvv - 1)) * a.n_slots + own_slot);
+                fv[vv] = ld_nt(F + (size_t)(ck.vrow + ck.ix0 + min(vv, ck.nvv - 1)) * a.n_cols + a.fs_col[own_slot]);
             const int f = a.fs_face[own_slot];
             const d3 Sf = (D == 2 || sizeof(S) == 4) ? a.fs_sf[own_slot] : g.Sf[f];
             const EqCoef k = a.fs_coef[own_slot];
@@ -1480,7 +1471,7 @@ __global__ void __launch_bounds__(kNTB)
                 const int vv = i - q * ck.nvv;
                 const int j = FW * kNTB + q;
                 const int own_slot = bi.fs_off + j;
-                const P2<S> fv = ld_nt(F + (size_t)(ck.vrow + ck.ix0 + vv) * a.n_slots + own_slot);
+                const P2<S> fv = ld_nt(F + (size_t)(ck.vrow + ck.ix0 + vv) * a.n_cols + a.fs_col[own_slot]);
                 const int f = a.fs_face[own_slot];
                 const d3 Sf = (D == 2 || sizeof(S) == 4) ? a.fs_sf[own_slot] : g.Sf[f];
                 const EqCoef k = a.fs_coef[own_slot];
@@ -1599,18 +1590,20 @@ __global__ void __launch_bounds__(kNTB3)
             }
         }
     }
-    int sfc[kSlotsB3];
+    int sfc[kSlotsB3], scol[kSlotsB3];  // the slots' faces and face-value columns
     d3 sS[kSlotsB3];
     EqCoef sk[kSlotsB3];
 #pragma unroll
     for (int s = 0; s < kSlotsB3; ++s) {
         const int j = threadIdx.x + s * kNTB3;
         sfc[s] = -1;
+        scol[s] = 0;
         sS[s] = d3{0, 0, 0};
         sk[s] = EqCoef{};
         if (j < bi.nfs) {
             const int own_slot = bi.fs_off + j;
             sfc[s] = a.fs_face[own_slot];
+            scol[s] = a.fs_col[own_slot];
             sS[s] = a.fs_sf[own_slot];
             sk[s] = a.fs_coef[own_slot];
         }
@@ -1646,10 +1639,9 @@ __global__ void __launch_bounds__(kNTB3)
 #pragma unroll
         for (int s = 0; s < kSlotsB3; ++s) {
             if (sfc[s] < 0) continue;
-            const int own_slot = bi.fs_off + threadIdx.x + s * kNTB3;
 #pragma unroll
             for (int vv = 0; vv < VC; ++vv)
-                fn[s][vv] = ld_nt(F + (size_t)(c.vrow + c.ix0 + min(vv, c.nvv - 1)) * a.n_slots + own_slot);
+                fn[s][vv] = ld_nt(F + (size_t)(c.vrow + c.ix0 + min(vv, c.nvv - 1)) * a.n_cols + scol[s]);
         }
     };
     if (ch0 < ch1) load_next(ch0);
@@ -2069,7 +2061,7 @@ bool fused_plan_upload(FusedPlan &p, const Geo &g, size_t es, std::string &err)
     fused_velocity_groups(p, g.D, g.n1d, vc);
     // the face values, then the k_bar3 form's two more [Nv][nC] pair arrays: without the
     // memory for those the two-pass form runs (it needs 2 Nv nC es bytes less than that)
-    if (!dbuf(p, &p.F, (size_t)g.Nv * p.n_slots * 2 * es, err)) return false;
+    if (!dbuf(p, &p.F, (size_t)g.Nv * p.n_cols * 2 * es, err)) return false;
     p.U = p.Fb = nullptr;
     if (fbar) {
         void *u = nullptr, *fb = nullptr;
@@ -2097,7 +2089,7 @@ bool fused_plan_upload(FusedPlan &p, const Geo &g, size_t es, std::string &err)
     }
     if (!up(p, &p.d_blk, p.blk, err) || !up(p, &p.d_lcell, p.lcell, err) || !up(p, &p.d_gs_beg, p.gs_beg, err) ||
         !up(p, &p.d_gs_nb, p.gs_nb, err) || !up(p, &p.d_fs_face, p.fs_face, err) ||
-        !up(p, &p.d_fs_twin, p.fs_twin, err) || !up(p, &p.d_fs_lo, p.fs_lo, err) || !up(p, &p.d_fs_ln, p.fs_ln, err) ||
+        !up(p, &p.d_fs_col, p.fs_col, err) || !up(p, &p.d_fs_lo, p.fs_lo, err) || !up(p, &p.d_fs_ln, p.fs_ln, err) ||
         !up(p, &p.d_fs_flag, p.fs_flag, err) || !up(p, &p.d_fbc_beg, p.fbc_beg, err) ||
         !up(p, &p.d_fbc_idx, p.fbc_idx, err) || !up(p, &p.d_fbc_sgn, p.fbc_sgn, err))
         return false;

@@ -1,7 +1,7 @@
 // Host-side planning of the fused path: cell blocks by recursive coordinate bisection, the
 // halo ring of each block, and the block-local tables (gradient stencils of the block
-// cells, the faces of the block cells with their gather order, the twin slot of every face
-// two blocks share) that the kernels follow.
+// cells, the faces of the block cells with their gather order, the face-value column of
+// every face slot, one per face even where two blocks share the face) that the kernels follow.
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -162,8 +162,10 @@ bool fused_plan_build(FusedPlan &p, const Geo &g, const HostPrep &prep, const st
     std::vector<int> block_of(nC);
     for (int b = 0; b < nblk; ++b)
         for (int c = block_start[b]; c < block_start[b + 1]; ++c) block_of[c] = b;
-    // a face shared by two blocks has a slot in each; twin links them
+    // a face shared by two blocks has a slot in each; both slots name the face's one column
+    // of the face-value array (the column of the face's first slot, numbered in slot order)
     std::vector<int> first_slot(nF, -1);
+    int ncol = 0;
 
     p.nblk = nblk;
     p.blk.resize(nblk);
@@ -212,8 +214,8 @@ bool fused_plan_build(FusedPlan &p, const Geo &g, const HostPrep &prep, const st
         // faces of the block cells, one slot each: faces inside the block first, then the
         // faces shared with each adjacent block (grouped by that block, ascending face
         // label), then boundary faces.  Both blocks of a shared face order their group the
-        // same way, so the twin slots of a run of lanes are a contiguous run in the other
-        // block and pass B's reads of the other block's values stay coalesced.
+        // same way, so the columns of a run of lanes' shared faces are a contiguous run
+        // whichever block numbered them, and the passes' stores and loads stay coalesced.
         bi.fs_off = (int)p.fs_face.size();
         std::unordered_map<int, int> slot_of;
         std::vector<std::array<int, 3>> keyed;  // (group, adjacent block, face)
@@ -241,12 +243,11 @@ bool fused_plan_build(FusedPlan &p, const Geo &g, const HostPrep &prep, const st
             if (n >= 0 && block_of[n] == b) flag |= kSlotNeiIn;
             const int gslot = (int)p.fs_face.size();
             p.fs_face.push_back(f);
-            p.fs_twin.push_back(-1);
             if (first_slot[f] < 0) {
                 first_slot[f] = gslot;
+                p.fs_col.push_back(ncol++);
             } else {
-                p.fs_twin[gslot] = first_slot[f];
-                p.fs_twin[first_slot[f]] = gslot;
+                p.fs_col.push_back(p.fs_col[first_slot[f]]);
             }
             p.fs_lo.push_back(local[o]);
             p.fs_ln.push_back(n >= 0 ? local[n] : -1);
@@ -284,6 +285,7 @@ bool fused_plan_build(FusedPlan &p, const Geo &g, const HostPrep &prep, const st
         return false;
     }
     p.n_slots = (int)p.fs_face.size();
+    p.n_cols = ncol;
     p.nm = g.D == 3 ? 17 : 11;
     p.nma = p.nm + 1;
     // about 6k work-groups per pass: the last round of work-groups on each die then holds a
