@@ -126,7 +126,8 @@ def test_determinism_and_checkpoint_resume(tmp_path, fused):
 @pytest.mark.parametrize("fused", [2, 3])
 def test_fused_and_staged_agree_on_multiblock_mesh(fused):
     """A 48x48 cavity gives nine 16x16 RCB blocks, so block halos, ring-2 cells, faces shared
-    between blocks (twin face slots) and, in the face-value schedule, the 32 face slots
+    between blocks (a slot in each block, one face-value column) and, in the face-value
+    schedule, the 32 face slots
     beyond 512 per block that pass A spreads over velocity lanes (13 velocities per axis:
     a short last stripe) are all exercised; FP64 fused vs staged after 50 steps."""
     from dugks_amd import hip
