@@ -935,10 +935,13 @@ __global__ void __launch_bounds__(kNTA)
                     const double xs = sigma_row(xix, fS[s].x, spy[s], spz[s]);
                     const bool po = xs > tO[s], pn = xs < tN[s];  // owner / neighbour upwind, in the block
                     const bool act = po || pn;
-                    tg[s] = tg[s] || (xs >= tT[s] && xs < kVSmall);
+                    tg[s] = tg[s] | ((xs >= tT[s]) & (xs < kVSmall));  // no short-circuit branch
                     const P2<S> r = recon_s<S, D>(pq, po ? flO[s] : flN[s], po ? aO[s] : aN[s]);
                     if (act) st_nt(F + (size_t)v * a.n_cols + fcol[s], r);
-                    const double gf = (double)(act ? r.g : (S)0), hf = (double)(act ? r.h : (S)0);
+                    // scaled by 0 or 1 rather than selected, so that the reconstruction is not
+                    // moved into the store's branch (inactive lanes read a block cell: finite)
+                    const S am = act ? (S)1 : (S)0;
+                    const double gf = (double)(r.g * am), hf = (double)(r.h * am);
                     row_add_w(rs[s], cload(a.rw, v), gf, hf);
                 }
             }
