@@ -20,9 +20,11 @@
 //
 // 3-D with FP32 storage runs a three-kernel form of the same step:
 //   k_bar3     f-bar-plus and u = 4/3 f-bar-plus - 1/3 g~ of every cell, once (streaming)
-//   pass A     (BARIN) stage 1 gathers f-bar-plus of the block and halo a chunk ahead into
-//              registers; stage 2 also forms f* = f-bar-plus - (dt/2) xi.grad, so stage 3's
-//              face value f* + grad.(x_f - x_cell) has a velocity-independent arm
+//   pass A     (BARIN) no stage 1: the next chunk's f-bar-plus of the block and halo goes
+//              into LDS by DMA during the face loop; stage 2 also forms
+//              f* = f-bar-plus - (dt/2) xi.grad, so stage 3's face value
+//              f* + grad.(x_f - x_cell) has a velocity-independent arm; stage 3 picks the
+//              upwind side by sigma thresholds with no branch but the face-value stores
 //   pass B     (k_pass_b3) 512 threads, records in registers, u instead of stage 1:
 //              g~ <- u - dt/V sum(+-flux)
 // FP64 storage and 2-D keep the two-pass form above.
@@ -743,9 +745,9 @@ __device__ inline P2<S> recon_l(const P2<S> *fb, const P2<S> *gr, int l, double 
 }
 
 // ------------------------------------------------------------------------- pass A
-// BARIN: f-bar-plus of every cell comes from Fb (k_bar3), so stage 1 only gathers the local
-// cells' values, each cell's VC values of a chunk one contiguous run; otherwise stage 1 forms
-// them from g~, h~ and the cell records.
+// BARIN: f-bar-plus of every cell comes from Fb (k_bar3), each cell's VC values of a chunk
+// one contiguous run that LDS DMA moves into the chunk's tile (no stage 1); otherwise stage 1
+// forms them from g~, h~ and the cell records.
 template <class S, int D, int VC, bool BARIN>
 __global__ void __launch_bounds__(kNTA)
     k_pass_a(Geo g, FArgs a, const S *__restrict__ gT, const S *__restrict__ hT, P2<S> *__restrict__ F,
